@@ -1,0 +1,165 @@
+"""Device memory owned by the HIP library, and the array type that
+`return_device=True` hands back.
+
+DeviceArray keeps the CuPy idiom the reference's callers use
+(`.get()`, `.shape`, `.dtype`; tests/test_lib_gpu.py:30-33) without CuPy, and
+exports DLPack (device type kDLROCM) so torch and friends can take the result
+without a copy.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._capi import call, lib
+
+KDL_ROCM = 10
+
+
+class DeviceBuffer:
+    """One bl_malloc allocation, freed with the object."""
+
+    __slots__ = ("ptr", "nbytes", "__weakref__")
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        call("bl_malloc", ctypes.byref(p), self.nbytes)
+        self.ptr = p.value or 0
+
+    def free(self):
+        if self.ptr:
+            p, self.ptr = self.ptr, 0
+            lib().bl_free(p)
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def upload(self, host: np.ndarray, stream=None, offset: int = 0):
+        host = np.ascontiguousarray(host)
+        if host.nbytes + offset > self.nbytes:
+            raise ValueError("upload larger than the buffer")
+        call("bl_memcpy_htod", self.ptr + offset, host.ctypes.data, host.nbytes, stream)
+
+    @classmethod
+    def from_host(cls, host: np.ndarray, stream=None) -> "DeviceBuffer":
+        host = np.ascontiguousarray(host)
+        b = cls(max(host.nbytes, 1))
+        b.upload(host, stream)
+        return b
+
+
+class DeviceArray:
+    """A typed view of device memory. `.get()` copies to a numpy array."""
+
+    def __init__(self, shape, dtype, buffer: DeviceBuffer | None = None, offset: int = 0):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        self._buf = buffer if buffer is not None else DeviceBuffer(nbytes)
+        self._off = int(offset)
+        if self._off + nbytes > self._buf.nbytes:
+            raise ValueError("view exceeds its buffer")
+
+    @classmethod
+    def empty(cls, shape, dtype) -> "DeviceArray":
+        return cls(shape, dtype)
+
+    @property
+    def size(self) -> int:
+        return int(np.prod(self.shape, dtype=np.int64))
+
+    @property
+    def nbytes(self) -> int:
+        return self.size * self.dtype.itemsize
+
+    @property
+    def ndim(self) -> int:
+        return len(self.shape)
+
+    @property
+    def ptr(self) -> int:
+        """Device address of element 0 (0 for an empty array)."""
+        return self._buf.ptr + self._off if self._buf.ptr else 0
+
+    def __len__(self):
+        return self.shape[0] if self.shape else 0
+
+    def reshape(self, *shape) -> "DeviceArray":
+        if len(shape) == 1 and isinstance(shape[0], (tuple, list)):
+            shape = tuple(shape[0])
+        if int(np.prod(shape, dtype=np.int64)) != self.size:
+            raise ValueError(f"cannot reshape {self.shape} into {shape}")
+        return DeviceArray(shape, self.dtype, self._buf, self._off)
+
+    def get(self, stream=None) -> np.ndarray:
+        """Copy to a new numpy array (waits for the copy)."""
+        out = np.empty(self.shape, dtype=self.dtype)
+        if self.nbytes:
+            call("bl_memcpy_dtoh", out.ctypes.data, self.ptr, self.nbytes, stream)
+            call("bl_stream_sync", stream)
+        return out
+
+    def __repr__(self):
+        return f"DeviceArray(shape={self.shape}, dtype={self.dtype})"
+
+    # ------------------------------------------------------------ DLPack export
+    def __dlpack_device__(self):
+        return (KDL_ROCM, 0)
+
+    def __dlpack__(self, stream=None, **_):
+        return _dlpack_capsule(self)
+
+
+# DLPack v0.x structures (dlpack.h), enough to hand a contiguous array over.
+class _DLDevice(ctypes.Structure):
+    _fields_ = [("device_type", ctypes.c_int), ("device_id", ctypes.c_int)]
+
+
+class _DLDataType(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_uint8), ("bits", ctypes.c_uint8), ("lanes", ctypes.c_uint16)]
+
+
+class _DLTensor(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("device", _DLDevice), ("ndim", ctypes.c_int),
+                ("dtype", _DLDataType), ("shape", ctypes.POINTER(ctypes.c_int64)),
+                ("strides", ctypes.POINTER(ctypes.c_int64)), ("byte_offset", ctypes.c_uint64)]
+
+
+class _DLManagedTensor(ctypes.Structure):
+    pass
+
+
+_DELETER = ctypes.CFUNCTYPE(None, ctypes.POINTER(_DLManagedTensor))
+_DLManagedTensor._fields_ = [("dl_tensor", _DLTensor), ("manager_ctx", ctypes.c_void_p),
+                             ("deleter", _DELETER)]
+
+_live = {}          # id -> (managed tensor, shape array, owner) kept until the consumer deletes
+
+
+@_DELETER
+def _dl_deleter(ptr):
+    _live.pop(ctypes.addressof(ptr.contents), None)
+
+
+def _dlpack_capsule(arr: DeviceArray):
+    code = {"i": 0, "u": 1}[arr.dtype.kind]
+    shape = (ctypes.c_int64 * max(arr.ndim, 1))(*arr.shape)
+    mt = _DLManagedTensor()
+    mt.dl_tensor.data = arr.ptr
+    mt.dl_tensor.device = _DLDevice(KDL_ROCM, 0)
+    mt.dl_tensor.ndim = arr.ndim
+    mt.dl_tensor.dtype = _DLDataType(code, arr.dtype.itemsize * 8, 1)
+    mt.dl_tensor.shape = ctypes.cast(shape, ctypes.POINTER(ctypes.c_int64))
+    mt.dl_tensor.strides = None
+    mt.dl_tensor.byte_offset = 0
+    mt.deleter = _dl_deleter
+    _live[ctypes.addressof(mt)] = (mt, shape, arr)
+    new = ctypes.pythonapi.PyCapsule_New
+    new.restype = ctypes.py_object
+    new.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p]
+    return new(ctypes.addressof(mt), b"dltensor", None)

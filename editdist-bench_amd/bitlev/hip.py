@@ -1,0 +1,194 @@
+"""bitlev.hip -- exact edit distance on AMD Instinct MI355X (gfx950).
+
+Drop-in for the reference's `bitlev.gpu` (bitlev/gpu.py:71-138): same
+functions, arguments, results, dtypes and exceptions, with hand-written HIP
+kernels behind the C ABI of libbitlev_hip.so instead of CuPy/NVRTC.
+
+    from bitlev import hip
+    d = hip.cdist(queries, choices, max_dist=2, dtype="uint8")
+    d = hip.pairwise(a, b, return_device=True)        # DeviceArray, .get() -> numpy
+
+Every distance is computed on the GPU, including the strings the reference
+hands to its CPU core (pairs whose shorter string exceeds 256 bytes and cdist
+queries longer than 512 bytes): those run on the wavefront kernel. There is no
+CPU fallback; without the built library or a GPU the calls raise.
+
+This module is the plain (per-class) form: the host splits pairs and queries
+into length classes exactly as the reference does (bitlev/gpu.py:85-86,
+:116-118) and launches one kernel per class.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _edpy
+from ._capi import BackendUnavailable, call, lib
+from ._common import DTYPE_CODE, as_list, check_dtype, check_k
+from .device import DeviceArray, DeviceBuffer
+
+PAIR_CAPS = (16, 32, 64, 128, 256)   # pairs kernels: shorter string <= CAP
+CROSS_NWS = (1, 2, 4, 8)             # cdist kernels: query <= 64 * NW
+
+__all__ = ["available", "distance", "pairwise", "cdist", "Packed", "pairs_into", "cdist_into",
+           "pair_classes", "query_classes", "DeviceArray", "BackendUnavailable"]
+
+
+def available() -> bool:
+    """True when the HIP library loads and a GPU is visible. Never raises
+    (bitlev/gpu.py:36-41)."""
+    try:
+        return lib().bl_device_count() > 0
+    except Exception:
+        return False
+
+
+def _require_device():
+    if lib().bl_device_count() <= 0:
+        raise BackendUnavailable("no HIP device visible")
+
+
+# ------------------------------------------------------------------ host side --
+def pack(strings) -> tuple[np.ndarray, np.ndarray]:
+    """str (chars <= U+00FF) / bytes -> (uint8 buffer, int64 offsets[n + 1])."""
+    buf, off = _edpy.pack(strings)
+    return np.frombuffer(buf, dtype=np.uint8), np.frombuffer(off, dtype=np.int64)
+
+
+def pair_classes(off_a: np.ndarray, off_b: np.ndarray):
+    """Pair indices per pairs kernel, by the shorter length before trimming,
+    and the rest (shorter > 256) for the long kernel (bench/cudamyers.py:23-28)."""
+    m = np.minimum(np.diff(off_a), np.diff(off_b))
+    cls = np.searchsorted(np.array(PAIR_CAPS), m)
+    groups = [np.nonzero(cls == c)[0].astype(np.int32) for c in range(len(PAIR_CAPS))]
+    return groups, np.nonzero(cls == len(PAIR_CAPS))[0].astype(np.int32)
+
+
+def query_classes(off_q: np.ndarray):
+    """Query indices per cdist kernel by words = ceil(len / 64) (an empty query
+    goes to NW 1), and the rest (> 512) for the long kernel (bench/cudamyers.py:31-36)."""
+    words = (np.diff(off_q) + 63) // 64
+    cls = np.searchsorted(np.array(CROSS_NWS), words)
+    groups = [np.nonzero(cls == c)[0].astype(np.int32) for c in range(len(CROSS_NWS))]
+    return groups, np.nonzero(cls == len(CROSS_NWS))[0].astype(np.int32)
+
+
+class Packed:
+    """A packed string set resident on the device (buffer + int64 offsets),
+    with its host offsets kept for the class split. Build once, reuse across
+    calls (the reference's CrossState/PairsState, bench/cudamyers.py:83-151)."""
+
+    def __init__(self, buf: np.ndarray, off: np.ndarray, stream=None):
+        self.h_off = np.ascontiguousarray(off, dtype=np.int64)
+        self.n = len(self.h_off) - 1
+        self.lengths = np.diff(self.h_off)
+        self.max_len = int(self.lengths.max()) if self.n else 0
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        if buf.size == 0:
+            buf = np.zeros(1, np.uint8)
+        self.nbytes = int(buf.nbytes)
+        self.d_buf = DeviceBuffer.from_host(buf, stream)
+        self.d_off = DeviceBuffer.from_host(self.h_off, stream)
+
+    @classmethod
+    def from_strings(cls, strings, stream=None) -> "Packed":
+        return cls(*pack(strings), stream=stream)
+
+
+def _idx_buffers(groups):
+    return [DeviceBuffer.from_host(g) if len(g) else None for g in groups]
+
+
+def pairs_into(A: Packed, B: Packed, out: DeviceArray, k: int = -1, trim: bool = True,
+               stream=None, keep=None):
+    """Launch the plain pairs kernels: out[i] = d(A[i], B[i]) (min(d, k+1) for
+    k >= 0) in out's dtype. Asynchronous on `stream`; the index buffers are
+    appended to `keep` (they must live until the work is done)."""
+    if A.n != B.n:
+        raise ValueError(f"length mismatch: {A.n} vs {B.n}")
+    if A.n == 0:
+        return
+    dt = DTYPE_CODE[out.dtype]
+    groups, rest = pair_classes(A.h_off, B.h_off)
+    bufs = _idx_buffers(groups)
+    for cap, g, b in zip(PAIR_CAPS, groups, bufs):
+        if b is not None:
+            call("bl_pairs_class", cap, A.d_buf.ptr, A.d_off.ptr, B.d_buf.ptr, B.d_off.ptr, b.ptr,
+                 len(g), out.ptr, dt, int(trim), int(k), stream)
+    if len(rest):
+        rb = DeviceBuffer.from_host(rest)
+        bufs.append(rb)
+        mx = int(max(A.lengths[rest].max(), B.lengths[rest].max()))
+        call("bl_long", 1, A.d_buf.ptr, A.d_off.ptr, B.d_buf.ptr, B.d_off.ptr, rb.ptr, len(rest),
+             0, out.ptr, dt, int(trim), int(k), mx, stream)
+    if keep is not None:
+        keep.extend(bufs)
+    return bufs
+
+
+def cdist_into(Q: Packed, C: Packed, out: DeviceArray, k: int = -1, stream=None, keep=None):
+    """Launch the plain cdist kernels into the row-major (Q.n, C.n) `out`."""
+    if Q.n == 0 or C.n == 0:
+        return
+    dt = DTYPE_CODE[out.dtype]
+    groups, rest = query_classes(Q.h_off)
+    bufs = _idx_buffers(groups)
+    for nw, g, b in zip(CROSS_NWS, groups, bufs):
+        if b is not None:
+            call("bl_cdist_class", nw, Q.d_buf.ptr, Q.d_off.ptr, b.ptr, len(g), C.d_buf.ptr,
+                 C.d_off.ptr, C.n, out.ptr, dt, int(k), stream)
+    if len(rest):
+        rb = DeviceBuffer.from_host(rest)
+        bufs.append(rb)
+        mx = int(max(Q.lengths[rest].max(), C.max_len))
+        call("bl_long", 0, Q.d_buf.ptr, Q.d_off.ptr, C.d_buf.ptr, C.d_off.ptr, rb.ptr, len(rest),
+             C.n, out.ptr, dt, 0, int(k), mx, stream)
+    if keep is not None:
+        keep.extend(bufs)
+    return bufs
+
+
+def _finish(out: DeviceArray, return_device: bool, stream=None):
+    call("bl_stream_sync", stream)
+    return out if return_device else out.get(stream)
+
+
+# ------------------------------------------------------------------ public API --
+def pairwise(a, b, max_dist=None, dtype=np.int32, return_device: bool = False):
+    """out[i] = d(a[i], b[i]) on the GPU (bitlev/gpu.py:71-103).
+    return_device=True returns a DeviceArray (`.get()` gives numpy)."""
+    a, b = as_list(a), as_list(b)
+    if len(a) != len(b):
+        raise ValueError(f"length mismatch: {len(a)} vs {len(b)}")
+    k, dt = check_k(max_dist), check_dtype(dtype)
+    n = len(a)
+    pa, pb = pack(a), pack(b)
+    _require_device()
+    out = DeviceArray.empty((n,), dt)
+    if n == 0:
+        return out if return_device else np.empty(0, dt)
+    A, B = Packed(*pa), Packed(*pb)
+    keep = []
+    pairs_into(A, B, out, k, True, None, keep)
+    return _finish(out, return_device)
+
+
+def cdist(queries, choices, max_dist=None, dtype=np.int32, return_device: bool = False):
+    """out[i, j] = d(queries[i], choices[j]) on the GPU (bitlev/gpu.py:106-138)."""
+    q, c = as_list(queries), as_list(choices)
+    k, dt = check_k(max_dist), check_dtype(dtype)
+    nq, nc = len(q), len(c)
+    pq, pc = pack(q), pack(c)
+    _require_device()
+    out = DeviceArray.empty((nq, nc), dt)
+    if nq == 0 or nc == 0:
+        return out if return_device else np.empty((nq, nc), dt)
+    Q, C = Packed(*pq), Packed(*pc)
+    keep = []
+    cdist_into(Q, C, out, k, None, keep)
+    return _finish(out, return_device)
+
+
+def distance(a, b, max_dist=None) -> int:
+    """Edit distance of one pair (min(d, max_dist + 1) with max_dist),
+    bitlev/__init__.py:41-45."""
+    return int(pairwise([a], [b], max_dist=max_dist)[0])

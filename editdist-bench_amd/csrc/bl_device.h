@@ -1,0 +1,493 @@
+/*
+ * bl_device.h -- per-lane building blocks of the gfx950 edit-distance kernels.
+ *
+ * Everything here compiles both as HIP device code (hipcc, gfx950) and as plain
+ * C++ (g++), so bl_hostcheck.cpp runs the very same per-lane algorithms on the
+ * CPU against the oracle before any of it reaches a GPU.
+ *
+ * Reference behaviour followed (karims/editdist-bench):
+ *   word_step          csrc/myers.c:22-36, csrc/gpu_myers.cuh:45-59 (same recurrence)
+ *   trim_pair          csrc/myers.c:80-87, csrc/gpu_myers.cuh:62-68 (exact affix trim)
+ *   pair_lane<NW>      csrc/gpu_myers.cuh:73-119  (pattern in registers, one lane per pair)
+ *   peq_lane<NW>       csrc/gpu_myers.cuh:124-145 (pattern via a Peq table)
+ *   store_out          csrc/gpu_myers.cuh:149-152, bitlev/gpu.py:64-68 (saturating dtypes)
+ *
+ * Design points that differ from the reference (results are identical):
+ *   * Eq comes from eight bit-planes of the pattern, not byte-wise SWAR compares:
+ *     Eq = AND_b ~(plane_b ^ broadcast(bit b of the text byte)), one v_bitop3 per
+ *     plane per 32 pattern rows. All 64 rows of a word compare in 16 ops.
+ *   * The score is never tracked per column. Row 0 of a global alignment is j, so
+ *     D[m][j] = j + popcount(Pv & mask) - popcount(Mv & mask); the threshold exit is
+ *     tested every 4 columns (a later exit never changes min(d, k + 1)).
+ *   * Text bytes are read 4 at a time with aligned dword loads only, and only
+ *     dwords that hold at least one byte of the string are ever touched, so no
+ *     load can leave the caller's allocation whatever the string alignment.
+ */
+#pragma once
+#include <stdint.h>
+#include <string.h>
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define BL_HD __host__ __device__ __forceinline__
+#define BL_HDM __host__ __device__ __forceinline__
+#else
+#define BL_HD static inline
+#define BL_HDM inline
+#endif
+
+namespace bl {
+
+typedef uint8_t u8;
+typedef uint32_t u32;
+typedef uint64_t u64;
+
+/* Output dtype codes of the C ABI. */
+enum { DT_INT32 = 0, DT_UINT16 = 1, DT_UINT8 = 2 };
+
+/* ------------------------------------------------------------ bit utilities */
+/* v_bitop3_b32 truth tables: operand S0, S1, S2 stand for 0xF0, 0xCC, 0xAA. */
+constexpr u32 TA = 0xF0, TB = 0xCC, TC = 0xAA;
+constexpr u32 tt(u32 v) { return v & 0xFFu; }
+
+template <u32 TT>
+BL_HD u32 bop3(u32 a, u32 b, u32 c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, TT);
+#else
+    u32 r = 0;
+    for (int i = 0; i < 8; i++)
+        if ((TT >> i) & 1u)
+            r |= ((i & 4) ? a : ~a) & ((i & 2) ? b : ~b) & ((i & 1) ? c : ~c);
+    return r;
+#endif
+}
+
+template <u32 TT>
+BL_HD u64 bop3(u64 a, u64 b, u64 c)
+{
+    const u32 lo = bop3<TT>((u32)a, (u32)b, (u32)c);
+    const u32 hi = bop3<TT>((u32)(a >> 32), (u32)(b >> 32), (u32)(c >> 32));
+    return ((u64)hi << 32) | lo;
+}
+
+BL_HD int popc64(u64 x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __popcll(x);
+#else
+    return __builtin_popcountll(x);
+#endif
+}
+
+BL_HD int ctz32(u32 x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_ctz(x);
+#else
+    return __builtin_ctz(x);
+#endif
+}
+
+BL_HD int clz32(u32 x) { return __builtin_clz(x); }
+
+/* bytes [sh, sh+4) of the little-endian 8-byte value hi:lo (v_alignbyte_b32). */
+BL_HD u32 align_bytes(u32 hi, u32 lo, u32 sh)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_alignbyte(hi, lo, sh);
+#else
+    return (u32)((((u64)hi << 32) | lo) >> (8 * (sh & 3)));
+#endif
+}
+
+/* 0 or ~0: bit b of c broadcast (v_bfe_i32 c, b, 1). */
+BL_HD u32 bit_mask(u32 c, int b) { return (u32)((int32_t)(c << (31 - b)) >> 31); }
+
+/* Aligned 4-byte load. */
+BL_HD u32 ld32(const u8 *p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const u32 *)p;
+#else
+    u32 v;
+    memcpy(&v, p, 4);
+    return v;
+#endif
+}
+
+/* s[i .. i+4) as a little-endian u32, 0 <= i < n. Bytes at or past n are
+ * unspecified. Only aligned dwords holding a byte of s[0..n) are read. */
+BL_HD u32 load4(const u8 *s, int i, int n)
+{
+    const uintptr_t a = (uintptr_t)(s + i);
+    const u8 *al = (const u8 *)(a & ~(uintptr_t)3);
+    const u8 *last = (const u8 *)((uintptr_t)(s + n - 1) & ~(uintptr_t)3);
+    const u32 lo = ld32(al);
+    const u32 hi = ld32(al + 4 <= last ? al + 4 : last);
+    return align_bytes(hi, lo, (u32)(a & 3));
+}
+
+/* Sequential reader of a string, 4 bytes per call, one aligned load each. */
+struct ByteStream {
+    const u8 *al, *last;
+    u32 cur, sh;
+    BL_HDM void open(const u8 *s, int n)
+    {
+        const uintptr_t a = (uintptr_t)s;
+        al = (const u8 *)(a & ~(uintptr_t)3);
+        last = (const u8 *)((uintptr_t)(s + (n > 0 ? n - 1 : 0)) & ~(uintptr_t)3);
+        sh = (u32)(a & 3);
+        cur = n > 0 ? ld32(al) : 0;
+    }
+    /* The first call returns bytes 0..3, the next 4..7, ... (the caller stops
+     * at n; bytes past n are unspecified). */
+    BL_HDM u32 next4()
+    {
+        const u8 *nxa = al + 4;
+        const u32 nx = ld32(nxa <= last ? nxa : last);
+        const u32 v = align_bytes(nx, cur, sh);
+        cur = nx;
+        al = nxa;
+        return v;
+    }
+};
+
+/* ------------------------------------------------------------ output */
+BL_HD void store_out(void *out, int64_t i, int d, int dtype)
+{
+    if (dtype == DT_UINT8) ((u8 *)out)[i] = (u8)(d > 255 ? 255 : d);
+    else if (dtype == DT_UINT16) ((uint16_t *)out)[i] = (uint16_t)(d > 65535 ? 65535 : d);
+    else ((int32_t *)out)[i] = d;
+}
+
+/* ------------------------------------------------------------ Myers column step */
+/* One text column on one 64-bit block of pattern rows. hp / hn (0 or 1) are
+ * the +1 / -1 horizontal delta entering the lowest row. Ph / Mh receive the
+ * deltas leaving every row (bit i = row i) before they are shifted down; a
+ * following block takes bit 63 of them as its own hp / hn. */
+BL_HD void word_step(u64 &Pv, u64 &Mv, u64 Eq, u32 hp, u32 hn, u64 &Ph, u64 &Mh)
+{
+    const u64 Xv = Eq | Mv;
+    const u64 E = Eq | (u64)hn;
+    const u64 S = (E & Pv) + Pv;
+    const u64 Xh = bop3<tt((TA ^ TB) | TC)>(S, Pv, E);          /* (S ^ Pv) | E   */
+    Ph = bop3<tt(TA | ~(TB | TC))>(Mv, Xh, Pv);                 /* Mv | ~(Xh | Pv) */
+    Mh = Pv & Xh;
+    const u64 Phs = (Ph << 1) | (u64)hp;
+    const u64 Mhs = (Mh << 1) | (u64)hn;
+    Pv = bop3<tt(TA | ~(TB | TC))>(Mhs, Xv, Phs);               /* Mhs | ~(Xv | Phs) */
+    Mv = Phs & Xv;
+}
+
+/* Rows [0, rows) of a block: all-ones below `rows` (1 <= rows <= 64). */
+BL_HD u64 row_mask(int rows) { return rows >= 64 ? ~(u64)0 : (((u64)1 << rows) - 1); }
+
+/* ------------------------------------------------------------ bit-plane pattern */
+/* 8x8 bit transpose: byte c of the result holds, in bit r, bit c of byte r. */
+BL_HD u64 transpose8(u64 x)
+{
+    u64 t;
+    t = (x ^ (x >> 7)) & 0x00AA00AA00AA00AAull;  x ^= t ^ (t << 7);
+    t = (x ^ (x >> 14)) & 0x0000CCCC0000CCCCull; x ^= t ^ (t << 14);
+    t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull; x ^= t ^ (t << 28);
+    return x;
+}
+
+/* Planes of one 64-row block: pl[b] bit i = bit b of p[i] (i < n; rows at or
+ * past n hold unspecified bits, which never reach a row below them). */
+BL_HD void build_planes(const u8 *p, int n, u64 (&pl)[8])
+{
+    u32 half[2][8];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int b = 0; b < 8; b++) half[h][b] = 0;
+#pragma unroll
+    for (int g = 0; g < 8; g++) {                       /* 8 rows per group */
+        if (8 * g < n) {
+            const u32 lo = load4(p, 8 * g, n);
+            const u32 hi = 8 * g + 4 < n ? load4(p, 8 * g + 4, n) : 0u;
+            const u64 y = transpose8(((u64)hi << 32) | lo);
+#pragma unroll
+            for (int b = 0; b < 8; b++)
+                half[g >> 2][b] |= (u32)((y >> (8 * b)) & 0xFFu) << (8 * (g & 3));
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < 8; b++) pl[b] = ((u64)half[1][b] << 32) | half[0][b];
+}
+
+/* Per text byte: m[b] = bit b of c broadcast; m[0] is stored inverted. */
+BL_HD void text_masks(u32 c, u32 (&m)[8])
+{
+#pragma unroll
+    for (int b = 0; b < 8; b++) m[b] = bit_mask(c, b);
+    m[0] = ~m[0];
+}
+
+/* Eq of one block: AND over b of ~(plane_b ^ m_b). */
+BL_HD u64 eq_planes(const u64 (&pl)[8], const u32 (&m)[8])
+{
+    u32 lo = (u32)pl[0] ^ m[0], hi = (u32)(pl[0] >> 32) ^ m[0];
+#pragma unroll
+    for (int b = 1; b < 8; b++) {
+        lo = bop3<tt(TA & ~(TB ^ TC))>(lo, (u32)pl[b], m[b]);
+        hi = bop3<tt(TA & ~(TB ^ TC))>(hi, (u32)(pl[b] >> 32), m[b]);
+    }
+    return ((u64)hi << 32) | lo;
+}
+
+/* ------------------------------------------------------------ trimming */
+/* d(xPy, xQy) == d(P, Q): drop the common prefix, then the common suffix of
+ * what remains (never overlapping the prefix). 4 bytes per compare. */
+BL_HD void trim_pair(const u8 *&a, int &la, const u8 *&b, int &lb)
+{
+    const int n = la < lb ? la : lb;
+    int p = 0;
+    while (p < n) {
+        u32 x = load4(a, p, la) ^ load4(b, p, lb);
+        const int r = n - p;
+        if (r < 4) x &= (1u << (8 * r)) - 1u;
+        if (x) { p += ctz32(x) >> 3; break; }
+        p += 4;
+    }
+    if (p > n) p = n;
+    const int lim = n - p;
+    int s = 0;
+    while (s < lim) {
+        /* bytes a[la-4-s .. la-1-s] (top byte = a[la-1-s]) */
+        const int ea = la - s, eb = lb - s;
+        u32 xa = ea >= 4 ? load4(a, ea - 4, la) : load4(a, 0, la) << (8 * (4 - ea));
+        u32 xb = eb >= 4 ? load4(b, eb - 4, lb) : load4(b, 0, lb) << (8 * (4 - eb));
+        u32 x = xa ^ xb;
+        const int r = lim - s;
+        if (r < 4) x &= ~((1u << (8 * (4 - r))) - 1u);
+        if (x) { s += clz32(x) >> 3; break; }
+        s += 4;
+    }
+    if (s > lim) s = lim;
+    a += p; b += p;
+    la -= p + s; lb -= p + s;
+}
+
+/* ------------------------------------------------------------ pair lane */
+/* Distance of one pair whose shorter string (after the optional trim) has at
+ * most 64*NW bytes: the pattern lives in registers as bit-planes.
+ * k < 0: exact; k >= 0: min(d, k + 1). Semantics of pair_distance_swar,
+ * csrc/gpu_myers.cuh:73-119, and pair_distance_k, csrc/myers.c:132-143. */
+template <int NW>
+BL_HD int pair_lane(const u8 *a, int la, const u8 *b, int lb, int trim, int k)
+{
+    if (trim) trim_pair(a, la, b, lb);
+    if (la > lb) {
+        const u8 *t = a; a = b; b = t;
+        const int x = la; la = lb; lb = x;
+    }
+    if (k >= 0 && lb - la > k) return k + 1;
+    if (la == 0) return lb;
+
+    u64 pl[NW][8];
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        if (64 * w < la) build_planes(a + 64 * w, la - 64 * w, pl[w]);
+    }
+    const int nw = (la + 63) >> 6;
+    const u64 lastmask = row_mask(la - 64 * (nw - 1));
+    u64 Pv[NW], Mv[NW];
+#pragma unroll
+    for (int w = 0; w < NW; w++) { Pv[w] = ~(u64)0; Mv[w] = 0; }
+
+    ByteStream ts;
+    ts.open(b, lb);
+    for (int j = 0; j < lb; j += 4) {
+        const u32 c4 = ts.next4();
+        const int cnt = lb - j < 4 ? lb - j : 4;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (u < cnt) {
+                u32 m[8];
+                text_masks((c4 >> (8 * u)) & 0xFFu, m);
+                u32 hp = 1, hn = 0;
+#pragma unroll
+                for (int w = 0; w < NW; w++) {
+                    if (w < nw) {
+                        u64 Ph, Mh;
+                        word_step(Pv[w], Mv[w], eq_planes(pl[w], m), hp, hn, Ph, Mh);
+                        hp = (u32)(Ph >> 63);
+                        hn = (u32)(Mh >> 63);
+                    }
+                }
+            }
+        }
+        if (k >= 0) {
+            int sc = j + cnt;
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                if (w < nw) {
+                    const u64 mk = w == nw - 1 ? lastmask : ~(u64)0;
+                    sc += popc64(Pv[w] & mk) - popc64(Mv[w] & mk);
+                }
+            }
+            if (sc - (lb - j - cnt) > k) return k + 1;
+        }
+    }
+    int d = lb;
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        if (w < nw) {
+            const u64 mk = w == nw - 1 ? lastmask : ~(u64)0;
+            d += popc64(Pv[w] & mk) - popc64(Mv[w] & mk);
+        }
+    }
+    return d;
+}
+
+/* ------------------------------------------------------------ Peq lane */
+/* Distance of a pattern given by its Peq table (row c: NW words, peq[c*NW + w])
+ * of length 1 <= m <= 64*NW against text t. No trim: the pattern is the query.
+ * Semantics of text_distance_peq, csrc/gpu_myers.cuh:124-145, and run_text /
+ * run_text_k, csrc/myers.c:58-77, :107-130. */
+template <int NW, typename PeqPtr>
+BL_HD int peq_lane(PeqPtr peq, int m, const u8 *t, int n, int k)
+{
+    if (k >= 0 && (n > m ? n - m : m - n) > k) return k + 1;
+    if (n == 0) return m;
+    const int nw = (m + 63) >> 6;
+    const u64 lastmask = row_mask(m - 64 * (nw - 1));
+    u64 Pv[NW], Mv[NW];
+#pragma unroll
+    for (int w = 0; w < NW; w++) { Pv[w] = ~(u64)0; Mv[w] = 0; }
+    ByteStream ts;
+    ts.open(t, n);
+    for (int j = 0; j < n; j += 4) {
+        const u32 c4 = ts.next4();
+        const int cnt = n - j < 4 ? n - j : 4;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (u < cnt) {
+                const u32 c = (c4 >> (8 * u)) & 0xFFu;
+                u32 hp = 1, hn = 0;
+#pragma unroll
+                for (int w = 0; w < NW; w++) {
+                    if (w < nw) {
+                        u64 Ph, Mh;
+                        word_step(Pv[w], Mv[w], peq[c * NW + w], hp, hn, Ph, Mh);
+                        hp = (u32)(Ph >> 63);
+                        hn = (u32)(Mh >> 63);
+                    }
+                }
+            }
+        }
+        if (k >= 0) {
+            int sc = j + cnt;
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                if (w < nw) {
+                    const u64 mk = w == nw - 1 ? lastmask : ~(u64)0;
+                    sc += popc64(Pv[w] & mk) - popc64(Mv[w] & mk);
+                }
+            }
+            if (sc - (n - j - cnt) > k) return k + 1;
+        }
+    }
+    int d = n;
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        if (w < nw) {
+            const u64 mk = w == nw - 1 ? lastmask : ~(u64)0;
+            d += popc64(Pv[w] & mk) - popc64(Mv[w] & mk);
+        }
+    }
+    return d;
+}
+
+/* Empty cross query: d = n, or k + 1 when k >= 0 and n > k
+ * (csrc/gpu_kernels.cu:53-54, csrc/edpy.c:111-115). */
+BL_HD int empty_pattern(int n, int k) { return (k >= 0 && n > k) ? k + 1 : n; }
+
+/* ------------------------------------------------------------ long strings */
+/* Pairs whose shorter string exceeds 256 bytes, and cross queries longer than
+ * 512 bytes, run here instead of on the CPU (the reference falls back to its C
+ * core for them, bitlev/gpu.py:97-101, :131-136). One wave per (pattern, text):
+ * lane l owns pattern word 64*s + l of stripe s (4096 rows per stripe). The
+ * carry between words moves one lane per step, so lane l works on text column
+ * step - l (an anti-diagonal wavefront); each step hands (text byte, hp, hn) to
+ * the next lane. A stripe's bottom-row deltas go to hbuf[j] for the next
+ * stripe, so patterns of any length are covered. No threshold early exit here:
+ * with k >= 0 the result is min(d, k + 1) after the gap test.
+ *
+ * The lane logic below is shared by the device kernel and the host simulation
+ * in bl_hostcheck.cpp; only the lane-to-lane hand-off differs. */
+struct LongCtx {
+    const u8 *p;        /* pattern (the shorter string for pairs; the query for cross) */
+    int m;
+    const u8 *t;        /* text */
+    int n;
+    int W;              /* pattern words, ceil(m / 64) */
+    int s;              /* stripe */
+    int nl;             /* lanes with a word in this stripe */
+    int8_t *hbuf;       /* n bottom-row deltas between stripes (W > 64 only) */
+};
+
+struct LongLane {
+    u64 pl[8];
+    u64 Pv, Mv;
+    int w, bit, delta;
+};
+
+BL_HD void long_lane_init(const LongCtx &C, LongLane &L, int lane)
+{
+    L.w = 64 * C.s + lane;
+    L.Pv = ~(u64)0;
+    L.Mv = 0;
+    L.bit = L.w == C.W - 1 ? ((C.m - 1) & 63) : 63;
+    L.delta = 0;
+    if (lane < C.nl) build_planes(C.p + 64 * L.w, C.m - 64 * L.w, L.pl);
+}
+
+/* Input of lane 0 at `step`: bits 0..7 text byte, bit 8 hp, bit 9 hn. */
+BL_HD u32 long_source(const LongCtx &C, int step)
+{
+    if (step >= C.n) return 0;
+    const u32 c = C.t[step];
+    if (C.s == 0) return c | (1u << 8);                /* row 0: +1 per column */
+    const int h = ((volatile int8_t *)C.hbuf)[step];
+    return c | ((u32)(h > 0) << 8) | ((u32)(h < 0) << 9);
+}
+
+/* One step of one lane; returns what the next lane receives on the next step. */
+BL_HD u32 long_lane_step(const LongCtx &C, LongLane &L, int lane, int step, u32 in)
+{
+    const int j = step - lane;
+    if (lane >= C.nl || j < 0 || j >= C.n) return 0;
+    u32 m8[8];
+    text_masks(in & 0xFFu, m8);
+    u64 Ph, Mh;
+    word_step(L.Pv, L.Mv, eq_planes(L.pl, m8), (in >> 8) & 1u, (in >> 9) & 1u, Ph, Mh);
+    const u32 hp = (u32)(Ph >> L.bit) & 1u, hn = (u32)(Mh >> L.bit) & 1u;
+    if (L.w == C.W - 1) L.delta += (int)hp - (int)hn;
+    else if (lane == C.nl - 1) ((volatile int8_t *)C.hbuf)[j] = (int8_t)((int)hp - (int)hn);
+    return (in & 0xFFu) | (hp << 8) | (hn << 9);
+}
+
+/* Resolve one long item to (pattern, text). Pairs trim (optionally) as the
+ * reference does; an empty cross query keeps the reference's rule. Then the
+ * shorter string becomes the pattern (distance is symmetric, so this only
+ * shortens the wavefront) and the gap test applies. Returns the final result
+ * (>= 0) when no scan is needed, else -1. */
+BL_HD int long_prepare(const u8 *&p, int &m, const u8 *&t, int &n, int is_pair, int trim, int k)
+{
+    if (is_pair && trim) trim_pair(p, m, t, n);
+    if (!is_pair && m == 0) return empty_pattern(n, k);
+    if (m > n) {
+        const u8 *x = p; p = t; t = x;
+        const int y = m; m = n; n = y;
+    }
+    if (k >= 0 && n - m > k) return k + 1;
+    if (m == 0) return n;
+    return -1;
+}
+
+}  // namespace bl

@@ -1,0 +1,152 @@
+/*
+ * bl_hostcheck.cpp -- the kernels' per-lane code (bl_device.h) built for the
+ * host, so tests/test_hostcheck.py checks it against the oracle on a CPU.
+ * Counterpart of the reference's csrc/gpu_hostcheck.cpp:9-37.
+ *
+ * The long-string path is simulated lane by lane: the same long_lane_step()
+ * runs for all 64 lanes of a wave in lockstep and the hand-off between lanes
+ * is an array shift, which is what __shfl_up does on the device.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "bl_device.h"
+
+using namespace bl;
+
+static int long_sim(const u8 *p, int m, const u8 *t, int n, int is_pair, int trim, int k)
+{
+    const int r = long_prepare(p, m, t, n, is_pair, trim, k);
+    if (r >= 0) return r;
+    LongCtx C;
+    C.p = p; C.m = m; C.t = t; C.n = n;
+    C.W = (m + 63) >> 6;
+    std::vector<int8_t> hbuf((size_t)n + 1);
+    C.hbuf = hbuf.data();
+    const int S = (C.W + 63) >> 6;
+    static LongLane L[64];
+    for (int s = 0; s < S; s++) {
+        C.s = s;
+        C.nl = C.W - 64 * s < 64 ? C.W - 64 * s : 64;
+        for (int lane = 0; lane < 64; lane++) long_lane_init(C, L[lane], lane);
+        u32 prev[64], cur[64];
+        memset(prev, 0, sizeof prev);
+        for (int step = 0; step < n + C.nl - 1; step++) {
+            for (int lane = 0; lane < 64; lane++) {
+                const u32 in = lane == 0 ? long_source(C, step) : prev[lane - 1];
+                cur[lane] = long_lane_step(C, L[lane], lane, step, in);
+            }
+            memcpy(prev, cur, sizeof prev);
+        }
+    }
+    const int d = m + L[(C.W - 1) - 64 * (S - 1)].delta;
+    return (k >= 0 && d > k) ? k + 1 : d;
+}
+
+static int pair_host(const u8 *a, int la, const u8 *b, int lb, int trim, int k)
+{
+    const int m = la < lb ? la : lb;
+    if (m <= 64) return pair_lane<1>(a, la, b, lb, trim, k);
+    if (m <= 128) return pair_lane<2>(a, la, b, lb, trim, k);
+    if (m <= 256) return pair_lane<4>(a, la, b, lb, trim, k);
+    return long_sim(a, la, b, lb, 1, trim, k);
+}
+
+template <int NW>
+static int peq_host_nw(const u8 *q, int m, const u8 *t, int n, int k)
+{
+    std::vector<u64> peq((size_t)256 * NW, 0);
+    for (int i = 0; i < m; i++) peq[(size_t)q[i] * NW + (i >> 6)] |= (u64)1 << (i & 63);
+    return peq_lane<NW>(peq.data(), m, t, n, k);
+}
+
+static int peq_host(const u8 *q, int m, const u8 *t, int n, int k)
+{
+    if (m == 0) return empty_pattern(n, k);
+    const int w = (m + 63) >> 6;
+    if (w <= 1) return peq_host_nw<1>(q, m, t, n, k);
+    if (w <= 2) return peq_host_nw<2>(q, m, t, n, k);
+    if (w <= 4) return peq_host_nw<4>(q, m, t, n, k);
+    if (w <= 8) return peq_host_nw<8>(q, m, t, n, k);
+    return long_sim(q, m, t, n, 0, 0, k);
+}
+
+extern "C" {
+
+/* One pair through the lane code the pairs kernels run (class by the shorter
+ * length: <= 64, <= 128, <= 256 in registers; longer via the wavefront). */
+int blh_pair(const u8 *a, int la, const u8 *b, int lb, int trim, int k)
+{
+    return pair_host(a, la, b, lb, trim, k);
+}
+
+/* One (query, choice) through the Peq lane the cdist kernels run. */
+int blh_peq(const u8 *q, int m, const u8 *t, int n, int k)
+{
+    return peq_host(q, m, t, n, k);
+}
+
+/* The wavefront path for any lengths (is_pair: trim/swap as pairs do). */
+int blh_long(const u8 *p, int m, const u8 *t, int n, int is_pair, int trim, int k)
+{
+    return long_sim(p, m, t, n, is_pair, trim, k);
+}
+
+/* Packed batches, int32 results. */
+int blh_pairs(const u8 *ba, const int64_t *oa, const u8 *bb, const int64_t *ob, int64_t n,
+              int32_t *out, int trim, int k, int force_long)
+{
+    for (int64_t i = 0; i < n; i++) {
+        const u8 *a = ba + oa[i], *b = bb + ob[i];
+        const int la = (int)(oa[i + 1] - oa[i]), lb = (int)(ob[i + 1] - ob[i]);
+        out[i] = force_long ? long_sim(a, la, b, lb, 1, trim, k) : pair_host(a, la, b, lb, trim, k);
+    }
+    return 0;
+}
+
+int blh_cross(const u8 *bq, const int64_t *oq, int64_t nq, const u8 *bc, const int64_t *oc,
+              int64_t nc, int32_t *out, int k, int force_long)
+{
+    for (int64_t i = 0; i < nq; i++)
+        for (int64_t j = 0; j < nc; j++) {
+            const u8 *q = bq + oq[i], *t = bc + oc[j];
+            const int m = (int)(oq[i + 1] - oq[i]), n = (int)(oc[j + 1] - oc[j]);
+            out[i * nc + j] = force_long ? long_sim(q, m, t, n, 0, 0, k) : peq_host(q, m, t, n, k);
+        }
+    return 0;
+}
+
+/* Output conversion exactly as the kernels store it. */
+int blh_store(int d, int dtype)
+{
+    union { int32_t i; uint16_t h; u8 b; } o;
+    o.i = 0;
+    store_out(&o, 0, d, dtype);
+    return dtype == DT_UINT8 ? o.b : dtype == DT_UINT16 ? o.h : o.i;
+}
+
+/* Pieces the tests pin individually. */
+uint64_t blh_transpose8(uint64_t x) { return transpose8(x); }
+uint32_t blh_bop3(uint32_t a, uint32_t b, uint32_t c, int which)
+{
+    switch (which) {
+    case 0: return bop3<tt((TA ^ TB) | TC)>(a, b, c);
+    case 1: return bop3<tt(TA | ~(TB | TC))>(a, b, c);
+    case 2: return bop3<tt(TA & ~(TB ^ TC))>(a, b, c);
+    default: return 0;
+    }
+}
+void blh_trim(const u8 *a, int la, const u8 *b, int lb, int *res)
+{
+    const u8 *a0 = a, *b0 = b;
+    trim_pair(a, la, b, lb);
+    res[0] = (int)(a - a0);
+    res[1] = la;
+    res[2] = (int)(b - b0);
+    res[3] = lb;
+}
+
+}  /* extern "C" */

@@ -1,0 +1,105 @@
+/*
+ * bitlev_hip.h -- C ABI of libbitlev_hip.so, the MI355X (gfx950) backend of
+ * bitlev's exact unit-cost Levenshtein distance.
+ *
+ * Conventions (all entry points):
+ *   * Strings are packed: string i of a set is buf[off[i] : off[i+1]] with
+ *     int64 offsets (the layout of the reference's _edpy.pack, csrc/edpy.c:200-224).
+ *     Byte buffers need no padding or alignment.
+ *   * Pointers named d_* are device pointers (from bl_malloc or any HIP allocation
+ *     on the current device); h_* are host pointers.
+ *   * k < 0 asks for the exact distance; k >= 0 returns min(d, k + 1)
+ *     (bitlev/__init__.py:14-18). Values above 2^30 are clamped like the
+ *     reference's check_k (bitlev/_common.py:9-15).
+ *   * dtype: BL_INT32, BL_UINT16, BL_UINT8; narrow types saturate at their
+ *     maximum after the k cap (bitlev/_common.py:25-30).
+ *   * stream: a hipStream_t cast to void* (NULL = the default stream). Calls are
+ *     asynchronous on that stream unless documented otherwise.
+ *   * Return value: BL_OK (0) or a negative BL_E* code; bl_last_error() gives
+ *     the message of the calling thread's last failure.
+ *   * No entry point calls back into its caller; callers may release the GIL.
+ */
+#ifndef BITLEV_HIP_H
+#define BITLEV_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    BL_OK = 0,
+    BL_EALLOC = -1,    /* allocation failed (MemoryError)      -- csrc/myers.c:155-156 */
+    BL_EINVAL = -2,    /* invalid argument (ValueError)        -- csrc/myers.c:372-373 */
+    BL_EHIP = -3,      /* HIP runtime error                                        */
+    BL_ENODEV = -4,    /* no usable GPU                                            */
+};
+
+enum { BL_INT32 = 0, BL_UINT16 = 1, BL_UINT8 = 2 };
+
+/* ---------------------------------------------------------------- runtime */
+int bl_version(void);
+const char *bl_last_error(void);
+/* Number of visible HIP devices (0 when none); never fails. Replaces
+ * bitlev/gpu.py:36-41 (available()). */
+int bl_device_count(void);
+int bl_set_device(int device);
+/* Name and compute-unit count of the current device. */
+int bl_device_info(char *name, size_t name_len, int *cu_count, int *arch_major, int *arch_minor);
+
+int bl_malloc(void **d_ptr, size_t bytes);
+int bl_free(void *d_ptr);
+int bl_host_malloc(void **h_ptr, size_t bytes);        /* pinned host memory */
+int bl_host_free(void *h_ptr);
+int bl_memcpy_htod(void *d_dst, const void *h_src, size_t bytes, void *stream);
+int bl_memcpy_dtoh(void *h_dst, const void *d_src, size_t bytes, void *stream);
+int bl_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes, void *stream);
+int bl_memset(void *d_dst, int value, size_t bytes, void *stream);
+int bl_stream_create(void **stream);
+int bl_stream_destroy(void *stream);
+int bl_stream_sync(void *stream);
+int bl_device_sync(void);
+int bl_event_create(void **event);
+int bl_event_destroy(void *event);
+int bl_event_record(void *event, void *stream);
+int bl_event_sync(void *event);
+int bl_event_query(void *event);                       /* 1 done, 0 pending, <0 error */
+int bl_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* ---------------------------------------------------------------- plain form */
+/* One pairs class, one lane per pair: pair i = d_idx[t] for t < cnt, result
+ * written to out[i]. cap in {16, 32, 64, 128, 256}: every listed pair has
+ * min(la, lb) <= cap. Replaces the kernels ed_pairs_c{16,32,64,128,256}
+ * (csrc/gpu_kernels.cu:21-32) and their launch in bitlev/gpu.py:89-96. */
+int bl_pairs_class(int cap, const uint8_t *d_buf_a, const int64_t *d_off_a,
+                   const uint8_t *d_buf_b, const int64_t *d_off_b, const int32_t *d_idx,
+                   int64_t cnt, void *d_out, int dtype, int trim, int k, void *stream);
+
+/* One cdist class: queries d_qidx[0..nqc) (each at most 64*nw bytes, nw in
+ * {1, 2, 4, 8}) against all nc choices; out is the row-major nq x nc matrix of
+ * the call (out[q * nc + c]). One workgroup per (query, choice block), the
+ * query's Peq table in LDS. Replaces ed_cross_w{1,2,4,8}
+ * (csrc/gpu_kernels.cu:59-69) and their launch in bitlev/gpu.py:121-130. */
+int bl_cdist_class(int nw, const uint8_t *d_buf_q, const int64_t *d_off_q,
+                   const int32_t *d_qidx, int64_t nqc, const uint8_t *d_buf_c,
+                   const int64_t *d_off_c, int64_t nc, void *d_out, int dtype, int k,
+                   void *stream);
+
+/* Long strings on the GPU: pairs whose shorter string exceeds 256 bytes
+ * (is_pair = 1: d_idx lists pair indices; trim applies) or cdist queries
+ * longer than 512 bytes (is_pair = 0: d_idx lists query indices, each against
+ * all nc choices). One wave per (pattern, text), any length. max_len bounds
+ * every string involved; above 4096 it sizes a stream-ordered scratch of
+ * bottom-row deltas. Replaces the CPU fallbacks of bitlev/gpu.py:97-101 and
+ * :131-136. */
+int bl_long(int is_pair, const uint8_t *d_buf_a, const int64_t *d_off_a,
+            const uint8_t *d_buf_b, const int64_t *d_off_b, const int32_t *d_idx, int64_t cnt,
+            int64_t nc, void *d_out, int dtype, int trim, int k, int64_t max_len,
+            void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BITLEV_HIP_H */

@@ -1,0 +1,149 @@
+"""CPU-side checks of the boundary: the HIP library loads and exports every
+C ABI symbol include/*.h declares, the packing glue keeps the reference's
+contract (tests/test_pack.py:9-28, tests/test_phase3b.py:68-79 there), the
+argument checks match bitlev/_common.py, and the host class split matches
+bench/cudamyers.py:23-36 (tests/test_cuda_host.py:13-41 there). No GPU calls."""
+from __future__ import annotations
+
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import PKG, ROOT
+
+import bitlev
+from bitlev import _capi, _edpy, hip
+from bitlev._common import check_dtype, check_k
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(bl_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(_capi.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) == set(_capi.SIGNATURES), set(names) ^ set(_capi.SIGNATURES)
+
+
+def test_library_is_gfx950_code_object():
+    data = open(_capi.LIB_PATH, "rb").read()
+    assert b"gfx950" in data
+    assert b"sm_" not in data.split(b"gfx950")[0][-64:]
+
+
+def test_available_never_raises_without_gpu():
+    assert isinstance(hip.available(), bool)
+    assert isinstance(bitlev.available(), bool)
+
+
+def test_device_count_is_callable():
+    assert _capi.lib().bl_device_count() >= 0
+    assert _capi.lib().bl_version() == 1
+
+
+@pytest.mark.parametrize("S", [[], [""], ["", ""], ["ab", "", "xyz"], ["\xff\x00\x80"],
+                               ["q" * 1000] * 3, [b"by", "tes"]])
+def test_pack_layout(S):
+    buf, off = hip.pack(S)
+    enc = [s.encode("latin-1") if isinstance(s, str) else s for s in S]
+    assert off.dtype == np.int64 and off.tolist() == list(np.cumsum([0] + [len(e) for e in enc]))
+    assert bytes(buf[:off[-1]]) == b"".join(enc)
+    assert len(buf) >= 1                         # never empty: the GPU needs a pointer
+
+
+def test_pack_errors():
+    with pytest.raises(ValueError):
+        hip.pack(["ok", "ā"])
+    with pytest.raises(TypeError):
+        hip.pack(["ok", 3])
+    with pytest.raises(TypeError):
+        hip.pack(5)
+    n, total, mx = _edpy.sizes(["ab", b"", "xyz"])
+    assert (n, total, mx) == (3, 5, 3)
+    b = bytearray(5)
+    o = np.zeros(4, np.int64)
+    assert _edpy.pack_into(["ab", b"", "xyz"], b, o) == 5 and bytes(b) == b"abxyz"
+    with pytest.raises(ValueError):
+        _edpy.pack_into(["abc"], bytearray(2), np.zeros(2, np.int64))
+
+
+def test_argument_checks():
+    assert check_k(None) == -1 and check_k(3) == 3 and check_k(2**40) == 2**30
+    with pytest.raises(ValueError):
+        check_k(-1)
+    for dt in (np.int32, np.uint16, np.uint8, "uint8"):
+        check_dtype(dt)
+    with pytest.raises(ValueError):
+        check_dtype(np.float32)
+    with pytest.raises(ValueError):
+        bitlev.pairwise(["a"], ["b"], method="scalar")
+
+
+def test_api_errors_before_any_device_work():
+    with pytest.raises(ValueError):
+        hip.pairwise(["a"], ["a", "b"])
+    with pytest.raises(ValueError):
+        hip.cdist(["a"], ["b"], max_dist=-1)
+    with pytest.raises(ValueError):
+        hip.cdist(["a"], ["b"], dtype=np.float32)
+    with pytest.raises(ValueError):
+        hip.distance("ok", "ā")
+    with pytest.raises(TypeError):
+        hip.distance("ok", 3)
+
+
+def _off(lengths):
+    o = np.zeros(len(lengths) + 1, dtype=np.int64)
+    o[1:] = np.cumsum(lengths)
+    return o
+
+
+def test_pair_classes_use_shorter_length_and_boundaries():
+    la = [0, 16, 17, 32, 33, 64, 65, 128, 129, 256, 257, 300, 5]
+    lb = [9, 99, 17, 32, 400, 64, 65, 999, 129, 256, 257, 10, 300]
+    groups, rest = hip.pair_classes(_off(la), _off(lb))
+    got = {int(i): c for c, g in enumerate(groups) for i in g}
+    for i, mm in enumerate(min(x, y) for x, y in zip(la, lb)):
+        if mm > 256:
+            assert i in rest and i not in got
+        else:
+            cap = hip.PAIR_CAPS[got[i]]
+            assert mm <= cap and (got[i] == 0 or mm > hip.PAIR_CAPS[got[i] - 1])
+    assert sorted(list(rest) + list(got)) == list(range(len(la)))
+
+
+def test_query_classes():
+    lq = [0, 1, 64, 65, 128, 129, 256, 257, 512, 513]
+    groups, rest = hip.query_classes(_off(lq))
+    got = {int(i): hip.CROSS_NWS[c] for c, g in enumerate(groups) for i in g}
+    assert got == {0: 1, 1: 1, 2: 1, 3: 2, 4: 2, 5: 4, 6: 4, 7: 8, 8: 8}
+    assert list(rest) == [9]
+
+
+def test_error_mapping():
+    with pytest.raises(MemoryError):
+        _capi.check(_capi.BL_EALLOC, "x")
+    with pytest.raises(ValueError):
+        _capi.check(_capi.BL_EINVAL, "x")
+    with pytest.raises(_capi.HipError):
+        _capi.check(_capi.BL_EHIP, "x")
+    assert _capi.check(0) == 0
+
+
+def test_no_product_module_imports_the_oracle():
+    for path in glob.glob(os.path.join(PKG, "bitlev", "*.py")):
+        text = open(path).read()
+        assert "oracle" not in text.replace("oracle.py", ""), path

@@ -1,0 +1,157 @@
+"""The kernels' per-lane code (csrc/bl_device.h) built for the host and checked
+against the golden vectors and the oracle -- no GPU needed. Mirrors the
+reference's tests/test_gpu_logic.py:42-122, which does the same for its CUDA
+device functions."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from conftest import PKG, cross_of, golden_cross_sets, golden_pair_sets, pairs_of
+from oracle import oracle as O
+
+PATH = os.path.join(PKG, "bitlev", "libbl_hostcheck.so")
+KS = [-1, 0, 1, 2, 3, 5, 10, 17, 100]
+u8p, i64p, i32p = (ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_int64),
+                   ctypes.POINTER(ctypes.c_int32))
+
+
+@pytest.fixture(scope="module")
+def hc():
+    if not os.path.exists(PATH):
+        pytest.fail(f"{PATH} not built (make -C editdist-bench_amd)")
+    L = ctypes.CDLL(PATH)
+    L.blh_pairs.argtypes = [u8p, i64p, u8p, i64p, ctypes.c_int64, i32p, ctypes.c_int, ctypes.c_int,
+                            ctypes.c_int]
+    L.blh_cross.argtypes = [u8p, i64p, ctypes.c_int64, u8p, i64p, ctypes.c_int64, i32p,
+                            ctypes.c_int, ctypes.c_int]
+    L.blh_store.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.blh_transpose8.argtypes = [ctypes.c_uint64]
+    L.blh_transpose8.restype = ctypes.c_uint64
+    L.blh_bop3.argtypes = [ctypes.c_uint32] * 3 + [ctypes.c_int]
+    L.blh_bop3.restype = ctypes.c_uint32
+    L.blh_trim.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                           ctypes.POINTER(ctypes.c_int)]
+    return L
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def hc_pairs(L, pa, pb, k=-1, trim=1, force_long=0):
+    (ba, oa), (bb, ob) = pa, pb
+    n = len(oa) - 1
+    out = np.empty(n, np.int32)
+    L.blh_pairs(_p(ba, u8p), _p(oa, i64p), _p(bb, u8p), _p(ob, i64p), n, _p(out, i32p), trim, k,
+                force_long)
+    return out
+
+
+def hc_cross(L, pq, pc, k=-1, force_long=0):
+    (bq, oq), (bc, oc) = pq, pc
+    nq, nc = len(oq) - 1, len(oc) - 1
+    out = np.empty(nq * nc, np.int32)
+    L.blh_cross(_p(bq, u8p), _p(oq, i64p), nq, _p(bc, u8p), _p(oc, i64p), nc, _p(out, i32p), k,
+                force_long)
+    return out.reshape(nq, nc)
+
+
+@pytest.mark.parametrize("force_long", [0, 1])
+def test_pairs_lane_code_matches_golden(hc, golden, force_long):
+    for name in golden_pair_sets(golden):
+        pa, pb = pairs_of(golden, name)
+        for k in KS:
+            for trim in (0, 1):
+                want = golden[f"pairs/{name}/pairs_k/k{k}/trim{trim}"]
+                np.testing.assert_array_equal(hc_pairs(hc, pa, pb, k, trim, force_long), want,
+                                              err_msg=f"{name} k={k} trim={trim}")
+
+
+@pytest.mark.parametrize("force_long", [0, 1])
+def test_peq_lane_code_matches_golden(hc, golden, force_long):
+    for name in golden_cross_sets(golden):
+        pq, pc = cross_of(golden, name)
+        for k in KS:
+            np.testing.assert_array_equal(hc_cross(hc, pq, pc, k, force_long),
+                                          golden[f"cross/{name}/cross_k/k{k}"],
+                                          err_msg=f"{name} k={k}")
+
+
+def test_wavefront_multi_stripe(hc):
+    """Patterns above 4096 bytes take several 64-word stripes and the bottom-row
+    scratch between them; compare with the oracle (no word limit)."""
+    rng = np.random.default_rng(8)
+    A, B = [], []
+    for m, extra in [(4095, 3), (4096, 0), (4097, 10), (5000, -7), (8193, 40), (9000, 0)]:
+        a = bytes(rng.integers(0, 4, m).astype(np.uint8) + 65)
+        b = bytearray(a)
+        for _ in range(30):
+            b[int(rng.integers(0, len(b)))] = 65 + int(rng.integers(0, 4))
+        b = bytes(b[: len(b) + extra]) if extra < 0 else bytes(b) + b"A" * extra
+        A.append(a)
+        B.append(b)
+    A.append(bytes(rng.integers(0, 256, 6000).astype(np.uint8)))
+    B.append(bytes(rng.integers(0, 256, 5500).astype(np.uint8)))
+    pa, pb = O.pack(A), O.pack(B)
+    for k in (-1, 50, 10**6):
+        want = O.pairs(pa, pb, k=k)
+        np.testing.assert_array_equal(hc_pairs(hc, pa, pb, k, 1, 1), want)
+        np.testing.assert_array_equal(hc_pairs(hc, pa, pb, k, 0, 1), want)
+    want = O.cross(O.pack(A[:3]), O.pack(B[3:]))
+    np.testing.assert_array_equal(hc_cross(hc, O.pack(A[:3]), O.pack(B[3:]), -1, 1), want)
+
+
+def test_random_against_oracle(hc):
+    rng = np.random.default_rng(21)
+    A, B = [], []
+    for _ in range(3000):
+        alpha = int(rng.choice([2, 4, 26, 256]))
+        A.append(bytes(rng.integers(0, alpha, int(rng.integers(0, 300))).astype(np.uint8)))
+        B.append(bytes(rng.integers(0, alpha, int(rng.integers(0, 300))).astype(np.uint8)))
+    pa, pb = O.pack(A), O.pack(B)
+    for k in (-1, 0, 4, 40):
+        np.testing.assert_array_equal(hc_pairs(hc, pa, pb, k), O.pairs(pa, pb, k=k))
+
+
+def test_store_saturates(hc, golden):
+    ins = golden["store8/in"]
+    assert [hc.blh_store(int(d), 2) for d in ins] == golden["store8/out"].tolist()
+    assert [hc.blh_store(d, 1) for d in (0, 65534, 65535, 65536, 10**9)] == [0, 65534, 65535, 65535, 65535]
+    assert [hc.blh_store(d, 0) for d in (0, 70000)] == [0, 70000]
+
+
+def test_transpose8_and_bitop3(hc):
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        x = int(rng.integers(0, 2**63)) * 2 + int(rng.integers(0, 2))
+        y = hc.blh_transpose8(x)
+        for r in range(8):
+            for c in range(8):
+                assert (y >> (8 * c + r)) & 1 == (x >> (8 * r + c)) & 1
+    for _ in range(200):
+        a, b, c = (int(v) for v in rng.integers(0, 2**32, 3))
+        M = 0xFFFFFFFF
+        assert hc.blh_bop3(a, b, c, 0) == ((a ^ b) | c) & M
+        assert hc.blh_bop3(a, b, c, 1) == (a | ~(b | c)) & M
+        assert hc.blh_bop3(a, b, c, 2) == (a & ~(b ^ c)) & M
+
+
+@pytest.mark.parametrize("a,b", [(b"", b""), (b"abc", b"abc"), (b"abcXdef", b"abcYdef"),
+                                 (b"aaaa", b"aa"), (b"xabcab", b"abab"), (b"0123456789", b"01234"),
+                                 (b"prefix-middle-suffix", b"prefix-MID-suffix"),
+                                 (b"ab", b"abab"), (b"abab", b"ab")])
+def test_trim_matches_definition(hc, a, b):
+    res = (ctypes.c_int * 4)()
+    hc.blh_trim(a, len(a), b, len(b), res)
+    n = min(len(a), len(b))
+    p = 0
+    while p < n and a[p] == b[p]:
+        p += 1
+    s = 0
+    while s < n - p and a[len(a) - 1 - s] == b[len(b) - 1 - s]:
+        s += 1
+    assert list(res) == [p, len(a) - p - s, p, len(b) - p - s]
