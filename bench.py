@@ -1,0 +1,301 @@
+#!/usr/bin/env python3
+"""Throughput of the bitlev MI355X backend on one GPU; prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--workload NAME] [--form plain|batched|both]
+
+A step is one pass of the hot path over one batch with inputs already
+resident in HBM: for the default workload, one cdist of 1,024 queries x
+1,000,000 choices (length 64, 'a'-'z', 30% near duplicates), max_dist=2,
+uint8 results kept on the device. Other workloads: see bitlev.workloads.
+
+Besides the timed steps the run
+  * checks a sample of GPU rows against the reference C core (or the oracle);
+  * times that CPU core on the same sample, 1 thread and all cores;
+  * counts the useful work of the sample with the oracle's column counter
+    (columns the reference's threshold exit rule evaluates x pattern rows) and
+    reports the kernel against the INT32 VALU roofline.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "editdist-bench_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+METRIC = "M pairs/s (cdist L64 k=2 uint8), batched vs plain; p50/p99 request latency"
+# INT32 VALU peak of MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz.
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+# Floor of the Myers recurrence: 10 INT32 ops per 32 DP cells (3-input bit ops).
+OPS_PER_CELL = 10.0 / 32.0
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="cdist-L64-k2")
+    ap.add_argument("--form", default="plain", choices=["plain", "batched", "both"])
+    ap.add_argument("--cpu-rows", type=int, default=None,
+                    help="sample size for parity / CPU baseline (queries for cdist, pairs/1024 for pairs)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity sample")
+    ap.add_argument("--seed", type=int, default=None)
+    return ap.parse_args()
+
+
+class Events:
+    """HIP events on one stream through the C ABI."""
+
+    def __init__(self, capi, stream):
+        self.capi, self.stream = capi, stream
+
+    def new(self):
+        e = ctypes.c_void_p()
+        self.capi.call("bl_event_create", ctypes.byref(e))
+        return e.value
+
+    def record(self, e):
+        self.capi.call("bl_event_record", e, self.stream)
+
+    def ms(self, a, b):
+        self.capi.call("bl_event_sync", b)
+        v = ctypes.c_float()
+        self.capi.call("bl_event_elapsed_ms", a, b, ctypes.byref(v))
+        return float(v.value)
+
+
+def device_info(capi):
+    name = ctypes.create_string_buffer(256)
+    cu, ma, mi = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    capi.call("bl_device_info", name, 256, ctypes.byref(cu), ctypes.byref(ma), ctypes.byref(mi))
+    return {"name": name.value.decode(), "cus": cu.value}
+
+
+def timed_steps(step, ev, steps, warmup):
+    for _ in range(warmup):
+        step()
+    ev.capi.call("bl_device_sync")
+    starts = [ev.new() for _ in range(steps)]
+    stops = [ev.new() for _ in range(steps)]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        ev.record(starts[i])
+        step()
+        ev.record(stops[i])
+    ev.capi.call("bl_device_sync")
+    wall = time.perf_counter() - t0
+    per = [ev.ms(a, b) for a, b in zip(starts, stops)]
+    for e in starts + stops:
+        ev.capi.call("bl_event_destroy", e)
+    return wall, per
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the path "
+              "shards by choice or pair ranges with no exchange -- run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from bitlev import _capi, hip
+    from bitlev import workloads as W
+
+    if not hip.available():
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    wl = W.WORKLOADS[args.workload]
+    seed = args.seed if args.seed is not None else W.SEED
+    k = -1 if wl["k"] is None else int(wl["k"])
+    dt = np.dtype(wl["dtype"])
+    info = device_info(_capi)
+
+    t0 = time.perf_counter()
+    if wl["kind"] == "cdist":
+        hq, hc = W.cdist(wl["gen"], wl["nq"], wl["nc"], seed)
+        X, Y = hip.Packed(*hq), hip.Packed(*hc)
+        out = hip.DeviceArray.empty((X.n, Y.n), dt)
+        plan = hip.cdist_plan(X, Y)
+        pairs_total = X.n * Y.n
+
+        def step():
+            hip.cdist_into(X, Y, out, k, None, plan)
+    else:
+        ha, hb = W.pairs(wl["gen"], wl["n"], seed)
+        X, Y = hip.Packed(*ha), hip.Packed(*hb)
+        out = hip.DeviceArray.empty((X.n,), dt)
+        plan = hip.pairs_plan(X, Y)
+        pairs_total = X.n
+
+        def step():
+            hip.pairs_into(X, Y, out, k, True, None, plan)
+    _capi.call("bl_device_sync")
+    setup_s = time.perf_counter() - t0
+
+    ev = Events(_capi, None)
+    wall, per = timed_steps(step, ev, args.steps, args.warmup)
+    ms_step = wall * 1e3 / args.steps
+    plain = {
+        "value": pairs_total / (ms_step / 1e3) / 1e6,
+        "ms_per_step": ms_step,
+        "kernel_ms_mean": float(np.mean(per)),
+        "kernel_ms_min": float(np.min(per)),
+    }
+
+    res = {
+        "metric": METRIC,
+        "value": round(plain["value"], 3),
+        "unit": "M pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": config_of(args.workload, wl),
+        "form": "plain",
+        "plain": rounded(plain),
+        "batched": None,
+        "device": info,
+        "setup_s": round(setup_s, 2),
+    }
+
+    if not args.no_cpu:
+        cpu_leg(res, args, wl, k, dt, hq if wl["kind"] == "cdist" else ha,
+                hc if wl["kind"] == "cdist" else hb, out, plain)
+    print(json.dumps(res), flush=True)
+
+
+def rounded(d):
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
+
+
+def config_of(name, wl):
+    c = {"workload": name}
+    if wl["kind"] == "cdist":
+        c.update(queries=wl["nq"], choices=wl["nc"])
+    else:
+        c.update(pairs=wl["n"])
+    c.update(max_dist=wl["k"], out_dtype=wl["dtype"], return_device=True, gpus=1)
+    return c
+
+
+def cpu_leg(res, args, wl, k, dt, hx, hy, out, plain):
+    """Reference C core on a bounded sample: parity of those GPU results, the
+    1-thread and all-core rates, and the useful work (band cells) of the
+    sample for the roofline. The sample is a prefix of the same workload."""
+    from bitlev import workloads as W
+    from oracle import oracle as O
+
+    ref = O.Ref()
+    use_ref = ref.driver is not None
+    cpus = O.effective_cpus()
+    threads = max(1, min(cpus["effective"], 64))
+
+    if wl["kind"] == "cdist":
+        rows = args.cpu_rows or (128 if wl["nc"] <= 1_000_000 else 8)
+        rows = min(rows, wl["nq"])
+        sq = W.head(hx, rows)
+        # GPU rows [0, rows) of the result
+        gpu_rows = np.empty((rows, wl["nc"]), dt)
+        _copy_rows(out, gpu_rows)
+
+        def run_cpu(sample_q, th):
+            return ref.cross(sample_q, hy, k, th) if use_ref else O.cross(sample_q, hy, k, th)
+
+        t = time.perf_counter()
+        d = run_cpu(sq, threads)
+        t_all = time.perf_counter() - t
+        rows1 = max(1, min(rows, 8 if wl["nc"] <= 1_000_000 else 1))
+        t = time.perf_counter()
+        run_cpu(W.head(hx, rows1), 1)
+        t_one = time.perf_counter() - t
+        equal = bool(np.array_equal(O.saturate(d, dt), gpu_rows))
+        _, cols = O.cross(sq, hy, k, threads, cols=True)
+        m = np.diff(sq[1]).astype(np.float64)
+        cells_sample = float((cols.astype(np.float64) * m[:, None]).sum())
+        cells_total = cells_sample * wl["nq"] / rows
+        pairs_sample, pairs_one = rows * wl["nc"], rows1 * wl["nc"]
+        sample_txt = (f"first {rows} of {wl['nq']} queries x all {wl['nc']:,} choices "
+                      f"(all cores); first {rows1} queries (1 thread)")
+    else:
+        n = wl["n"]
+        rows = min(n, (args.cpu_rows or 2048) * 1024)
+        sa, sb = W.head(hx, rows), W.head(hy, rows)
+        gpu_part = np.empty(rows, dt)
+        _copy_prefix(out, gpu_part)
+
+        def run_cpu(a, b, th):
+            return ref.pairs(a, b, k, True, th) if use_ref else O.pairs(a, b, k, True, th)
+
+        t = time.perf_counter()
+        d = run_cpu(sa, sb, threads)
+        t_all = time.perf_counter() - t
+        rows1 = min(rows, 262_144)
+        t = time.perf_counter()
+        run_cpu(W.head(hx, rows1), W.head(hy, rows1), 1)
+        t_one = time.perf_counter() - t
+        equal = bool(np.array_equal(O.saturate(d, dt), gpu_part))
+        la, lb = np.diff(hx[1]).astype(np.float64), np.diff(hy[1]).astype(np.float64)
+        cells_total = float((la * lb).sum())          # exact scan: untrimmed m x n
+        pairs_sample, pairs_one = rows, rows1
+        sample_txt = f"first {rows:,} of {n:,} pairs (all cores); first {rows1:,} (1 thread)"
+
+    res["parity"] = {"checked": pairs_sample, "equal": equal,
+                     "against": "reference C core" if use_ref else "oracle"}
+    res["cpu_baseline"] = {
+        "value": round(pairs_sample / t_all / 1e6, 3),
+        "unit": "M pairs/s",
+        "cores": threads,
+        "kind": "reference" if use_ref else "port",
+        "sample": sample_txt,
+        "single_thread": round(pairs_one / t_one / 1e6, 3),
+        "host_cpus": cpus,
+        "build": "-O3 -std=c11 (portable, no -march=native)",
+    }
+    # Dominant kernel: the timed step is one launch of the class kernel for
+    # these uniform-length workloads (one class), so its event time is the
+    # launch duration.
+    kernel_s = plain["kernel_ms_mean"] / 1e3
+    achieved = cells_total * OPS_PER_CELL / kernel_s / 1e12
+    res["roofline"] = {
+        "bound": "valu",
+        "achieved": round(achieved, 3),
+        "peak": round(VALU_PEAK_TOPS, 2),
+        "unit": "TOP/s",
+        "frac": round(achieved / VALU_PEAK_TOPS, 4),
+        "traffic": None,
+        "cells_per_launch": cells_total,
+        "gcups": round(cells_total / kernel_s / 1e9, 1),
+        "ops_per_cell": OPS_PER_CELL,
+        "kernel": "cdist_class_kernel<1>" if wl["kind"] == "cdist" else "pairs_class_kernel",
+    }
+    if not equal:
+        print("bench.py: WARNING: GPU results differ from the CPU reference on the sample",
+              file=sys.stderr)
+
+
+def _copy_rows(out, host):
+    from bitlev import _capi
+    _capi.call("bl_memcpy_dtoh", host.ctypes.data, out.ptr, host.nbytes, None)
+    _capi.call("bl_stream_sync", None)
+
+
+def _copy_prefix(out, host):
+    _copy_rows(out, host)
+
+
+if __name__ == "__main__":
+    main()

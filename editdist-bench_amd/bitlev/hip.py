@@ -29,8 +29,9 @@ from .device import DeviceArray, DeviceBuffer
 PAIR_CAPS = (16, 32, 64, 128, 256)   # pairs kernels: shorter string <= CAP
 CROSS_NWS = (1, 2, 4, 8)             # cdist kernels: query <= 64 * NW
 
-__all__ = ["available", "distance", "pairwise", "cdist", "Packed", "pairs_into", "cdist_into",
-           "pair_classes", "query_classes", "DeviceArray", "BackendUnavailable"]
+__all__ = ["available", "distance", "pairwise", "cdist", "Packed", "Plan", "pairs_plan",
+           "cdist_plan", "pairs_into", "cdist_into", "pair_classes", "query_classes",
+           "DeviceArray", "BackendUnavailable"]
 
 
 def available() -> bool:
@@ -98,53 +99,69 @@ def _idx_buffers(groups):
     return [DeviceBuffer.from_host(g) if len(g) else None for g in groups]
 
 
+class Plan:
+    """Class split of one call shape, with its index lists on the device.
+    Built once per (A, B) pair set or per query set and reused by every call
+    on the same resident inputs, so a repeated call is kernel launches only."""
+
+    def __init__(self, groups, rest, max_len):
+        self.groups = groups
+        self.bufs = _idx_buffers(groups)
+        self.rest = rest
+        self.rest_buf = DeviceBuffer.from_host(rest) if len(rest) else None
+        self.max_len = max_len
+
+
+def pairs_plan(A: Packed, B: Packed) -> Plan:
+    groups, rest = pair_classes(A.h_off, B.h_off)
+    mx = int(max(A.lengths[rest].max(), B.lengths[rest].max())) if len(rest) else 0
+    return Plan(groups, rest, mx)
+
+
+def cdist_plan(Q: Packed, C: Packed) -> Plan:
+    groups, rest = query_classes(Q.h_off)
+    mx = int(max(Q.lengths[rest].max(), C.max_len)) if len(rest) else 0
+    return Plan(groups, rest, mx)
+
+
 def pairs_into(A: Packed, B: Packed, out: DeviceArray, k: int = -1, trim: bool = True,
-               stream=None, keep=None):
+               stream=None, plan: Plan | None = None) -> Plan:
     """Launch the plain pairs kernels: out[i] = d(A[i], B[i]) (min(d, k+1) for
-    k >= 0) in out's dtype. Asynchronous on `stream`; the index buffers are
-    appended to `keep` (they must live until the work is done)."""
+    k >= 0) in out's dtype. Asynchronous on `stream`; keep the returned plan
+    alive until the work is done (it owns the index lists)."""
     if A.n != B.n:
         raise ValueError(f"length mismatch: {A.n} vs {B.n}")
+    if plan is None:
+        plan = pairs_plan(A, B)
     if A.n == 0:
-        return
+        return plan
     dt = DTYPE_CODE[out.dtype]
-    groups, rest = pair_classes(A.h_off, B.h_off)
-    bufs = _idx_buffers(groups)
-    for cap, g, b in zip(PAIR_CAPS, groups, bufs):
+    for cap, g, b in zip(PAIR_CAPS, plan.groups, plan.bufs):
         if b is not None:
             call("bl_pairs_class", cap, A.d_buf.ptr, A.d_off.ptr, B.d_buf.ptr, B.d_off.ptr, b.ptr,
                  len(g), out.ptr, dt, int(trim), int(k), stream)
-    if len(rest):
-        rb = DeviceBuffer.from_host(rest)
-        bufs.append(rb)
-        mx = int(max(A.lengths[rest].max(), B.lengths[rest].max()))
-        call("bl_long", 1, A.d_buf.ptr, A.d_off.ptr, B.d_buf.ptr, B.d_off.ptr, rb.ptr, len(rest),
-             0, out.ptr, dt, int(trim), int(k), mx, stream)
-    if keep is not None:
-        keep.extend(bufs)
-    return bufs
+    if plan.rest_buf is not None:
+        call("bl_long", 1, A.d_buf.ptr, A.d_off.ptr, B.d_buf.ptr, B.d_off.ptr, plan.rest_buf.ptr,
+             len(plan.rest), 0, out.ptr, dt, int(trim), int(k), plan.max_len, stream)
+    return plan
 
 
-def cdist_into(Q: Packed, C: Packed, out: DeviceArray, k: int = -1, stream=None, keep=None):
+def cdist_into(Q: Packed, C: Packed, out: DeviceArray, k: int = -1, stream=None,
+               plan: Plan | None = None) -> Plan:
     """Launch the plain cdist kernels into the row-major (Q.n, C.n) `out`."""
+    if plan is None:
+        plan = cdist_plan(Q, C)
     if Q.n == 0 or C.n == 0:
-        return
+        return plan
     dt = DTYPE_CODE[out.dtype]
-    groups, rest = query_classes(Q.h_off)
-    bufs = _idx_buffers(groups)
-    for nw, g, b in zip(CROSS_NWS, groups, bufs):
+    for nw, g, b in zip(CROSS_NWS, plan.groups, plan.bufs):
         if b is not None:
             call("bl_cdist_class", nw, Q.d_buf.ptr, Q.d_off.ptr, b.ptr, len(g), C.d_buf.ptr,
                  C.d_off.ptr, C.n, out.ptr, dt, int(k), stream)
-    if len(rest):
-        rb = DeviceBuffer.from_host(rest)
-        bufs.append(rb)
-        mx = int(max(Q.lengths[rest].max(), C.max_len))
-        call("bl_long", 0, Q.d_buf.ptr, Q.d_off.ptr, C.d_buf.ptr, C.d_off.ptr, rb.ptr, len(rest),
-             C.n, out.ptr, dt, 0, int(k), mx, stream)
-    if keep is not None:
-        keep.extend(bufs)
-    return bufs
+    if plan.rest_buf is not None:
+        call("bl_long", 0, Q.d_buf.ptr, Q.d_off.ptr, C.d_buf.ptr, C.d_off.ptr, plan.rest_buf.ptr,
+             len(plan.rest), C.n, out.ptr, dt, 0, int(k), plan.max_len, stream)
+    return plan
 
 
 def _finish(out: DeviceArray, return_device: bool, stream=None):
@@ -167,9 +184,10 @@ def pairwise(a, b, max_dist=None, dtype=np.int32, return_device: bool = False):
     if n == 0:
         return out if return_device else np.empty(0, dt)
     A, B = Packed(*pa), Packed(*pb)
-    keep = []
-    pairs_into(A, B, out, k, True, None, keep)
-    return _finish(out, return_device)
+    plan = pairs_into(A, B, out, k, True, None)
+    res = _finish(out, return_device)
+    del plan
+    return res
 
 
 def cdist(queries, choices, max_dist=None, dtype=np.int32, return_device: bool = False):
@@ -183,9 +201,10 @@ def cdist(queries, choices, max_dist=None, dtype=np.int32, return_device: bool =
     if nq == 0 or nc == 0:
         return out if return_device else np.empty((nq, nc), dt)
     Q, C = Packed(*pq), Packed(*pc)
-    keep = []
-    cdist_into(Q, C, out, k, None, keep)
-    return _finish(out, return_device)
+    plan = cdist_into(Q, C, out, k, None)
+    res = _finish(out, return_device)
+    del plan
+    return res
 
 
 def distance(a, b, max_dist=None) -> int:

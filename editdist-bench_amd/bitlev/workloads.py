@@ -1,0 +1,95 @@
+"""Seeded synthetic inputs for the benchmark workloads (libbl_workloads.so).
+
+Each workload returns packed host sets (uint8 buffer, int64 offsets). The
+generator is counter-based per string, so results do not depend on the thread
+count. Definitions are in csrc/workloads.c; sizes and parameters of the five
+benchmark configurations are in WORKLOADS below.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+
+SEED = 20261203
+
+# name: (generator id, kind, sizes, max_dist, dtype)
+WORKLOADS = {
+    "pairs-L32-acgt": dict(gen=1, kind="pairs", n=65_536, k=None, dtype="int32"),
+    "cdist-L64-k2": dict(gen=2, kind="cdist", nq=1_024, nc=1_000_000, k=2, dtype="uint8"),
+    "pairs-uneven-1-256": dict(gen=3, kind="pairs", n=16_777_216, k=None, dtype="uint16"),
+    "cdist-L200-256-k8": dict(gen=4, kind="cdist", nq=256, nc=4_194_304, k=8, dtype="uint8"),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = ctypes.CDLL(os.path.join(HERE, "libbl_workloads.so"))
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        L.blw_lengths.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int64,
+                                  ctypes.c_int64, i64p, ctypes.c_int]
+        L.blw_fill.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int64,
+                               ctypes.c_int64, i64p, ctypes.POINTER(ctypes.c_uint8), ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _threads(threads):
+    if threads:
+        return int(threads)
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return 4
+
+
+def generate(gen: int, n: int, seed: int = SEED, set_: int = 0, nq: int = 0, threads=None):
+    """Strings [0, n) of one set of generator `gen` -> (buf, off)."""
+    t = _threads(threads)
+    lens = np.empty(n, np.int64)
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    rc = lib().blw_lengths(gen, seed, set_, n, nq, lens.ctypes.data_as(i64p), t)
+    if rc:
+        raise ValueError(f"workload generator {gen}: error {rc}")
+    off = np.zeros(n + 1, np.int64)
+    np.cumsum(lens, out=off[1:])
+    buf = np.empty(max(int(off[-1]), 1), np.uint8)
+    rc = lib().blw_fill(gen, seed, set_, n, nq, off.ctypes.data_as(i64p),
+                        buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), t)
+    if rc:
+        raise ValueError(f"workload generator {gen}: error {rc}")
+    return buf, off
+
+
+def pairs(gen: int, n: int, seed: int = SEED, threads=None):
+    """(a, b) packed sets of a pairs workload."""
+    return generate(gen, n, seed, 0, 0, threads), generate(gen, n, seed, 1, 0, threads)
+
+
+def cdist(gen: int, nq: int, nc: int, seed: int = SEED, threads=None):
+    """(queries, choices) packed sets of a cdist workload."""
+    return generate(gen, nq, seed, 0, nq, threads), generate(gen, nc, seed, 1, nq, threads)
+
+
+def subset(packed, idx):
+    """Host (buf, off) of a subset of a packed set, in the order of idx."""
+    buf, off = packed
+    idx = np.asarray(idx, np.int64)
+    lens = off[idx + 1] - off[idx]
+    o = np.zeros(len(idx) + 1, np.int64)
+    np.cumsum(lens, out=o[1:])
+    out = np.empty(max(int(o[-1]), 1), np.uint8)
+    for j, i in enumerate(idx):
+        out[o[j]:o[j + 1]] = buf[off[i]:off[i + 1]]
+    return out, o
+
+
+def head(packed, n):
+    """The first n strings of a packed set (a view of the buffer)."""
+    buf, off = packed
+    return buf[: max(int(off[n]), 1)], off[: n + 1].copy()

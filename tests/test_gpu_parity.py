@@ -26,17 +26,19 @@ def hip():
 def gpu_pairs(hip, pa, pb, k=-1, trim=True, dtype=np.int32):
     A, B = hip.Packed(*pa), hip.Packed(*pb)
     out = hip.DeviceArray.empty((A.n,), dtype)
-    keep = []
-    hip.pairs_into(A, B, out, k, trim, None, keep)
-    return out.get()
+    plan = hip.pairs_into(A, B, out, k, trim, None)
+    res = out.get()
+    del plan
+    return res
 
 
 def gpu_cross(hip, pq, pc, k=-1, dtype=np.int32):
     Q, C = hip.Packed(*pq), hip.Packed(*pc)
     out = hip.DeviceArray.empty((Q.n, C.n), dtype)
-    keep = []
-    hip.cdist_into(Q, C, out, k, None, keep)
-    return out.get()
+    plan = hip.cdist_into(Q, C, out, k, None)
+    res = out.get()
+    del plan
+    return res
 
 
 def test_golden_pairs_every_k_trim_dtype(hip, golden):
