@@ -223,10 +223,14 @@ def cpu_leg(res, args, wl, k, dt, hx, hy, out, plain):
         run_cpu(W.head(hx, rows1), 1)
         t_one = time.perf_counter() - t
         equal = bool(np.array_equal(O.saturate(d, dt), gpu_rows))
-        _, cols = O.cross(sq, hy, k, threads, cols=True)
         m = np.diff(sq[1]).astype(np.float64)
-        cells_sample = float((cols.astype(np.float64) * m[:, None]).sum())
-        cells_total = cells_sample * wl["nq"] / rows
+        cells = {}
+        for rule in ("diagonal", "reference"):
+            O.count_rule(rule)
+            _, cols = O.cross(sq, hy, k, threads, cols=True)
+            cells[rule] = float((cols.astype(np.float64) * m[:, None]).sum()) * wl["nq"] / rows
+        O.count_rule("reference")
+        cells_total, cells_ref = cells["diagonal"], cells["reference"]
         pairs_sample, pairs_one = rows * wl["nc"], rows1 * wl["nc"]
         sample_txt = (f"first {rows} of {wl['nq']} queries x all {wl['nc']:,} choices "
                       f"(all cores); first {rows1} queries (1 thread)")
@@ -249,7 +253,7 @@ def cpu_leg(res, args, wl, k, dt, hx, hy, out, plain):
         t_one = time.perf_counter() - t
         equal = bool(np.array_equal(O.saturate(d, dt), gpu_part))
         la, lb = np.diff(hx[1]).astype(np.float64), np.diff(hy[1]).astype(np.float64)
-        cells_total = float((la * lb).sum())          # exact scan: untrimmed m x n
+        cells_total = cells_ref = float((la * lb).sum())   # exact scan: untrimmed m x n
         pairs_sample, pairs_one = rows, rows1
         sample_txt = f"first {rows:,} of {n:,} pairs (all cores); first {rows1:,} (1 thread)"
 
@@ -278,13 +282,22 @@ def cpu_leg(res, args, wl, k, dt, hx, hy, out, plain):
         "frac": round(achieved / VALU_PEAK_TOPS, 4),
         "traffic": None,
         "cells_per_launch": cells_total,
+        "cells_reference_rule": cells_ref,
         "gcups": round(cells_total / kernel_s / 1e9, 1),
+        "gcups_reference_rule": round(cells_ref / kernel_s / 1e9, 1),
         "ops_per_cell": OPS_PER_CELL,
-        "kernel": "cdist_class_kernel<1>" if wl["kind"] == "cdist" else "pairs_class_kernel",
+        "kernel": kernel_name(wl, res.get("form", "plain")),
     }
     if not equal:
         print("bench.py: WARNING: GPU results differ from the CPU reference on the sample",
               file=sys.stderr)
+
+
+def kernel_name(wl, form):
+    if wl["kind"] == "cdist":
+        nw = {2: 1, 4: 4}.get(wl["gen"], 0)
+        return f"cdist_class_kernel<{nw}>" if form == "plain" else f"cdist_batched_kernel"
+    return "pairs_class_kernel<1,2,4> (5 class launches)" if form == "plain" else "pairs_batched_kernel"
 
 
 def _copy_rows(out, host):

@@ -118,15 +118,19 @@ BL_HD u32 ld32(const u8 *p)
 }
 
 /* s[i .. i+4) as a little-endian u32, 0 <= i < n. Bytes at or past n are
- * unspecified. Only aligned dwords holding a byte of s[0..n) are read. */
+ * unspecified. Only aligned dwords holding a byte of s[0..n) are read.
+ * (Pointer arithmetic, not integer casts, so the compiler keeps the global
+ * address space and emits global loads.) */
 BL_HD u32 load4(const u8 *s, int i, int n)
 {
-    const uintptr_t a = (uintptr_t)(s + i);
-    const u8 *al = (const u8 *)(a & ~(uintptr_t)3);
-    const u8 *last = (const u8 *)((uintptr_t)(s + n - 1) & ~(uintptr_t)3);
+    const u8 *p = s + i;
+    const u32 sh = (u32)((uintptr_t)p & 3);
+    const u8 *al = p - sh;
+    const u8 *e = s + (n - 1);
+    const u8 *last = e - ((uintptr_t)e & 3);
     const u32 lo = ld32(al);
     const u32 hi = ld32(al + 4 <= last ? al + 4 : last);
-    return align_bytes(hi, lo, (u32)(a & 3));
+    return align_bytes(hi, lo, sh);
 }
 
 /* Sequential reader of a string, 4 bytes per call, one aligned load each. */
@@ -135,10 +139,10 @@ struct ByteStream {
     u32 cur, sh;
     BL_HDM void open(const u8 *s, int n)
     {
-        const uintptr_t a = (uintptr_t)s;
-        al = (const u8 *)(a & ~(uintptr_t)3);
-        last = (const u8 *)((uintptr_t)(s + (n > 0 ? n - 1 : 0)) & ~(uintptr_t)3);
-        sh = (u32)(a & 3);
+        sh = (u32)((uintptr_t)s & 3);
+        al = s - sh;
+        const u8 *e = s + (n > 0 ? n - 1 : 0);
+        last = e - ((uintptr_t)e & 3);
         cur = n > 0 ? ld32(al) : 0;
     }
     /* The first call returns bytes 0..3, the next 4..7, ... (the caller stops
@@ -272,6 +276,107 @@ BL_HD void trim_pair(const u8 *&a, int &la, const u8 *&b, int &lb)
     la -= p + s; lb -= p + s;
 }
 
+/* ------------------------------------------------------------ exit bound */
+/* After jj text columns, every alignment still has to cross column jj, so
+ *   d >= min_i D[i][jj] + |(m - i) - (n - jj)|,
+ * and because D changes by at most 1 per row the minimum sits on the final
+ * diagonal i0 = jj + m - n: d >= D[i0][jj] (d >= n - m when i0 <= 0). With
+ * D[0][jj] = jj, D[i0][jj] = jj + popcount(Pv & rows<i0) - popcount(Mv & rows<i0).
+ * This dominates the reference's bottom-row test (score - (n-1-j) > k,
+ * csrc/myers.c:117, :127), so exits come earlier and results are identical:
+ * when the bound exceeds k the answer is k + 1 either way. At jj = n the
+ * bound is the exact distance. */
+template <int NW>
+BL_HD int diag_bound(const u64 (&Pv)[NW], const u64 (&Mv)[NW], int nw, int jj, int m, int n)
+{
+    const int i0 = jj + m - n;
+    if (i0 <= 0) return n - m;
+    int s = jj;
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        if (w < nw) {
+            const int r = i0 - 64 * w;
+            const u64 mk = r >= 64 ? ~(u64)0 : r <= 0 ? (u64)0 : (((u64)1 << r) - 1);
+            s += popc64(Pv[w] & mk) - popc64(Mv[w] & mk);
+        }
+    }
+    return s;
+}
+
+/* One text column over the first nw words, Eq words given. */
+template <int NW>
+BL_HD void column(u64 (&Pv)[NW], u64 (&Mv)[NW], int nw, const u64 (&E)[NW])
+{
+    u32 hp = 1, hn = 0;                    /* row 0 of a global alignment: +1 per column */
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        if (NW == 1 || w < nw) {
+            u64 Ph, Mh;
+            word_step(Pv[w], Mv[w], E[w], hp, hn, Ph, Mh);
+            hp = (u32)(Ph >> 63);
+            hn = (u32)(Mh >> 63);
+        }
+    }
+}
+
+/* Scan text t (n >= 1 bytes) against a pattern of m rows held in nw <= NW
+ * words; eq(c, w) yields Eq of word w for text byte c. Four columns per
+ * aligned text load (the next load is issued before the current columns
+ * run), the exit bound tested after each group of four.
+ * EAGER: fetch the Eq words of all four columns before stepping (table
+ * lookups: the four loads overlap); otherwise compute each column's Eq just
+ * before its step (register bit-planes: nothing to overlap, fewer live VGPRs).
+ * k < 0: exact distance; k >= 0: min(d, k + 1). */
+template <int NW, bool EAGER, typename EqFn>
+BL_HD int scan_text(int m, int nw, const u8 *t, int n, int k, EqFn eq)
+{
+    u64 Pv[NW], Mv[NW];
+#pragma unroll
+    for (int w = 0; w < NW; w++) { Pv[w] = ~(u64)0; Mv[w] = 0; }
+    ByteStream ts;
+    ts.open(t, n);
+    u32 c4 = ts.next4();
+    int j = 0;
+    for (;;) {
+        const u32 c4n = ts.next4();
+        const int cnt = n - j < 4 ? n - j : 4;
+        if (cnt == 4) {
+            if (EAGER) {
+                u64 E[4][NW];
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int w = 0; w < NW; w++) E[u][w] = eq((c4 >> (8 * u)) & 0xFFu, w);
+#pragma unroll
+                for (int u = 0; u < 4; u++) column<NW>(Pv, Mv, nw, E[u]);
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    u64 E[NW];
+#pragma unroll
+                    for (int w = 0; w < NW; w++) E[w] = eq((c4 >> (8 * u)) & 0xFFu, w);
+                    column<NW>(Pv, Mv, nw, E);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < 3; u++) {
+                if (u < cnt) {
+                    u64 E[NW];
+#pragma unroll
+                    for (int w = 0; w < NW; w++) E[w] = eq((c4 >> (8 * u)) & 0xFFu, w);
+                    column<NW>(Pv, Mv, nw, E);
+                }
+            }
+        }
+        j += cnt;
+        const int bound = diag_bound<NW>(Pv, Mv, nw, j, m, n);
+        if (j >= n) return (k >= 0 && bound > k) ? k + 1 : bound;
+        if (k >= 0 && bound > k) return k + 1;
+        c4 = c4n;
+    }
+}
+
 /* ------------------------------------------------------------ pair lane */
 /* Distance of one pair whose shorter string (after the optional trim) has at
  * most 64*NW bytes: the pattern lives in registers as bit-planes.
@@ -291,57 +396,16 @@ BL_HD int pair_lane(const u8 *a, int la, const u8 *b, int lb, int trim, int k)
     u64 pl[NW][8];
 #pragma unroll
     for (int w = 0; w < NW; w++) {
+#pragma unroll
+        for (int b8 = 0; b8 < 8; b8++) pl[w][b8] = 0;
         if (64 * w < la) build_planes(a + 64 * w, la - 64 * w, pl[w]);
     }
     const int nw = (la + 63) >> 6;
-    const u64 lastmask = row_mask(la - 64 * (nw - 1));
-    u64 Pv[NW], Mv[NW];
-#pragma unroll
-    for (int w = 0; w < NW; w++) { Pv[w] = ~(u64)0; Mv[w] = 0; }
-
-    ByteStream ts;
-    ts.open(b, lb);
-    for (int j = 0; j < lb; j += 4) {
-        const u32 c4 = ts.next4();
-        const int cnt = lb - j < 4 ? lb - j : 4;
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            if (u < cnt) {
-                u32 m[8];
-                text_masks((c4 >> (8 * u)) & 0xFFu, m);
-                u32 hp = 1, hn = 0;
-#pragma unroll
-                for (int w = 0; w < NW; w++) {
-                    if (w < nw) {
-                        u64 Ph, Mh;
-                        word_step(Pv[w], Mv[w], eq_planes(pl[w], m), hp, hn, Ph, Mh);
-                        hp = (u32)(Ph >> 63);
-                        hn = (u32)(Mh >> 63);
-                    }
-                }
-            }
-        }
-        if (k >= 0) {
-            int sc = j + cnt;
-#pragma unroll
-            for (int w = 0; w < NW; w++) {
-                if (w < nw) {
-                    const u64 mk = w == nw - 1 ? lastmask : ~(u64)0;
-                    sc += popc64(Pv[w] & mk) - popc64(Mv[w] & mk);
-                }
-            }
-            if (sc - (lb - j - cnt) > k) return k + 1;
-        }
-    }
-    int d = lb;
-#pragma unroll
-    for (int w = 0; w < NW; w++) {
-        if (w < nw) {
-            const u64 mk = w == nw - 1 ? lastmask : ~(u64)0;
-            d += popc64(Pv[w] & mk) - popc64(Mv[w] & mk);
-        }
-    }
-    return d;
+    return scan_text<NW, false>(la, nw, b, lb, k, [&](u32 c, int w) {
+        u32 m8[8];
+        text_masks(c, m8);
+        return eq_planes(pl[w], m8);
+    });
 }
 
 /* ------------------------------------------------------------ Peq lane */
@@ -355,52 +419,7 @@ BL_HD int peq_lane(PeqPtr peq, int m, const u8 *t, int n, int k)
     if (k >= 0 && (n > m ? n - m : m - n) > k) return k + 1;
     if (n == 0) return m;
     const int nw = (m + 63) >> 6;
-    const u64 lastmask = row_mask(m - 64 * (nw - 1));
-    u64 Pv[NW], Mv[NW];
-#pragma unroll
-    for (int w = 0; w < NW; w++) { Pv[w] = ~(u64)0; Mv[w] = 0; }
-    ByteStream ts;
-    ts.open(t, n);
-    for (int j = 0; j < n; j += 4) {
-        const u32 c4 = ts.next4();
-        const int cnt = n - j < 4 ? n - j : 4;
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            if (u < cnt) {
-                const u32 c = (c4 >> (8 * u)) & 0xFFu;
-                u32 hp = 1, hn = 0;
-#pragma unroll
-                for (int w = 0; w < NW; w++) {
-                    if (w < nw) {
-                        u64 Ph, Mh;
-                        word_step(Pv[w], Mv[w], peq[c * NW + w], hp, hn, Ph, Mh);
-                        hp = (u32)(Ph >> 63);
-                        hn = (u32)(Mh >> 63);
-                    }
-                }
-            }
-        }
-        if (k >= 0) {
-            int sc = j + cnt;
-#pragma unroll
-            for (int w = 0; w < NW; w++) {
-                if (w < nw) {
-                    const u64 mk = w == nw - 1 ? lastmask : ~(u64)0;
-                    sc += popc64(Pv[w] & mk) - popc64(Mv[w] & mk);
-                }
-            }
-            if (sc - (n - j - cnt) > k) return k + 1;
-        }
-    }
-    int d = n;
-#pragma unroll
-    for (int w = 0; w < NW; w++) {
-        if (w < nw) {
-            const u64 mk = w == nw - 1 ? lastmask : ~(u64)0;
-            d += popc64(Pv[w] & mk) - popc64(Mv[w] & mk);
-        }
-    }
-    return d;
+    return scan_text<NW, true>(m, nw, t, n, k, [&](u32 c, int w) { return (u64)peq[c * NW + w]; });
 }
 
 /* Empty cross query: d = n, or k + 1 when k >= 0 and n > k

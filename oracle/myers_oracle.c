@@ -120,6 +120,27 @@ static void orc_peq_unload(orc_peq *q, const uint8_t *p, int m)
  * k >= 0: run_text_k (csrc/myers.c:107-130): |n - m| > k answers k + 1 with no
  * column evaluated; after column j, score - (n - 1 - j) > k answers k + 1.
  * *cols (if given) receives the number of text columns evaluated. */
+/* Lower bound on the final distance after jj columns: the DP cell on the
+ * final diagonal, D[jj + m - n][jj] (n - m when that row is <= 0). Used only
+ * to count the columns an exit test on this bound would evaluate. */
+static int orc_diag_bound(const orc_peq *q, int m, int n, int jj)
+{
+    const int i0 = jj + m - n;
+    if (i0 <= 0) return n - m;
+    int s = jj;
+    for (int w = 0; w * 64 < i0; w++) {
+        const int r = i0 - 64 * w;
+        const u64 mk = r >= 64 ? ~(u64)0 : (((u64)1 << r) - 1);
+        s += __builtin_popcountll(q->pv[w] & mk) - __builtin_popcountll(q->mv[w] & mk);
+    }
+    return s;
+}
+
+/* Which columns *cols counts: the reference's bottom-row exit test, or the
+ * final-diagonal bound (same results, earlier exits). */
+static int g_count_rule = 0;
+void orc_set_count_rule(int rule) { g_count_rule = rule; }
+
 static int orc_scan(orc_peq *q, int m, const uint8_t *t, int n, int k, long long *cols)
 {
     if (cols) *cols = 0;
@@ -133,13 +154,21 @@ static int orc_scan(orc_peq *q, int m, const uint8_t *t, int n, int k, long long
         int h = 1;                               /* row 0 of a global alignment: +1 per column */
         for (int w = 0; w + 1 < W; w++) h = orc_step(&q->pv[w], &q->mv[w], row[w], h, 63);
         score += orc_step(&q->pv[W - 1], &q->mv[W - 1], row[W - 1], h, last);
-        if (cols) *cols = j + 1;
+        if (cols && *cols >= 0) {
+            *cols = j + 1;
+            if (g_count_rule == 1 && k >= 0 && orc_diag_bound(q, m, n, j + 1) > k)
+                *cols = -(j + 1);               /* diagonal rule fired here; keep scanning */
+        }
 #if ORC_MUTANT == 4
         if (k >= 0 && score - (n - 2 - j) > k) return k + 1;   /* exit rule off by one */
 #else
-        if (k >= 0 && score - (n - 1 - j) > k) return k + 1;
+        if (k >= 0 && score - (n - 1 - j) > k) {
+            if (cols && *cols < 0) *cols = -*cols;
+            return k + 1;
+        }
 #endif
     }
+    if (cols && *cols < 0) *cols = -*cols;
     return score;
 }
 

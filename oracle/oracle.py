@@ -72,6 +72,8 @@ def lib():
         L.orc_dp.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
         L.orc_dp_pairs.argtypes = [_u8p, _i64p, _u8p, _i64p, ctypes.c_int64, _i32p, ctypes.c_int]
         L.orc_saturate.argtypes = [_i32p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p]
+        L.orc_set_count_rule.argtypes = [ctypes.c_int]
+        L.orc_set_count_rule.restype = None
         for f in (L.orc_pairs, L.orc_cross, L.orc_dp, L.orc_dp_pairs, L.orc_saturate):
             f.restype = ctypes.c_int
         _lib = L
@@ -100,6 +102,12 @@ def pairs(pa, pb, k: int = -1, trim: bool = True, threads: int = 1, cols: bool =
                            _ptr(out, _i32p), int(trim), int(k), _ptr(c, _llp), int(threads)),
            "orc_pairs")
     return (out, c) if cols else out
+
+
+def count_rule(rule: str):
+    """Column counting for cols=True: "reference" (bottom-row exit test,
+    csrc/myers.c:117, :127) or "diagonal" (final-diagonal bound)."""
+    lib().orc_set_count_rule({"reference": 0, "diagonal": 1}[rule])
 
 
 def cross(pq, pc, k: int = -1, threads: int = 1, cols: bool = False):
