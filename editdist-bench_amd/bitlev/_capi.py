@@ -46,8 +46,25 @@ SIGNATURES = {
     "bl_pairs_class": [_int, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _int, _int, _int, _vp],
     "bl_cdist_class": [_int, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _int, _int, _vp],
     "bl_long": [_int, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _int, _int, _int, _i64, _vp],
+    "bl_cdist_workspace_size": [],
+    "bl_cdist": [_vp, _vp, _i64, _vp, _vp, _i64, _int, _vp, _int, _int, _vp, _sz, _vp],
+    "bl_cdist_tuned": [_vp, _vp, _i64, _vp, _vp, _i64, _int, _vp, _int, _int, _vp, _sz, _vp, _vp],
+    "bl_pairs_workspace_size": [_i64, _i64],
+    "bl_pairs": [_vp, _vp, _vp, _vp, _i64, _i64, _vp, _int, _int, _int, _vp, _sz, _vp],
+    "bl_pairs_tuned": [_vp, _vp, _vp, _vp, _i64, _i64, _vp, _int, _int, _int, _vp, _sz, _vp, _vp],
 }
-_RESTYPE = {"bl_last_error": _c.c_char_p}
+_RESTYPE = {"bl_last_error": _c.c_char_p, "bl_cdist_workspace_size": _sz,
+            "bl_pairs_workspace_size": _sz}
+
+
+class CdistParams(ctypes.Structure):
+    """bl_cdist_params (include/bitlev_hip.h)."""
+    _fields_ = [("waves_per_cu", _int), ("tile", _i64), ("reserved", _int * 4)]
+
+
+class PairsParams(ctypes.Structure):
+    """bl_pairs_params (include/bitlev_hip.h)."""
+    _fields_ = [("waves_per_cu", _int), ("block", _int), ("bin_width", _int), ("reserved", _int * 4)]
 
 
 class BackendUnavailable(RuntimeError):

@@ -13,25 +13,34 @@ hands to its CPU core (pairs whose shorter string exceeds 256 bytes and cdist
 queries longer than 512 bytes): those run on the wavefront kernel. There is no
 CPU fallback; without the built library or a GPU the calls raise.
 
-This module is the plain (per-class) form: the host splits pairs and queries
-into length classes exactly as the reference does (bitlev/gpu.py:85-86,
-:116-118) and launches one kernel per class.
+Two forms compute the same results:
+  * "batched" (default): one call per batch, no host round trip -- on-device
+    length bucketing and one persistent launch (bl_pairs / bl_cdist);
+  * "plain": the host splits pairs and queries into the reference's length
+    classes (bitlev/gpu.py:85-86, :116-118) and launches one kernel per class.
+Select with the module attribute FORM or the `form` argument of the *_into
+functions.
 """
 from __future__ import annotations
 
 import numpy as np
 
 from . import _edpy
-from ._capi import BackendUnavailable, call, lib
+import ctypes
+
+from ._capi import BackendUnavailable, CdistParams, PairsParams, call, lib
 from ._common import DTYPE_CODE, as_list, check_dtype, check_k
 from .device import DeviceArray, DeviceBuffer
 
 PAIR_CAPS = (16, 32, 64, 128, 256)   # pairs kernels: shorter string <= CAP
 CROSS_NWS = (1, 2, 4, 8)             # cdist kernels: query <= 64 * NW
+BATCHED_MAX_QUERY = 512              # longer cdist queries take the wavefront kernel
+FORM = "batched"                     # default form of pairwise() / cdist()
 
 __all__ = ["available", "distance", "pairwise", "cdist", "Packed", "Plan", "pairs_plan",
-           "cdist_plan", "pairs_into", "cdist_into", "pair_classes", "query_classes",
-           "DeviceArray", "BackendUnavailable"]
+           "cdist_plan", "pairs_into", "cdist_into", "pairs_batched_into", "cdist_batched_into",
+           "run_pairs", "run_cdist", "Workspace", "pair_classes", "query_classes",
+           "DeviceArray", "BackendUnavailable", "FORM"]
 
 
 def available() -> bool:
@@ -164,6 +173,103 @@ def cdist_into(Q: Packed, C: Packed, out: DeviceArray, k: int = -1, stream=None,
     return plan
 
 
+class Workspace:
+    """Device workspace of the batched entry points, grown on demand and
+    reused across calls (calls sharing one workspace must be ordered on one
+    stream)."""
+
+    def __init__(self):
+        self.buf = None
+
+    def get(self, nbytes: int) -> DeviceBuffer:
+        if self.buf is None or self.buf.nbytes < nbytes:
+            self.buf = DeviceBuffer(max(int(nbytes), 256))
+        return self.buf
+
+
+_default_ws = {}
+
+
+def _workspace(stream):
+    ws = _default_ws.get(stream)
+    if ws is None:
+        ws = _default_ws[stream] = Workspace()
+    return ws
+
+
+def _cdist_params(params):
+    if params is None:
+        return None
+    p = CdistParams()
+    p.waves_per_cu = int(params.get("waves_per_cu", 0))
+    p.tile = int(params.get("tile", 0))
+    return ctypes.byref(p)
+
+
+def _pairs_params(params):
+    if params is None:
+        return None
+    p = PairsParams()
+    p.waves_per_cu = int(params.get("waves_per_cu", 0))
+    p.block = int(params.get("block", 0))
+    p.bin_width = int(params.get("bin_width", 0))
+    return ctypes.byref(p)
+
+
+def pairs_batched_into(A: Packed, B: Packed, out: DeviceArray, k: int = -1, trim: bool = True,
+                       stream=None, workspace: Workspace | None = None, params=None):
+    """One bl_pairs pipeline (bucketing + persistent launch) into `out`."""
+    if A.n != B.n:
+        raise ValueError(f"length mismatch: {A.n} vs {B.n}")
+    if A.n == 0:
+        return
+    max_len = max(A.max_len, B.max_len)
+    ws = (workspace or _workspace(stream)).get(lib().bl_pairs_workspace_size(A.n, max_len))
+    call("bl_pairs_tuned", A.d_buf.ptr, A.d_off.ptr, B.d_buf.ptr, B.d_off.ptr, A.n, max_len,
+         out.ptr, DTYPE_CODE[out.dtype], int(trim), int(k), ws.ptr, ws.nbytes, _pairs_params(params),
+         stream)
+
+
+def cdist_batched_into(Q: Packed, C: Packed, out: DeviceArray, k: int = -1, stream=None,
+                       workspace: Workspace | None = None, params=None, plan: Plan | None = None):
+    """One bl_cdist persistent launch into the row-major (Q.n, C.n) `out`;
+    queries above 512 bytes follow on the wavefront kernel (bl_long)."""
+    if Q.n == 0 or C.n == 0:
+        return plan
+    ws = (workspace or _workspace(stream)).get(lib().bl_cdist_workspace_size())
+    mq = min(Q.max_len, BATCHED_MAX_QUERY)
+    call("bl_cdist_tuned", Q.d_buf.ptr, Q.d_off.ptr, Q.n, C.d_buf.ptr, C.d_off.ptr, C.n, mq,
+         out.ptr, DTYPE_CODE[out.dtype], int(k), ws.ptr, ws.nbytes, _cdist_params(params), stream)
+    if Q.max_len > BATCHED_MAX_QUERY:
+        if plan is None:
+            plan = cdist_plan(Q, C)
+        call("bl_long", 0, Q.d_buf.ptr, Q.d_off.ptr, C.d_buf.ptr, C.d_off.ptr, plan.rest_buf.ptr,
+             len(plan.rest), C.n, out.ptr, DTYPE_CODE[out.dtype], 0, int(k), plan.max_len, stream)
+    return plan
+
+
+def run_pairs(A: Packed, B: Packed, out: DeviceArray, k: int = -1, trim: bool = True,
+              stream=None, form: str | None = None, plan=None, workspace=None, params=None):
+    """pairs in the chosen form; returns what must stay alive until the work is done."""
+    form = form or FORM
+    if form == "batched":
+        pairs_batched_into(A, B, out, k, trim, stream, workspace, params)
+        return None
+    if form == "plain":
+        return pairs_into(A, B, out, k, trim, stream, plan)
+    raise ValueError("form must be 'batched' or 'plain'")
+
+
+def run_cdist(Q: Packed, C: Packed, out: DeviceArray, k: int = -1, stream=None,
+              form: str | None = None, plan=None, workspace=None, params=None):
+    form = form or FORM
+    if form == "batched":
+        return cdist_batched_into(Q, C, out, k, stream, workspace, params, plan)
+    if form == "plain":
+        return cdist_into(Q, C, out, k, stream, plan)
+    raise ValueError("form must be 'batched' or 'plain'")
+
+
 def _finish(out: DeviceArray, return_device: bool, stream=None):
     call("bl_stream_sync", stream)
     return out if return_device else out.get(stream)
@@ -184,9 +290,9 @@ def pairwise(a, b, max_dist=None, dtype=np.int32, return_device: bool = False):
     if n == 0:
         return out if return_device else np.empty(0, dt)
     A, B = Packed(*pa), Packed(*pb)
-    plan = pairs_into(A, B, out, k, True, None)
+    keep = run_pairs(A, B, out, k, True, None)
     res = _finish(out, return_device)
-    del plan
+    del keep
     return res
 
 
@@ -201,9 +307,9 @@ def cdist(queries, choices, max_dist=None, dtype=np.int32, return_device: bool =
     if nq == 0 or nc == 0:
         return out if return_device else np.empty((nq, nc), dt)
     Q, C = Packed(*pq), Packed(*pc)
-    plan = cdist_into(Q, C, out, k, None)
+    keep = run_cdist(Q, C, out, k, None)
     res = _finish(out, return_device)
-    del plan
+    del keep
     return res
 
 

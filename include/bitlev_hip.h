@@ -99,6 +99,60 @@ int bl_long(int is_pair, const uint8_t *d_buf_a, const int64_t *d_off_a,
             int64_t nc, void *d_out, int dtype, int trim, int k, int64_t max_len,
             void *stream);
 
+/* ---------------------------------------------------------------- batched form */
+/* Launch shape of the persistent kernels; zero fields take the defaults.
+ * Chosen per workload by bitlev.autotune. */
+typedef struct {
+    int waves_per_cu;      /* persistent waves per compute unit (default 16) */
+    int64_t tile;          /* choices per (query, block) tile (default 8192) */
+    int reserved[4];
+} bl_cdist_params;
+
+typedef struct {
+    int waves_per_cu;      /* persistent waves per compute unit (default 16) */
+    int block;             /* workgroup size: 64, 128 or 256 (default 256) */
+    int bin_width;         /* text-length bucket width in bytes (default 16) */
+    int reserved[4];
+} bl_pairs_params;
+
+/* Bytes of device workspace bl_cdist needs (the work-queue counter). */
+size_t bl_cdist_workspace_size(void);
+
+/* All queries (each at most max_qlen <= 512 bytes) against all nc choices in
+ * one persistent launch: waves pull (choice block, query) tiles from a work
+ * queue, build the query's Peq in their own LDS slice, and refill idle lanes
+ * with in-band choices (length-gap and empty choices are answered without
+ * occupying a lane). Row-major out[q * nc + c]. d_work: >= bl_cdist_workspace_size()
+ * bytes of device memory the call may overwrite, or NULL for a stream-ordered
+ * allocation. Queries above 512 bytes go to bl_long. Replaces the per-class
+ * ed_cross_w{NW} launches and host split of bitlev/gpu.py:116-130. */
+int bl_cdist(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
+             const uint8_t *d_buf_c, const int64_t *d_off_c, int64_t nc, int max_qlen,
+             void *d_out, int dtype, int k, void *d_work, size_t work_bytes, void *stream);
+int bl_cdist_tuned(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
+                   const uint8_t *d_buf_c, const int64_t *d_off_c, int64_t nc, int max_qlen,
+                   void *d_out, int dtype, int k, void *d_work, size_t work_bytes,
+                   const bl_cdist_params *params, void *stream);
+
+/* Bytes of device workspace bl_pairs needs for n pairs whose strings are at
+ * most max_len bytes (descriptors, bucket keys, sorted order, counters, and
+ * the wavefront scratch when max_len > 4096). */
+size_t bl_pairs_workspace_size(int64_t n, int64_t max_len);
+
+/* All n pairs in one pipeline with no host round trip: classify (trim, swap,
+ * trivial answers, bucket key) -> bucket scan -> scatter -> one persistent
+ * launch over 64-pair tiles, most expensive bucket first; long pairs (shorter
+ * string > 256 bytes) run the wavefront in the same launch. out[i] in input
+ * order. Replaces the host class split and ed_pairs_c{CAP} launches and the CPU
+ * fallback of bitlev/gpu.py:85-101. */
+int bl_pairs(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t *d_buf_b,
+             const int64_t *d_off_b, int64_t n, int64_t max_len, void *d_out, int dtype,
+             int trim, int k, void *d_work, size_t work_bytes, void *stream);
+int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t *d_buf_b,
+                   const int64_t *d_off_b, int64_t n, int64_t max_len, void *d_out, int dtype,
+                   int trim, int k, void *d_work, size_t work_bytes,
+                   const bl_pairs_params *params, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,46 +23,52 @@ def hip():
     return h
 
 
-def gpu_pairs(hip, pa, pb, k=-1, trim=True, dtype=np.int32):
+FORMS = ("plain", "batched")
+
+
+def gpu_pairs(hip, pa, pb, k=-1, trim=True, dtype=np.int32, form=None):
     A, B = hip.Packed(*pa), hip.Packed(*pb)
     out = hip.DeviceArray.empty((A.n,), dtype)
-    plan = hip.pairs_into(A, B, out, k, trim, None)
+    keep = hip.run_pairs(A, B, out, k, trim, None, form)
     res = out.get()
-    del plan
+    del keep
     return res
 
 
-def gpu_cross(hip, pq, pc, k=-1, dtype=np.int32):
+def gpu_cross(hip, pq, pc, k=-1, dtype=np.int32, form=None):
     Q, C = hip.Packed(*pq), hip.Packed(*pc)
     out = hip.DeviceArray.empty((Q.n, C.n), dtype)
-    plan = hip.cdist_into(Q, C, out, k, None)
+    keep = hip.run_cdist(Q, C, out, k, None, form)
     res = out.get()
-    del plan
+    del keep
     return res
 
 
-def test_golden_pairs_every_k_trim_dtype(hip, golden):
+@pytest.mark.parametrize("form", FORMS)
+def test_golden_pairs_every_k_trim_dtype(hip, golden, form):
     for name in golden_pair_sets(golden):
         pa, pb = pairs_of(golden, name)
-        print(f"pairs golden set {name}", flush=True)
+        print(f"pairs golden set {name} ({form})", flush=True)
         for k in KS:
             for trim in (0, 1):
                 want = golden[f"pairs/{name}/pairs_k/k{k}/trim{trim}"]
                 for dt in DTS:
-                    got = gpu_pairs(hip, pa, pb, k, bool(trim), dt)
+                    got = gpu_pairs(hip, pa, pb, k, bool(trim), dt, form)
                     np.testing.assert_array_equal(got, O.saturate(want, dt),
                                                   err_msg=f"{name} k={k} trim={trim} {dt}")
 
 
-def test_golden_cross_every_k_dtype(hip, golden):
+@pytest.mark.parametrize("form", FORMS)
+def test_golden_cross_every_k_dtype(hip, golden, form):
     for name in golden_cross_sets(golden):
         pq, pc = cross_of(golden, name)
-        print(f"cross golden set {name}", flush=True)
-        np.testing.assert_array_equal(gpu_cross(hip, pq, pc), golden[f"cross/{name}/ed_cross"])
+        print(f"cross golden set {name} ({form})", flush=True)
+        np.testing.assert_array_equal(gpu_cross(hip, pq, pc, form=form),
+                                      golden[f"cross/{name}/ed_cross"])
         for k in KS:
             want = golden[f"cross/{name}/cross_k/k{k}"]
             for dt in DTS:
-                np.testing.assert_array_equal(gpu_cross(hip, pq, pc, k, dt), O.saturate(want, dt),
+                np.testing.assert_array_equal(gpu_cross(hip, pq, pc, k, dt, form), O.saturate(want, dt),
                                               err_msg=f"{name} k={k} {dt}")
 
 
@@ -82,17 +88,19 @@ def test_known_answers_and_shapes(hip):
     assert r8.dtype == np.uint8 and r8[0] == 255
 
 
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("m", [1, 15, 16, 17, 31, 32, 33, 63, 64, 65, 127, 128, 129, 255, 256, 257])
-def test_class_boundaries_mixed_batch(hip, m):
+def test_class_boundaries_mixed_batch(hip, m, form):
     rng = np.random.default_rng(m)
     A = [bytes(rng.integers(0, 256, int(rng.integers(1, m + 1))).astype(np.uint8)) for _ in range(300)]
     B = [bytes(rng.integers(0, 256, int(rng.integers(0, m + 30))).astype(np.uint8)) for _ in range(300)]
     pa, pb = O.pack(A), O.pack(B)
-    np.testing.assert_array_equal(gpu_pairs(hip, pa, pb), O.pairs(pa, pb))
-    np.testing.assert_array_equal(gpu_pairs(hip, pa, pb, trim=False), O.pairs(pa, pb))
+    np.testing.assert_array_equal(gpu_pairs(hip, pa, pb, form=form), O.pairs(pa, pb))
+    np.testing.assert_array_equal(gpu_pairs(hip, pa, pb, trim=False, form=form), O.pairs(pa, pb))
 
 
-def test_long_strings_on_gpu(hip):
+@pytest.mark.parametrize("form", FORMS)
+def test_long_strings_on_gpu(hip, form):
     """The reference's CPU-fallback cases run on the wavefront kernel here,
     including multi-stripe patterns (> 4096 bytes)."""
     rng = np.random.default_rng(12)
@@ -102,17 +110,18 @@ def test_long_strings_on_gpu(hip):
          bytes(rng.integers(0, 4, 5100).astype(np.uint8)), b"q" * 8990 + b"r" * 3]
     pa, pb = O.pack(A), O.pack(B)
     for k in (-1, 0, 10, 100):
-        np.testing.assert_array_equal(gpu_pairs(hip, pa, pb, k), O.pairs(pa, pb, k=k))
+        np.testing.assert_array_equal(gpu_pairs(hip, pa, pb, k, form=form), O.pairs(pa, pb, k=k))
     Q = [b"", b"a" * 64, b"b" * 65, b"c" * 512, b"d" * 513, b"abc", b"q" * 600, b"z" * 4500]
     C = [b"", b"aaa", b"b" * 70, b"c" * 500, b"xyz", b"q" * 590, b"z" * 4400 + b"y" * 90]
     pq, pc = O.pack(Q), O.pack(C)
     for k in (-1, 0, 2, 10):
         for dt in DTS:
-            np.testing.assert_array_equal(gpu_cross(hip, pq, pc, k, dt),
+            np.testing.assert_array_equal(gpu_cross(hip, pq, pc, k, dt, form),
                                           O.saturate(O.cross(pq, pc, k=k), dt))
 
 
-def test_random_batch_against_oracle(hip):
+@pytest.mark.parametrize("form", FORMS)
+def test_random_batch_against_oracle(hip, form):
     rng = np.random.default_rng(99)
     A, B = [], []
     for _ in range(20000):
@@ -131,10 +140,12 @@ def test_random_batch_against_oracle(hip):
         B.append(bytes(b))
     pa, pb = O.pack(A), O.pack(B)
     for k in (-1, 0, 3, 25):
-        np.testing.assert_array_equal(gpu_pairs(hip, pa, pb, k), O.pairs(pa, pb, k=k, threads=8))
+        np.testing.assert_array_equal(gpu_pairs(hip, pa, pb, k, form=form),
+                                      O.pairs(pa, pb, k=k, threads=8))
 
 
-def test_cdist_search_shape_against_oracle(hip):
+@pytest.mark.parametrize("form", FORMS)
+def test_cdist_search_shape_against_oracle(hip, form):
     """A small cut of the L64 search workload: 'a'-'z', 30% near-duplicates,
     max_dist = 2, uint8, more than 64 blocks of choices."""
     rng = np.random.default_rng(5)
@@ -150,7 +161,7 @@ def test_cdist_search_shape_against_oracle(hip):
     pc = (C.ravel(), np.arange(0, C.size + 1, 64, dtype=np.int64))
     for k, dt in ((2, np.uint8), (-1, np.int32), (5, np.uint16)):
         want = O.saturate(O.cross(pq, pc, k=k, threads=8), dt)
-        np.testing.assert_array_equal(gpu_cross(hip, pq, pc, k, dt), want)
+        np.testing.assert_array_equal(gpu_cross(hip, pq, pc, k, dt, form), want)
 
 
 @pytest.mark.parametrize("k", [None, 0, 3])
