@@ -48,10 +48,12 @@ SIGNATURES = {
     "bl_long": [_int, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _int, _int, _int, _i64, _vp],
     "bl_cdist_workspace_size": [],
     "bl_cdist": [_vp, _vp, _i64, _vp, _vp, _i64, _int, _vp, _int, _int, _vp, _sz, _vp],
-    "bl_cdist_tuned": [_vp, _vp, _i64, _vp, _vp, _i64, _int, _vp, _int, _int, _vp, _sz, _vp, _vp],
+    "bl_cdist_tuned": [_vp, _vp, _i64, _vp, _vp, _i64, _int, _vp, _int, _int, _vp, _vp, _sz, _vp,
+                       _vp],
     "bl_pairs_workspace_size": [_i64, _i64],
     "bl_pairs": [_vp, _vp, _vp, _vp, _i64, _i64, _vp, _int, _int, _int, _vp, _sz, _vp],
-    "bl_pairs_tuned": [_vp, _vp, _vp, _vp, _i64, _i64, _vp, _int, _int, _int, _vp, _sz, _vp, _vp],
+    "bl_pairs_tuned": [_vp, _vp, _vp, _vp, _i64, _i64, _vp, _int, _int, _int, _vp, _vp, _sz, _vp,
+                       _vp],
 }
 _RESTYPE = {"bl_last_error": _c.c_char_p, "bl_cdist_workspace_size": _sz,
             "bl_pairs_workspace_size": _sz}

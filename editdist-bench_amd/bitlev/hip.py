@@ -217,8 +217,10 @@ def _pairs_params(params):
 
 
 def pairs_batched_into(A: Packed, B: Packed, out: DeviceArray, k: int = -1, trim: bool = True,
-                       stream=None, workspace: Workspace | None = None, params=None):
-    """One bl_pairs pipeline (bucketing + persistent launch) into `out`."""
+                       stream=None, workspace: Workspace | None = None, params=None,
+                       kvec: DeviceBuffer | None = None):
+    """One bl_pairs pipeline (bucketing + persistent launch) into `out`;
+    kvec: optional device int32 per-pair thresholds (overrides k)."""
     if A.n != B.n:
         raise ValueError(f"length mismatch: {A.n} vs {B.n}")
     if A.n == 0:
@@ -226,21 +228,27 @@ def pairs_batched_into(A: Packed, B: Packed, out: DeviceArray, k: int = -1, trim
     max_len = max(A.max_len, B.max_len)
     ws = (workspace or _workspace(stream)).get(lib().bl_pairs_workspace_size(A.n, max_len))
     call("bl_pairs_tuned", A.d_buf.ptr, A.d_off.ptr, B.d_buf.ptr, B.d_off.ptr, A.n, max_len,
-         out.ptr, DTYPE_CODE[out.dtype], int(trim), int(k), ws.ptr, ws.nbytes, _pairs_params(params),
-         stream)
+         out.ptr, DTYPE_CODE[out.dtype], int(trim), int(k), kvec.ptr if kvec else None, ws.ptr,
+         ws.nbytes, _pairs_params(params), stream)
 
 
 def cdist_batched_into(Q: Packed, C: Packed, out: DeviceArray, k: int = -1, stream=None,
-                       workspace: Workspace | None = None, params=None, plan: Plan | None = None):
+                       workspace: Workspace | None = None, params=None, plan: Plan | None = None,
+                       kvec: DeviceBuffer | None = None):
     """One bl_cdist persistent launch into the row-major (Q.n, C.n) `out`;
-    queries above 512 bytes follow on the wavefront kernel (bl_long)."""
+    queries above 512 bytes follow on the wavefront kernel (bl_long).
+    kvec: optional device int32 per-query thresholds (overrides k; queries
+    above 512 bytes then need a single k)."""
     if Q.n == 0 or C.n == 0:
         return plan
     ws = (workspace or _workspace(stream)).get(lib().bl_cdist_workspace_size())
     mq = min(Q.max_len, BATCHED_MAX_QUERY)
     call("bl_cdist_tuned", Q.d_buf.ptr, Q.d_off.ptr, Q.n, C.d_buf.ptr, C.d_off.ptr, C.n, mq,
-         out.ptr, DTYPE_CODE[out.dtype], int(k), ws.ptr, ws.nbytes, _cdist_params(params), stream)
+         out.ptr, DTYPE_CODE[out.dtype], int(k), kvec.ptr if kvec else None, ws.ptr, ws.nbytes,
+         _cdist_params(params), stream)
     if Q.max_len > BATCHED_MAX_QUERY:
+        if kvec is not None:
+            raise ValueError("per-query thresholds need every query <= 512 bytes")
         if plan is None:
             plan = cdist_plan(Q, C)
         call("bl_long", 0, Q.d_buf.ptr, Q.d_off.ptr, C.d_buf.ptr, C.d_off.ptr, plan.rest_buf.ptr,

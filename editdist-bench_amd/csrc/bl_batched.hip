@@ -38,6 +38,11 @@ namespace {
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
+__host__ __device__ __forceinline__ int clamp_k_dev(int k)
+{
+    return k < 0 ? -1 : (k > (1 << 30) ? (1 << 30) : k);
+}
+
 __device__ __forceinline__ int rank_below(u64 mask)
 {
     return (int)__builtin_amdgcn_mbcnt_hi((u32)(mask >> 32), __builtin_amdgcn_mbcnt_lo((u32)mask, 0u));
@@ -191,7 +196,7 @@ template <int NWMAX>
 __global__ __launch_bounds__(256) void cdist_batched_kernel(
     const u8 *__restrict__ bq, const int64_t *__restrict__ oq, int64_t nq,
     const u8 *__restrict__ bc, const int64_t *__restrict__ oc, int64_t nc, void *out, int dtype,
-    int k, int64_t tile, unsigned long long *counter)
+    int k_all, const int32_t *__restrict__ kvec, int64_t tile, unsigned long long *counter)
 {
     extern __shared__ u64 lds[];
     const int wave = threadIdx.x >> 6;
@@ -211,6 +216,7 @@ __global__ __launch_bounds__(256) void cdist_batched_kernel(
         const int64_t q0 = oq[q];
         const int m = (int)(oq[q + 1] - q0);
         const int64_t orow = q * nc;
+        const int k = kvec ? clamp_k_dev(kvec[q]) : k_all;
         if (m == 0 || m > 64 * NWMAX) {
             /* empty query: n, or k + 1 (csrc/gpu_kernels.cu:53-54); too long for this
              * launch: caller error, marked -1 */
@@ -251,18 +257,21 @@ constexpr int PKEYS = PKEY_LONG + 1;
 struct PDesc {
     int64_t a, b;   /* pattern / text start (after trim and swap) */
     int32_t la, lb;
+    int32_t k, pad;
 };
 
 __global__ __launch_bounds__(256) void pairs_classify_kernel(
     const u8 *__restrict__ ba, const int64_t *__restrict__ oa, const u8 *__restrict__ bb,
-    const int64_t *__restrict__ ob, int64_t n, void *out, int dtype, int trim, int k,
-    int bin_width, PDesc *desc, unsigned short *key, unsigned long long *hist)
+    const int64_t *__restrict__ ob, int64_t n, void *out, int dtype, int trim, int k_all,
+    const int32_t *__restrict__ kvec, int bin_width, PDesc *desc, unsigned short *key,
+    unsigned long long *hist)
 {
     __shared__ unsigned int lh[PKEYS];
     for (int x = threadIdx.x; x < PKEYS; x += blockDim.x) lh[x] = 0;
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
+        const int k = kvec ? clamp_k_dev(kvec[i]) : k_all;
         const u8 *a = ba + oa[i], *b = bb + ob[i];
         int la = (int)(oa[i + 1] - oa[i]), lb = (int)(ob[i + 1] - ob[i]);
         const u8 *a0 = ba, *b0 = bb;
@@ -289,6 +298,8 @@ __global__ __launch_bounds__(256) void pairs_classify_kernel(
             /* bit 31 of la flags a swap: the pattern lives in buf_b */
             d.la = la | (swapped ? (int32_t)0x80000000 : 0);
             d.lb = lb;
+            d.k = k;
+            d.pad = 0;
             desc[i] = d;
         }
         key[i] = (unsigned short)kk;
@@ -326,7 +337,7 @@ __global__ __launch_bounds__(256) void pairs_scatter_kernel(const unsigned short
 
 template <int NW>
 __device__ void pairs_tile(const u8 *__restrict__ ba, const u8 *__restrict__ bb, const PDesc *desc,
-                           const int32_t *sorted, int64_t s0, int64_t s1, void *out, int dtype, int k)
+                           const int32_t *sorted, int64_t s0, int64_t s1, void *out, int dtype)
 {
     const int64_t s = s0 + lane_id();
     if (s >= s1) return;
@@ -336,13 +347,14 @@ __device__ void pairs_tile(const u8 *__restrict__ ba, const u8 *__restrict__ bb,
     const int la = d.la & 0x7fffffff;
     const u8 *a = (sw ? bb : ba) + d.a;
     const u8 *b = (sw ? ba : bb) + d.b;
-    store_out(out, i, pair_lane<NW>(a, la, b, d.lb, 0, k), dtype);
+    store_out(out, i, pair_lane<NW>(a, la, b, d.lb, 0, d.k), dtype);
 }
 
 __device__ void pairs_long_item(const u8 *__restrict__ ba, const u8 *__restrict__ bb,
-                                const PDesc &d, int64_t i, void *out, int dtype, int k,
+                                const PDesc &d, int64_t i, void *out, int dtype,
                                 int8_t *hbuf, int64_t slot)
 {
+    const int k = d.k;
     const int lane = lane_id();
     const bool sw = d.la < 0;
     const int m = d.la & 0x7fffffff, n = d.lb;
@@ -380,7 +392,7 @@ __device__ void pairs_long_item(const u8 *__restrict__ ba, const u8 *__restrict_
 __global__ __launch_bounds__(256) void pairs_batched_kernel(
     const u8 *__restrict__ ba, const u8 *__restrict__ bb, const PDesc *__restrict__ desc,
     const int32_t *__restrict__ sorted, const unsigned short *__restrict__ key_of,
-    const unsigned long long *__restrict__ meta, void *out, int dtype, int k,
+    const unsigned long long *__restrict__ meta, void *out, int dtype,
     unsigned long long *counter, int8_t *scratch, int64_t slot)
 {
     const int64_t total = (int64_t)meta[0];
@@ -393,17 +405,17 @@ __global__ __launch_bounds__(256) void pairs_batched_kernel(
         if (t >= ntiles) break;
         if (t < nlong) {                       /* long pairs come first, one per tile */
             const int64_t i = sorted[t];
-            pairs_long_item(ba, bb, desc[i], i, out, dtype, k, hbuf, slot);
+            pairs_long_item(ba, bb, desc[i], i, out, dtype, hbuf, slot);
             continue;
         }
         const int64_t s0 = nlong + (t - nlong) * 64;
         const int64_t s1 = s0 + 64 < total ? s0 + 64 : total;
         /* descending keys: the first pair of a tile has its largest word count */
         const int w = (key_of[sorted[s0]] - 1) / LB_BINS + 1;
-        if (w <= 1) pairs_tile<1>(ba, bb, desc, sorted, s0, s1, out, dtype, k);
-        else if (w == 2) pairs_tile<2>(ba, bb, desc, sorted, s0, s1, out, dtype, k);
-        else if (w == 3) pairs_tile<3>(ba, bb, desc, sorted, s0, s1, out, dtype, k);
-        else pairs_tile<4>(ba, bb, desc, sorted, s0, s1, out, dtype, k);
+        if (w <= 1) pairs_tile<1>(ba, bb, desc, sorted, s0, s1, out, dtype);
+        else if (w == 2) pairs_tile<2>(ba, bb, desc, sorted, s0, s1, out, dtype);
+        else if (w == 3) pairs_tile<3>(ba, bb, desc, sorted, s0, s1, out, dtype);
+        else pairs_tile<4>(ba, bb, desc, sorted, s0, s1, out, dtype);
     }
 }
 
@@ -437,8 +449,8 @@ size_t bl_cdist_workspace_size(void) { return 256; }
 
 int bl_cdist_tuned(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
                    const uint8_t *d_buf_c, const int64_t *d_off_c, int64_t nc, int max_qlen,
-                   void *d_out, int dtype, int k, void *d_work, size_t work_bytes,
-                   const bl_cdist_params *prm, void *stream)
+                   void *d_out, int dtype, int k, const int32_t *d_kvec, void *d_work,
+                   size_t work_bytes, const bl_cdist_params *prm, void *stream)
 {
     if (nq < 0 || nc < 0 || max_qlen < 0) return fail(BL_EINVAL, "bl_cdist: negative size");
     if (nq == 0 || nc == 0) return BL_OK;
@@ -448,7 +460,7 @@ int bl_cdist_tuned(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
     if (max_qlen > 512) return fail(BL_EINVAL, "bl_cdist: queries above 512 bytes go to bl_long");
     if (d_work && work_bytes < bl_cdist_workspace_size())
         return fail(BL_EINVAL, "bl_cdist: workspace below %zu bytes", bl_cdist_workspace_size());
-    k = k < 0 ? -1 : (k > (1 << 30) ? (1 << 30) : k);
+    k = clamp_k_dev(k);
     hipStream_t s = (hipStream_t)stream;
     const int wpc = prm && prm->waves_per_cu > 0 ? prm->waves_per_cu : 16;
     const int64_t tile = prm && prm->tile > 0 ? prm->tile : 8192;
@@ -468,19 +480,19 @@ int bl_cdist_tuned(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
         switch (nwmax) {
         case 1:
             cdist_batched_kernel<1><<<(unsigned)blocks, 256, lds, s>>>(
-                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, tile, ctr);
+                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, d_kvec, tile, ctr);
             break;
         case 2:
             cdist_batched_kernel<2><<<(unsigned)blocks, 256, lds, s>>>(
-                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, tile, ctr);
+                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, d_kvec, tile, ctr);
             break;
         case 4:
             cdist_batched_kernel<4><<<(unsigned)blocks, 256, lds, s>>>(
-                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, tile, ctr);
+                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, d_kvec, tile, ctr);
             break;
         default:
             cdist_batched_kernel<8><<<(unsigned)blocks, 256, lds, s>>>(
-                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, tile, ctr);
+                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, d_kvec, tile, ctr);
             break;
         }
         hipError_t e = hipGetLastError();
@@ -495,7 +507,7 @@ int bl_cdist(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq, const u
              void *d_work, size_t work_bytes, void *stream)
 {
     return bl_cdist_tuned(d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, max_qlen, d_out, dtype, k,
-                          d_work, work_bytes, nullptr, stream);
+                          nullptr, d_work, work_bytes, nullptr, stream);
 }
 
 size_t bl_pairs_workspace_size(int64_t n, int64_t max_len)
@@ -509,8 +521,8 @@ size_t bl_pairs_workspace_size(int64_t n, int64_t max_len)
 
 int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t *d_buf_b,
                    const int64_t *d_off_b, int64_t n, int64_t max_len, void *d_out, int dtype,
-                   int trim, int k, void *d_work, size_t work_bytes, const bl_pairs_params *prm,
-                   void *stream)
+                   int trim, int k, const int32_t *d_kvec, void *d_work, size_t work_bytes,
+                   const bl_pairs_params *prm, void *stream)
 {
     if (n < 0 || max_len < 0) return fail(BL_EINVAL, "bl_pairs: negative size");
     if (n == 0) return BL_OK;
@@ -522,7 +534,7 @@ int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t
     const size_t need = bl_pairs_workspace_size(n, max_len);
     if (d_work && work_bytes < need)
         return fail(BL_EINVAL, "bl_pairs: workspace of %zu bytes below the %zu needed", work_bytes, need);
-    k = k < 0 ? -1 : (k > (1 << 30) ? (1 << 30) : k);
+    k = clamp_k_dev(k);
     hipStream_t s = (hipStream_t)stream;
     const int bw = prm && prm->bin_width > 0 ? prm->bin_width : 16;
     const int wpc = prm && prm->waves_per_cu > 0 ? prm->waves_per_cu : 16;
@@ -549,7 +561,7 @@ int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t
     const unsigned cblocks = (unsigned)((n + 255) / 256);
     if (rc == BL_OK) {
         pairs_classify_kernel<<<cblocks, 256, 0, s>>>(d_buf_a, d_off_a, d_buf_b, d_off_b, n, d_out,
-                                                      dtype, trim, k, bw, desc, key, hist);
+                                                      dtype, trim, k, d_kvec, bw, desc, key, hist);
         pairs_scan_kernel<<<1, 64, 0, s>>>(hist, base, cursor, meta);
         pairs_scatter_kernel<<<cblocks, 256, 0, s>>>(key, n, cursor, sorted);
         int64_t blocks = (int64_t)device_cus() * wpc * 64 / wg;
@@ -557,7 +569,7 @@ int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t
         if (blocks > maxb) blocks = maxb;
         if (scratch && blocks * (wg / 64) > 2048) blocks = 2048 / (wg / 64);
         pairs_batched_kernel<<<(unsigned)blocks, wg, 0, s>>>(d_buf_a, d_buf_b, desc, sorted, key,
-                                                             meta, d_out, dtype, k, ctr, scratch,
+                                                             meta, d_out, dtype, ctr, scratch,
                                                              scratch ? max_len : 0);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) rc = fail(BL_EHIP, "pairs pipeline launch: %s", hipGetErrorString(e));
@@ -571,7 +583,7 @@ int bl_pairs(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t *d_bu
              int k, void *d_work, size_t work_bytes, void *stream)
 {
     return bl_pairs_tuned(d_buf_a, d_off_a, d_buf_b, d_off_b, n, max_len, d_out, dtype, trim, k,
-                          d_work, work_bytes, nullptr, stream);
+                          nullptr, d_work, work_bytes, nullptr, stream);
 }
 
 }  /* extern "C" */

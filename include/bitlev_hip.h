@@ -129,10 +129,13 @@ size_t bl_cdist_workspace_size(void);
 int bl_cdist(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
              const uint8_t *d_buf_c, const int64_t *d_off_c, int64_t nc, int max_qlen,
              void *d_out, int dtype, int k, void *d_work, size_t work_bytes, void *stream);
+/* bl_cdist with an optional per-query threshold d_kvec[q] (NULL: k for all)
+ * and an explicit launch shape (NULL: defaults). Lets one launch serve
+ * requests with different max_dist. */
 int bl_cdist_tuned(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
                    const uint8_t *d_buf_c, const int64_t *d_off_c, int64_t nc, int max_qlen,
-                   void *d_out, int dtype, int k, void *d_work, size_t work_bytes,
-                   const bl_cdist_params *params, void *stream);
+                   void *d_out, int dtype, int k, const int32_t *d_kvec, void *d_work,
+                   size_t work_bytes, const bl_cdist_params *params, void *stream);
 
 /* Bytes of device workspace bl_pairs needs for n pairs whose strings are at
  * most max_len bytes (descriptors, bucket keys, sorted order, counters, and
@@ -148,9 +151,11 @@ size_t bl_pairs_workspace_size(int64_t n, int64_t max_len);
 int bl_pairs(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t *d_buf_b,
              const int64_t *d_off_b, int64_t n, int64_t max_len, void *d_out, int dtype,
              int trim, int k, void *d_work, size_t work_bytes, void *stream);
+/* bl_pairs with an optional per-pair threshold d_kvec[i] (NULL: k for all)
+ * and an explicit launch shape (NULL: defaults). */
 int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t *d_buf_b,
                    const int64_t *d_off_b, int64_t n, int64_t max_len, void *d_out, int dtype,
-                   int trim, int k, void *d_work, size_t work_bytes,
+                   int trim, int k, const int32_t *d_kvec, void *d_work, size_t work_bytes,
                    const bl_pairs_params *params, void *stream);
 
 #ifdef __cplusplus
