@@ -9,6 +9,7 @@ without a copy.
 from __future__ import annotations
 
 import ctypes
+import threading
 
 import numpy as np
 
@@ -17,21 +18,82 @@ from ._capi import call, lib
 KDL_ROCM = 10
 
 
-class DeviceBuffer:
-    """One bl_malloc allocation, freed with the object."""
+class DevicePool:
+    """Caching allocator over bl_malloc. Freed blocks go back to a free list
+    keyed by their rounded size and are handed out again instead of calling
+    hipFree, which synchronises the whole device and would serialise work
+    running on other streams. Sizes round up to 1/8-power-of-two steps above
+    1 MiB (exact 256-byte multiples below)."""
 
-    __slots__ = ("ptr", "nbytes", "__weakref__")
+    def __init__(self):
+        self._free = {}
+        self._lock = threading.Lock()
+        self.cached = 0
+        self.live = 0
+
+    @staticmethod
+    def round_size(n: int) -> int:
+        n = max(int(n), 256)
+        if n <= 1 << 20:
+            return (n + 255) & ~255
+        step = 1 << (n.bit_length() - 4)
+        return (n + step - 1) // step * step
+
+    def alloc(self, nbytes: int):
+        cap = self.round_size(nbytes)
+        with self._lock:
+            lst = self._free.get(cap)
+            if lst:
+                ptr = lst.pop()
+                self.cached -= cap
+                self.live += cap
+                return ptr, cap
+        p = ctypes.c_void_p()
+        try:
+            call("bl_malloc", ctypes.byref(p), cap)
+        except MemoryError:
+            self.release_cached()
+            call("bl_malloc", ctypes.byref(p), cap)
+        with self._lock:
+            self.live += cap
+        return p.value, cap
+
+    def give_back(self, ptr: int, cap: int):
+        with self._lock:
+            self._free.setdefault(cap, []).append(ptr)
+            self.cached += cap
+            self.live -= cap
+
+    def release_cached(self):
+        """Return every cached block to the driver (synchronises the device)."""
+        with self._lock:
+            blocks = [(p, c) for c, lst in self._free.items() for p in lst]
+            self._free.clear()
+            self.cached = 0
+        for p, _ in blocks:
+            lib().bl_free(p)
+
+
+POOL = DevicePool()
+
+
+class DeviceBuffer:
+    """One device allocation from the caching pool, returned to it with the
+    object (or by free())."""
+
+    __slots__ = ("ptr", "nbytes", "cap", "__weakref__")
 
     def __init__(self, nbytes: int):
         self.nbytes = int(nbytes)
-        p = ctypes.c_void_p()
-        call("bl_malloc", ctypes.byref(p), self.nbytes)
-        self.ptr = p.value or 0
+        if self.nbytes == 0:
+            self.ptr, self.cap = 0, 0
+        else:
+            self.ptr, self.cap = POOL.alloc(self.nbytes)
 
     def free(self):
         if self.ptr:
             p, self.ptr = self.ptr, 0
-            lib().bl_free(p)
+            POOL.give_back(p, self.cap)
 
     def __del__(self):
         try:
