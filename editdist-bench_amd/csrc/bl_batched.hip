@@ -50,6 +50,16 @@ __device__ __forceinline__ int rank_below(u64 mask)
 
 __device__ __forceinline__ u64 wave_ballot(bool p) { return __ballot(p); }
 
+/* Values every lane holds identically, made scalar so that branches on them
+ * are uniform by construction. */
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uni64(int64_t v)
+{
+    const u32 lo = __builtin_amdgcn_readfirstlane((u32)v);
+    const u32 hi = __builtin_amdgcn_readfirstlane((u32)((u64)v >> 32));
+    return (int64_t)(((u64)hi << 32) | lo);
+}
+
 /* Make this wave's LDS writes visible to its own later LDS reads by other lanes. */
 __device__ __forceinline__ void wave_lds_sync()
 {
@@ -58,11 +68,22 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+/* Next work-queue index for the whole wave, as a scalar. Every lane executes
+ * the atomic (the first active lane adds `step`, the others add 0) and the
+ * wave takes that lane's ticket with readfirstlane. There must be no lane-specific branch here:
+ * with `if (lane == 0) atomicAdd(...)` the loop that calls this is compiled
+ * as divergent, and after a `continue` the structurizer sends only lane 0 back
+ * through the atomic while the other lanes re-read a stale (zero) ticket in an
+ * inner loop -- the same tile forever. */
 __device__ __forceinline__ long long wave_take(unsigned long long *counter, long long step)
 {
-    long long t = 0;
-    if (lane_id() == 0) t = (long long)atomicAdd(counter, (unsigned long long)step);
-    return __shfl(t, 0);
+    /* the first active lane adds the step; readfirstlane reads that lane */
+    const bool first = rank_below(__builtin_amdgcn_read_exec()) == 0;
+    const unsigned long long mine = first ? (unsigned long long)step : 0ull;
+    const unsigned long long t = atomicAdd(counter, mine);
+    const u32 lo = __builtin_amdgcn_readfirstlane((u32)t);
+    const u32 hi = __builtin_amdgcn_readfirstlane((u32)(t >> 32));
+    return (long long)(((u64)hi << 32) | lo);
 }
 
 __global__ void clear_kernel(unsigned long long *p, int n)
@@ -115,7 +136,7 @@ __device__ void cdist_tile(const u64 *peq, QEnt *queue, int m, const u8 *__restr
                     for (int w = 0; w < NW; w++) { Pv[w] = ~(u64)0; Mv[w] = 0; }
                     active = true;
                 }
-                qh += avail < nneed ? avail : nneed;
+                qh = uni(qh + (avail < nneed ? avail : nneed));
                 break;
             }
             const int64_t c = wnext + lane;
@@ -135,8 +156,8 @@ __device__ void cdist_tile(const u64 *peq, QEnt *queue, int m, const u8 *__restr
                 e.rel = (int)(c - c0);
                 queue[(qt + rank_below(sm)) & (QCAP - 1)] = e;
             }
-            qt += __popcll(sm);
-            wnext += 64;
+            qt = uni(qt + __popcll(sm));
+            wnext = uni64(wnext + 64);
             wave_lds_sync();
         }
         if (wave_ballot(active) == 0) break;
@@ -247,19 +268,6 @@ __global__ __launch_bounds__(256) void cdist_batched_kernel(
 }
 
 /* ======================================================================= pairs */
-/* Bucket keys: key = 1 + w * LB_BINS + min(lb / bin_width, LB_BINS - 1) for a
- * pattern of w + 1 words (w = 0..3), PKEY_LONG for patterns above 256 bytes,
- * 0 for pairs already answered. Larger key = more work, scheduled first. */
-constexpr int LB_BINS = 64;
-constexpr int PKEY_LONG = 1 + 4 * LB_BINS;
-constexpr int PKEYS = PKEY_LONG + 1;
-
-struct PDesc {
-    int64_t a, b;   /* pattern / text start (after trim and swap) */
-    int32_t la, lb;
-    int32_t k, pad;
-};
-
 __global__ __launch_bounds__(256) void pairs_classify_kernel(
     const u8 *__restrict__ ba, const int64_t *__restrict__ oa, const u8 *__restrict__ bb,
     const int64_t *__restrict__ ob, int64_t n, void *out, int dtype, int trim, int k_all,
@@ -272,36 +280,11 @@ __global__ __launch_bounds__(256) void pairs_classify_kernel(
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         const int k = kvec ? clamp_k_dev(kvec[i]) : k_all;
-        const u8 *a = ba + oa[i], *b = bb + ob[i];
-        int la = (int)(oa[i + 1] - oa[i]), lb = (int)(ob[i + 1] - ob[i]);
-        const u8 *a0 = ba, *b0 = bb;
-        if (trim) trim_pair(a, la, b, lb);
-        bool swapped = false;
-        if (la > lb) {
-            const u8 *t = a; a = b; b = t;
-            const int x = la; la = lb; lb = x;
-            swapped = true;
-        }
-        int kk = 0;
-        if (k >= 0 && lb - la > k) store_out(out, i, k + 1, dtype);
-        else if (la == 0) store_out(out, i, lb, dtype);
-        else {
-            const int w = (la + 63) >> 6;
-            if (w > 4) kk = PKEY_LONG;
-            else {
-                int bin = lb / bin_width;
-                kk = 1 + (w - 1) * LB_BINS + (bin < LB_BINS - 1 ? bin : LB_BINS - 1);
-            }
-            PDesc d;
-            d.a = swapped ? (int64_t)(a - b0) : (int64_t)(a - a0);
-            d.b = swapped ? (int64_t)(b - a0) : (int64_t)(b - b0);
-            /* bit 31 of la flags a swap: the pattern lives in buf_b */
-            d.la = la | (swapped ? (int32_t)0x80000000 : 0);
-            d.lb = lb;
-            d.k = k;
-            d.pad = 0;
-            desc[i] = d;
-        }
+        PDesc d;
+        int result = 0;
+        const int kk = classify_pair(ba, oa, bb, ob, i, trim, k, bin_width, d, result);
+        if (kk == 0) store_out(out, i, result, dtype);
+        else desc[i] = d;
         key[i] = (unsigned short)kk;
         if (kk) atomicAdd(&lh[kk], 1u);
     }
@@ -342,12 +325,7 @@ __device__ void pairs_tile(const u8 *__restrict__ ba, const u8 *__restrict__ bb,
     const int64_t s = s0 + lane_id();
     if (s >= s1) return;
     const int64_t i = sorted[s];
-    const PDesc d = desc[i];
-    const bool sw = d.la < 0;
-    const int la = d.la & 0x7fffffff;
-    const u8 *a = (sw ? bb : ba) + d.a;
-    const u8 *b = (sw ? ba : bb) + d.b;
-    store_out(out, i, pair_lane<NW>(a, la, b, d.lb, 0, d.k), dtype);
+    store_out(out, i, desc_distance<NW>(ba, bb, desc[i]), dtype);
 }
 
 __device__ void pairs_long_item(const u8 *__restrict__ ba, const u8 *__restrict__ bb,
@@ -356,12 +334,12 @@ __device__ void pairs_long_item(const u8 *__restrict__ ba, const u8 *__restrict_
 {
     const int k = d.k;
     const int lane = lane_id();
-    const bool sw = d.la < 0;
-    const int m = d.la & 0x7fffffff, n = d.lb;
+    const bool sw = uni(d.la) < 0;
+    const int m = uni(d.la & 0x7fffffff), n = uni(d.lb);
     LongCtx C;
-    C.p = (sw ? bb : ba) + d.a;
+    C.p = (sw ? bb : ba) + uni64(d.a);
     C.m = m;
-    C.t = (sw ? ba : bb) + d.b;
+    C.t = (sw ? ba : bb) + uni64(d.b);
     C.n = n;
     C.W = (m + 63) >> 6;
     C.hbuf = hbuf;
@@ -411,7 +389,7 @@ __global__ __launch_bounds__(256) void pairs_batched_kernel(
         const int64_t s0 = nlong + (t - nlong) * 64;
         const int64_t s1 = s0 + 64 < total ? s0 + 64 : total;
         /* descending keys: the first pair of a tile has its largest word count */
-        const int w = (key_of[sorted[s0]] - 1) / LB_BINS + 1;
+        const int w = key_words(key_of[sorted[s0]]);
         if (w <= 1) pairs_tile<1>(ba, bb, desc, sorted, s0, s1, out, dtype);
         else if (w == 2) pairs_tile<2>(ba, bb, desc, sorted, s0, s1, out, dtype);
         else if (w == 3) pairs_tile<3>(ba, bb, desc, sorted, s0, s1, out, dtype);

@@ -509,4 +509,63 @@ BL_HD int long_prepare(const u8 *&p, int &m, const u8 *&t, int &n, int is_pair, 
     return -1;
 }
 
+/* ------------------------------------------------------------ batched pairs */
+/* Bucket keys of the batched pairs pipeline: key = 1 + (w - 1) * LB_BINS +
+ * min(lb / bin_width, LB_BINS - 1) for a pattern of w <= 4 words, PKEY_LONG for
+ * longer patterns, 0 for a pair answered during classification. A larger key
+ * means more work; the pipeline schedules large keys first. */
+constexpr int LB_BINS = 64;
+constexpr int PKEY_LONG = 1 + 4 * LB_BINS;
+constexpr int PKEYS = PKEY_LONG + 1;
+
+struct PDesc {
+    int64_t a, b;   /* pattern / text start in their buffers (after trim and swap) */
+    int32_t la, lb; /* bit 31 of la: swapped, the pattern lives in buf_b */
+    int32_t k, pad;
+};
+
+/* Classify pair i: trim, swap, decide the trivial cases. Returns the bucket
+ * key; for key 0, *result holds the answer. */
+BL_HD int classify_pair(const u8 *ba, const int64_t *oa, const u8 *bb, const int64_t *ob,
+                        int64_t i, int trim, int k, int bin_width, PDesc &d, int &result)
+{
+    const u8 *a = ba + oa[i], *b = bb + ob[i];
+    int la = (int)(oa[i + 1] - oa[i]), lb = (int)(ob[i + 1] - ob[i]);
+    if (trim) trim_pair(a, la, b, lb);
+    bool swapped = false;
+    if (la > lb) {
+        const u8 *t = a; a = b; b = t;
+        const int x = la; la = lb; lb = x;
+        swapped = true;
+    }
+    if (k >= 0 && lb - la > k) { result = k + 1; return 0; }
+    if (la == 0) { result = lb; return 0; }
+    const int w = (la + 63) >> 6;
+    int key;
+    if (w > 4) key = PKEY_LONG;
+    else {
+        const int bin = lb / bin_width;
+        key = 1 + (w - 1) * LB_BINS + (bin < LB_BINS - 1 ? bin : LB_BINS - 1);
+    }
+    d.a = swapped ? (int64_t)(a - bb) : (int64_t)(a - ba);
+    d.b = swapped ? (int64_t)(b - ba) : (int64_t)(b - bb);
+    d.la = la | (swapped ? (int32_t)0x80000000 : 0);
+    d.lb = lb;
+    d.k = k;
+    d.pad = 0;
+    return key;
+}
+
+/* Pattern words of the tile whose largest key is `key`. */
+BL_HD int key_words(int key) { return (key - 1) / LB_BINS + 1; }
+
+/* One classified pair through the register-resident lane code. */
+template <int NW>
+BL_HD int desc_distance(const u8 *ba, const u8 *bb, const PDesc &d)
+{
+    const bool sw = d.la < 0;
+    const int la = d.la & 0x7fffffff;
+    return pair_lane<NW>((sw ? bb : ba) + d.a, la, (sw ? ba : bb) + d.b, d.lb, 0, d.k);
+}
+
 }  // namespace bl

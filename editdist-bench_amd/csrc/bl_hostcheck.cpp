@@ -119,6 +119,68 @@ int blh_cross(const u8 *bq, const int64_t *oq, int64_t nq, const u8 *bc, const i
     return 0;
 }
 
+/* The batched pairs pipeline (classify -> bucket scan -> scatter -> tiles of
+ * 64 in descending key order, long pairs first) emulated step by step with
+ * the kernels' shared code, checking every index and the tile invariant
+ * (no pair in a tile needs more words than the tile's first pair). Returns 0,
+ * or a negative code naming the violated check. */
+int blh_pairs_batched(const u8 *ba, const int64_t *oa, const u8 *bb, const int64_t *ob, int64_t n,
+                      int32_t *out, int trim, int k, int bin_width)
+{
+    std::vector<PDesc> desc((size_t)n);
+    std::vector<int> key((size_t)n);
+    std::vector<uint64_t> hist(PKEYS, 0), base(PKEYS, 0), cursor(PKEYS, 0);
+    for (int64_t i = 0; i < n; i++) {
+        int result = 0;
+        const int kk = classify_pair(ba, oa, bb, ob, i, trim, k, bin_width, desc[i], result);
+        if (kk < 0 || kk >= PKEYS) return -10;
+        if (kk == 0) out[i] = result;
+        key[i] = kk;
+        if (kk) hist[kk]++;
+    }
+    uint64_t acc = 0;
+    for (int x = PKEYS - 1; x >= 1; x--) { base[x] = acc; cursor[x] = acc; acc += hist[x]; }
+    const int64_t total = (int64_t)acc, nlong = (int64_t)hist[PKEY_LONG];
+    std::vector<int32_t> sorted((size_t)total, -1);
+    for (int64_t i = 0; i < n; i++) {
+        if (!key[i]) continue;
+        const uint64_t pos = cursor[key[i]]++;
+        if (pos >= (uint64_t)total) return -11;
+        sorted[pos] = (int32_t)i;
+    }
+    for (int64_t s = 0; s < total; s++)
+        if (sorted[s] < 0 || sorted[s] >= n) return -12;
+    const int64_t ntiles = nlong + (total - nlong + 63) / 64;
+    for (int64_t t = 0; t < ntiles; t++) {
+        if (t < nlong) {
+            const int64_t i = sorted[t];
+            if (key[i] != PKEY_LONG) return -13;
+            const PDesc &d = desc[i];
+            const bool sw = d.la < 0;
+            out[i] = long_sim((sw ? bb : ba) + d.a, d.la & 0x7fffffff, (sw ? ba : bb) + d.b, d.lb, 1,
+                              0, d.k);
+            continue;
+        }
+        const int64_t s0 = nlong + (t - nlong) * 64;
+        const int64_t s1 = s0 + 64 < total ? s0 + 64 : total;
+        if (s0 >= total) return -14;
+        const int w = key_words(key[sorted[s0]]);
+        if (w < 1 || w > 4) return -15;
+        for (int64_t s = s0; s < s1; s++) {
+            const int64_t i = sorted[s];
+            const PDesc &d = desc[i];
+            if (((d.la & 0x7fffffff) + 63) / 64 > w) return -16;
+            switch (w) {
+            case 1: out[i] = desc_distance<1>(ba, bb, d); break;
+            case 2: out[i] = desc_distance<2>(ba, bb, d); break;
+            case 3: out[i] = desc_distance<3>(ba, bb, d); break;
+            default: out[i] = desc_distance<4>(ba, bb, d); break;
+            }
+        }
+    }
+    return 0;
+}
+
 /* Output conversion exactly as the kernels store it. */
 int blh_store(int d, int dtype)
 {
