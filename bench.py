@@ -50,6 +50,9 @@ def parse():
                     help="sample size for parity / CPU baseline (queries for cdist, pairs/1024 for pairs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity sample")
     ap.add_argument("--seed", type=int, default=None)
+    ap.add_argument("--no-tune", action="store_true", help="batched form with default launch shape")
+    ap.add_argument("--retune", action="store_true", help="ignore the autotune cache")
+    ap.add_argument("--no-graph", action="store_true", help="batched form without HIP graph replay")
     return ap.parse_args()
 
 
@@ -119,39 +122,64 @@ def main():
     dt = np.dtype(wl["dtype"])
     info = device_info(_capi)
 
+    from bitlev import autotune
+
+    stream = hip.Stream()
+    sp = stream.ptr
     t0 = time.perf_counter()
     if wl["kind"] == "cdist":
         hq, hc = W.cdist(wl["gen"], wl["nq"], wl["nc"], seed)
-        X, Y = hip.Packed(*hq), hip.Packed(*hc)
+        X, Y = hip.Packed(*hq, stream=sp), hip.Packed(*hc, stream=sp)
         out = hip.DeviceArray.empty((X.n, Y.n), dt)
         plan = hip.cdist_plan(X, Y)
         pairs_total = X.n * Y.n
 
-        def make_step(form):
-            return lambda: hip.run_cdist(X, Y, out, k, None, form, plan)
+        def make_step(form, params=None, ws=None):
+            return lambda: hip.run_cdist(X, Y, out, k, sp, form, plan, ws, params)
+
+        def tune():
+            return autotune.tune_cdist(X, Y, out, k, sp, use_cache=not args.retune)
     else:
         ha, hb = W.pairs(wl["gen"], wl["n"], seed)
-        X, Y = hip.Packed(*ha), hip.Packed(*hb)
+        X, Y = hip.Packed(*ha, stream=sp), hip.Packed(*hb, stream=sp)
         out = hip.DeviceArray.empty((X.n,), dt)
         plan = hip.pairs_plan(X, Y)
         pairs_total = X.n
 
-        def make_step(form):
-            return lambda: hip.run_pairs(X, Y, out, k, True, None, form, plan)
-    _capi.call("bl_device_sync")
+        def make_step(form, params=None, ws=None):
+            return lambda: hip.run_pairs(X, Y, out, k, True, sp, form, plan, ws, params)
+
+        def tune():
+            return autotune.tune_pairs(X, Y, out, k, True, sp, use_cache=not args.retune)
+    stream.sync()
     setup_s = time.perf_counter() - t0
 
-    ev = Events(_capi, None)
+    ev = Events(_capi, sp)
     forms = ["plain", "batched"] if args.form == "both" else [args.form]
     measured = {}
+    tuned = None
     for form in forms:
-        wall, per = timed_steps(make_step(form), ev, args.steps, args.warmup)
+        step = make_step(form)
+        extra = {}
+        if form == "batched":
+            t1 = time.perf_counter()
+            tuned = tune() if not args.no_tune else None
+            extra["tune_s"] = round(time.perf_counter() - t1, 2)
+            extra["params"] = tuned
+            ws = hip.Workspace()
+            step = make_step(form, tuned, ws)
+            if not args.no_graph:
+                g = hip.Graph(step, stream)
+                step = g.launch
+                extra["graph"] = True
+        wall, per = timed_steps(step, ev, args.steps, args.warmup)
         ms_step = wall * 1e3 / args.steps
         measured[form] = {
             "value": pairs_total / (ms_step / 1e3) / 1e6,
             "ms_per_step": ms_step,
             "kernel_ms_mean": float(np.mean(per)),
             "kernel_ms_min": float(np.min(per)),
+            **extra,
         }
     best = "batched" if "batched" in measured else forms[0]
     main_m = measured[best]
@@ -192,6 +220,10 @@ def main():
 
 def rounded(d):
     return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
+
+
+def _flt(v):
+    return round(v, 4) if isinstance(v, float) else v
 
 
 def config_of(name, wl):

@@ -43,6 +43,10 @@ SIGNATURES = {
     "bl_event_sync": [_vp],
     "bl_event_query": [_vp],
     "bl_event_elapsed_ms": [_vp, _vp, _c.POINTER(_c.c_float)],
+    "bl_graph_begin": [_vp],
+    "bl_graph_end": [_vp, _c.POINTER(_vp)],
+    "bl_graph_launch": [_vp, _vp],
+    "bl_graph_destroy": [_vp],
     "bl_pairs_class": [_int, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _int, _int, _int, _vp],
     "bl_cdist_class": [_int, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _int, _int, _vp],
     "bl_long": [_int, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _int, _int, _int, _i64, _vp],

@@ -1,0 +1,166 @@
+"""Autotuner for the batched persistent kernels.
+
+For a resident workload it times every candidate launch shape of the batched
+pipeline on a sample of the batch (HIP events, median of a few runs), keeps
+the fastest and caches it per device and workload signature in a JSON file
+($BITLEV_TUNE_CACHE, default ~/.cache/bitlev/autotune.json).
+
+cdist: lane mode (queue refill vs strided lanes), persistent waves per CU,
+choices per tile. pairs: persistent waves per CU, workgroup size, text-length
+bucket width, bucketing chunk. Every candidate computes the same results;
+only speed differs.
+"""
+from __future__ import annotations
+
+import ctypes
+import itertools
+import json
+import os
+import threading
+
+import numpy as np
+
+from . import hip
+from ._capi import call
+
+CDIST_SPACE = {"mode": ["strided", "queue"], "waves_per_cu": [8, 16, 32],
+               "tile": [2048, 8192, 32768]}
+PAIRS_SPACE = {"waves_per_cu": [8, 16, 32], "block": [64, 256], "bin_width": [8, 16, 32],
+               "chunk": [16384, 65536, 262144]}
+
+_lock = threading.Lock()
+
+
+def _cache_path():
+    return os.environ.get("BITLEV_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "bitlev", "autotune.json"))
+
+
+def _load():
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(key, value):
+    with _lock:
+        d = _load()
+        d[key] = value
+        path = _cache_path()
+        try:
+            os.makedirs(os.path.dirname(path), exist_ok=True)
+            tmp = path + ".tmp"
+            with open(tmp, "w") as f:
+                json.dump(d, f, indent=1, sort_keys=True)
+            os.replace(tmp, path)
+        except OSError:
+            pass
+
+
+def _device_tag():
+    name = ctypes.create_string_buffer(256)
+    cu = ctypes.c_int()
+    call("bl_device_info", name, 256, ctypes.byref(cu), None, None)
+    return f"{name.value.decode().strip()}/{cu.value}cu"
+
+
+def _grid(space):
+    keys = list(space)
+    for vals in itertools.product(*(space[k] for k in keys)):
+        yield dict(zip(keys, vals))
+
+
+def _time(fn, stream, reps=3):
+    """Median ms of fn() on stream, measured with HIP events."""
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    call("bl_event_create", ctypes.byref(e0))
+    call("bl_event_create", ctypes.byref(e1))
+    try:
+        fn()
+        call("bl_stream_sync", stream)
+        ts = []
+        for _ in range(reps):
+            call("bl_event_record", e0, stream)
+            fn()
+            call("bl_event_record", e1, stream)
+            call("bl_event_sync", e1)
+            v = ctypes.c_float()
+            call("bl_event_elapsed_ms", e0, e1, ctypes.byref(v))
+            ts.append(v.value)
+        return float(np.median(ts))
+    finally:
+        call("bl_event_destroy", e0)
+        call("bl_event_destroy", e1)
+
+
+def _length_sig(packed: hip.Packed):
+    L = packed.lengths
+    if len(L) == 0:
+        return "0"
+    return f"{int(L.min())}-{int(np.median(L))}-{int(L.max())}"
+
+
+def cdist_signature(Q: hip.Packed, C: hip.Packed, k: int, dtype) -> str:
+    return f"cdist|{_device_tag()}|q{_length_sig(Q)}|c{_length_sig(C)}|k{k}|{np.dtype(dtype).str}"
+
+
+def pairs_signature(A: hip.Packed, B: hip.Packed, k: int, dtype) -> str:
+    return f"pairs|{_device_tag()}|a{_length_sig(A)}|b{_length_sig(B)}|k{k}|{np.dtype(dtype).str}"
+
+
+def tune_cdist(Q: hip.Packed, C: hip.Packed, out: hip.DeviceArray, k: int = -1, stream=None,
+               sample_queries: int = 64, space=None, use_cache: bool = True, log=None):
+    """Best bl_cdist launch shape for these resident inputs (dict of params)."""
+    sig = cdist_signature(Q, C, k, out.dtype)
+    if use_cache:
+        hit = _load().get(sig)
+        if hit:
+            return hit["params"]
+    nq = min(Q.n, sample_queries)
+    Qs = Q if nq == Q.n else hip.Packed(*_head(Q, nq))
+    ws = hip.Workspace()
+    res = []
+    for p in _grid(space or CDIST_SPACE):
+        ms = _time(lambda: hip.cdist_batched_into(Qs, C, out, k, stream, ws, p), stream)
+        res.append((ms, p))
+        if log:
+            log(f"cdist {p} {ms:.3f} ms")
+    ms, best = min(res, key=lambda r: r[0])
+    _store(sig, {"params": best, "ms_sample": ms, "sample_queries": nq})
+    return best
+
+
+def tune_pairs(A: hip.Packed, B: hip.Packed, out: hip.DeviceArray, k: int = -1, trim=True,
+               stream=None, sample_pairs: int = 1 << 20, space=None, use_cache: bool = True,
+               log=None):
+    """Best bl_pairs launch shape for these resident inputs (dict of params)."""
+    sig = pairs_signature(A, B, k, out.dtype)
+    if use_cache:
+        hit = _load().get(sig)
+        if hit:
+            return hit["params"]
+    n = min(A.n, sample_pairs)
+    As = A if n == A.n else hip.Packed(*_head(A, n))
+    Bs = B if n == B.n else hip.Packed(*_head(B, n))
+    ws = hip.Workspace()
+    res = []
+    for p in _grid(space or PAIRS_SPACE):
+        ms = _time(lambda: hip.pairs_batched_into(As, Bs, out, k, trim, stream, ws, p), stream)
+        res.append((ms, p))
+        if log:
+            log(f"pairs {p} {ms:.3f} ms")
+    ms, best = min(res, key=lambda r: r[0])
+    _store(sig, {"params": best, "ms_sample": ms, "sample_pairs": n})
+    return best
+
+
+def _head(P: hip.Packed, n: int):
+    """Host copy of the first n strings of a resident set."""
+    off = P.h_off[: n + 1].copy()
+    buf = np.empty(max(int(off[-1]), 1), np.uint8)
+    if off[-1]:
+        call("bl_memcpy_dtoh", buf.ctypes.data, P.d_buf.ptr, int(off[-1]), None)
+        call("bl_stream_sync", None)
+    return buf, off

@@ -39,7 +39,7 @@ FORM = "batched"                     # default form of pairwise() / cdist()
 
 __all__ = ["available", "distance", "pairwise", "cdist", "Packed", "Plan", "pairs_plan",
            "cdist_plan", "pairs_into", "cdist_into", "pairs_batched_into", "cdist_batched_into",
-           "run_pairs", "run_cdist", "Workspace", "pair_classes", "query_classes",
+           "run_pairs", "run_cdist", "Workspace", "Stream", "Graph", "pair_classes", "query_classes",
            "DeviceArray", "BackendUnavailable", "FORM"]
 
 
@@ -203,6 +203,7 @@ def _cdist_params(params):
     p = CdistParams()
     p.waves_per_cu = int(params.get("waves_per_cu", 0))
     p.tile = int(params.get("tile", 0))
+    p.reserved[0] = 1 if params.get("mode") == "strided" else 0
     return ctypes.byref(p)
 
 
@@ -213,6 +214,7 @@ def _pairs_params(params):
     p.waves_per_cu = int(params.get("waves_per_cu", 0))
     p.block = int(params.get("block", 0))
     p.bin_width = int(params.get("bin_width", 0))
+    p.reserved[0] = int(params.get("chunk", 0))
     return ctypes.byref(p)
 
 
@@ -276,6 +278,57 @@ def run_cdist(Q: Packed, C: Packed, out: DeviceArray, k: int = -1, stream=None,
     if form == "plain":
         return cdist_into(Q, C, out, k, stream, plan)
     raise ValueError("form must be 'batched' or 'plain'")
+
+
+class Stream:
+    """A HIP stream (default priority) owned by Python."""
+
+    def __init__(self):
+        p = ctypes.c_void_p()
+        call("bl_stream_create", ctypes.byref(p))
+        self.ptr = p.value
+
+    def sync(self):
+        call("bl_stream_sync", self.ptr)
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().bl_stream_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+class Graph:
+    """Work issued by fn() on `stream`, captured into a HIP graph and replayed
+    with launch(). fn runs once eagerly first, so every buffer it needs (the
+    workspace included) exists before capture; the captured calls must then
+    allocate nothing and sync nothing."""
+
+    def __init__(self, fn, stream: Stream, warm: bool = True):
+        self.stream = stream
+        if warm:
+            fn()
+            stream.sync()
+        call("bl_graph_begin", stream.ptr)
+        ge = ctypes.c_void_p()
+        try:
+            fn()
+        finally:
+            rc = lib().bl_graph_end(stream.ptr, ctypes.byref(ge))
+        from ._capi import check
+        check(rc, "bl_graph_end")
+        self.exec = ge.value
+
+    def launch(self):
+        call("bl_graph_launch", self.exec, self.stream.ptr)
+
+    def __del__(self):
+        try:
+            if self.exec:
+                lib().bl_graph_destroy(self.exec)
+        except Exception:
+            pass
 
 
 def _finish(out: DeviceArray, return_device: bool, stream=None):

@@ -185,14 +185,31 @@ __device__ void cdist_tile(const u64 *peq, QEnt *queue, int m, const u8 *__restr
                 }
             }
             j += cnt;
-            const int bound = diag_bound<NW>(Pv, Mv, nw, j, m, m - dmn);
-            const bool over = k >= 0 && bound > k;
-            if (over || j >= n) {
-                store_out(out, orow + c0 + rel, over ? k + 1 : bound, dtype);
-                active = false;
+            if (k >= 0 || j >= n) {
+                const int bound = diag_bound<NW>(Pv, Mv, nw, j, m, m - dmn);
+                const bool over = k >= 0 && bound > k;
+                if (over || j >= n) {
+                    store_out(out, orow + c0 + rel, over ? k + 1 : bound, dtype);
+                    active = false;
+                }
             }
             c4 = c4n;
         }
+    }
+}
+
+/* Strided mode: lane l takes choices c0 + l, c0 + l + 64, ... and runs each to
+ * its end (the plain kernel's lane loop inside the persistent tiles). Cheaper
+ * than queue refill when every choice needs about the same number of columns. */
+template <int NW>
+__device__ void cdist_tile_strided(const u64 *peq, int m, const u8 *__restrict__ bc,
+                                   const int64_t *__restrict__ oc, int64_t c0, int64_t c1,
+                                   void *out, int64_t orow, int dtype, int k)
+{
+    for (int64_t c = c0 + lane_id(); c < c1; c += 64) {
+        const int64_t o0 = oc[c];
+        const int n = (int)(oc[c + 1] - o0);
+        store_out(out, orow + c, peq_lane<NW>(peq, m, bc + o0, n, k), dtype);
     }
 }
 
@@ -217,7 +234,8 @@ template <int NWMAX>
 __global__ __launch_bounds__(256) void cdist_batched_kernel(
     const u8 *__restrict__ bq, const int64_t *__restrict__ oq, int64_t nq,
     const u8 *__restrict__ bc, const int64_t *__restrict__ oc, int64_t nc, void *out, int dtype,
-    int k_all, const int32_t *__restrict__ kvec, int64_t tile, unsigned long long *counter)
+    int k_all, const int32_t *__restrict__ kvec, int64_t tile, int strided,
+    unsigned long long *counter)
 {
     extern __shared__ u64 lds[];
     const int wave = threadIdx.x >> 6;
@@ -257,6 +275,16 @@ __global__ __launch_bounds__(256) void cdist_batched_kernel(
             cur_q = q;
             cur_nw = cls;
         }
+        if (strided) {
+            if (cls == 1) cdist_tile_strided<1>(peq, m, bc, oc, c0, c1, out, orow, dtype, k);
+            else if (NWMAX >= 2 && cls == 2)
+                cdist_tile_strided<NWMAX >= 2 ? 2 : 1>(peq, m, bc, oc, c0, c1, out, orow, dtype, k);
+            else if (NWMAX >= 4 && cls == 4)
+                cdist_tile_strided<NWMAX >= 4 ? 4 : 1>(peq, m, bc, oc, c0, c1, out, orow, dtype, k);
+            else if (NWMAX >= 8)
+                cdist_tile_strided<NWMAX >= 8 ? 8 : 1>(peq, m, bc, oc, c0, c1, out, orow, dtype, k);
+            continue;
+        }
         if (cls == 1) cdist_tile<1, CD_QCAP>(peq, queue, m, bc, oc, c0, c1, out, orow, dtype, k);
         else if (NWMAX >= 2 && cls == 2)
             cdist_tile<NWMAX >= 2 ? 2 : 1, CD_QCAP>(peq, queue, m, bc, oc, c0, c1, out, orow, dtype, k);
@@ -268,11 +296,17 @@ __global__ __launch_bounds__(256) void cdist_batched_kernel(
 }
 
 /* ======================================================================= pairs */
+/* Pairs are bucketed within chunks of CH consecutive pairs (CH a multiple of
+ * 256), chunk-major, so the 64 pairs of a tile lie close together in the
+ * string buffers (a global sort would scatter every tile over gigabytes and
+ * thrash the TLB). Per chunk c the order is: long pairs, then buckets by
+ * descending key. Arrays: hist / cursor [nchunks * PKEYS], chunk_base and
+ * tile_base [nchunks + 1]. */
 __global__ __launch_bounds__(256) void pairs_classify_kernel(
     const u8 *__restrict__ ba, const int64_t *__restrict__ oa, const u8 *__restrict__ bb,
     const int64_t *__restrict__ ob, int64_t n, void *out, int dtype, int trim, int k_all,
-    const int32_t *__restrict__ kvec, int bin_width, PDesc *desc, unsigned short *key,
-    unsigned long long *hist)
+    const int32_t *__restrict__ kvec, int bin_width, int64_t chunk, PDesc *desc,
+    unsigned short *key, unsigned long long *hist)
 {
     __shared__ unsigned int lh[PKEYS];
     for (int x = threadIdx.x; x < PKEYS; x += blockDim.x) lh[x] = 0;
@@ -289,33 +323,84 @@ __global__ __launch_bounds__(256) void pairs_classify_kernel(
         if (kk) atomicAdd(&lh[kk], 1u);
     }
     __syncthreads();
+    /* a block never straddles chunks (chunk is a multiple of the block size) */
+    const int64_t c = ((int64_t)blockIdx.x * blockDim.x) / chunk;
+    unsigned long long *h = hist + c * PKEYS;
     for (int x = threadIdx.x; x < PKEYS; x += blockDim.x)
-        if (lh[x]) atomicAdd(&hist[x], (unsigned long long)lh[x]);
+        if (lh[x]) atomicAdd(&h[x], (unsigned long long)lh[x]);
 }
 
-/* Exclusive bases in descending key order; cursor = base (for the scatter);
- * meta[0] = number of pairs to scan, meta[1] = number of long pairs. */
-__global__ void pairs_scan_kernel(const unsigned long long *hist, unsigned long long *base,
-                                  unsigned long long *cursor, unsigned long long *meta)
+/* One block per chunk: within-chunk bases in descending key order (cursor),
+ * the chunk's pair count and tile count (long pairs one tile each). */
+__global__ __launch_bounds__(256) void pairs_chunk_scan_kernel(
+    const unsigned long long *hist, unsigned long long *cursor, unsigned long long *chunk_cnt,
+    unsigned long long *tile_cnt)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    unsigned long long acc = 0;
-    for (int x = PKEYS - 1; x >= 1; x--) {
-        base[x] = acc;
-        cursor[x] = acc;
-        acc += hist[x];
+    __shared__ unsigned long long h[PKEYS];
+    const int64_t c = blockIdx.x;
+    for (int x = threadIdx.x; x < PKEYS; x += blockDim.x) h[x] = hist[c * PKEYS + x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long acc = 0;
+        for (int x = PKEYS - 1; x >= 1; x--) {
+            const unsigned long long v = h[x];
+            h[x] = acc;
+            acc += v;
+        }
+        chunk_cnt[c] = acc;
+        const unsigned long long nlong = hist[c * PKEYS + PKEY_LONG];
+        tile_cnt[c] = nlong + (acc - nlong + 63) / 64;
     }
-    meta[0] = acc;
-    meta[1] = hist[PKEY_LONG];
+    __syncthreads();
+    for (int x = threadIdx.x; x < PKEYS; x += blockDim.x) cursor[c * PKEYS + x] = h[x];
 }
 
-__global__ __launch_bounds__(256) void pairs_scatter_kernel(const unsigned short *key, int64_t n,
-                                                            unsigned long long *cursor, int32_t *sorted)
+/* Exclusive prefix sums of the per-chunk pair and tile counts (one block of
+ * 1024 threads, any number of chunks): base[c] for c <= nchunks. */
+__global__ __launch_bounds__(1024) void prefix_kernel(const unsigned long long *cnt_a,
+                                                      unsigned long long *base_a,
+                                                      const unsigned long long *cnt_b,
+                                                      unsigned long long *base_b, int64_t nchunks)
+{
+    __shared__ unsigned long long sa[1024], sb[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (nchunks + 1023) / 1024;
+    const int64_t lo = t * per, hi = lo + per < nchunks ? lo + per : nchunks;
+    unsigned long long a = 0, b = 0;
+    for (int64_t c = lo; c < hi; c++) { a += cnt_a[c]; b += cnt_b[c]; }
+    sa[t] = a;
+    sb[t] = b;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {            /* inclusive Hillis-Steele scan */
+        const unsigned long long xa = t >= off ? sa[t - off] : 0, xb = t >= off ? sb[t - off] : 0;
+        __syncthreads();
+        sa[t] += xa;
+        sb[t] += xb;
+        __syncthreads();
+    }
+    unsigned long long ra = sa[t] - a, rb = sb[t] - b;      /* exclusive start of this thread */
+    for (int64_t c = lo; c < hi; c++) {
+        base_a[c] = ra;
+        base_b[c] = rb;
+        ra += cnt_a[c];
+        rb += cnt_b[c];
+    }
+    if (t == 1023) {
+        base_a[nchunks] = sa[t];
+        base_b[nchunks] = sb[t];
+    }
+}
+
+__global__ __launch_bounds__(256) void pairs_scatter_kernel(
+    const unsigned short *key, int64_t n, int64_t chunk, unsigned long long *cursor,
+    const unsigned long long *chunk_base, int32_t *sorted)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int kk = key[i];
-    if (kk) sorted[atomicAdd(&cursor[kk], 1ull)] = (int32_t)i;
+    if (!kk) return;
+    const int64_t c = i / chunk;
+    sorted[chunk_base[c] + atomicAdd(&cursor[c * PKEYS + kk], 1ull)] = (int32_t)i;
 }
 
 template <int NW>
@@ -370,26 +455,41 @@ __device__ void pairs_long_item(const u8 *__restrict__ ba, const u8 *__restrict_
 __global__ __launch_bounds__(256) void pairs_batched_kernel(
     const u8 *__restrict__ ba, const u8 *__restrict__ bb, const PDesc *__restrict__ desc,
     const int32_t *__restrict__ sorted, const unsigned short *__restrict__ key_of,
-    const unsigned long long *__restrict__ meta, void *out, int dtype,
+    const unsigned long long *__restrict__ hist, const unsigned long long *__restrict__ chunk_base,
+    const unsigned long long *__restrict__ tile_base, int64_t nchunks, void *out, int dtype,
     unsigned long long *counter, int8_t *scratch, int64_t slot)
 {
-    const int64_t total = (int64_t)meta[0];
-    const int64_t nlong = (int64_t)meta[1];
-    const int64_t ntiles = nlong + (total - nlong + 63) / 64;
+    const int64_t ntiles = (int64_t)tile_base[nchunks];
     const int64_t wave_gid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     int8_t *hbuf = scratch ? scratch + wave_gid * slot : nullptr;
+    int64_t c = 0;
     for (;;) {
         const long long t = wave_take(counter, 1);
         if (t >= ntiles) break;
-        if (t < nlong) {                       /* long pairs come first, one per tile */
-            const int64_t i = sorted[t];
+        /* chunk of tile t: tile_base[c] <= t < tile_base[c + 1]; tiles are taken in
+         * increasing order, so search upwards from the last chunk, then bisect */
+        if ((int64_t)tile_base[c + 1] <= t || (int64_t)tile_base[c] > t) {
+            int64_t lo = 0, hi = nchunks;
+            while (hi - lo > 1) {
+                const int64_t mid = (lo + hi) >> 1;
+                if ((int64_t)tile_base[mid] <= t) lo = mid;
+                else hi = mid;
+            }
+            c = uni64(lo);
+        }
+        const int64_t lt = t - (int64_t)tile_base[c];
+        const int64_t cb = (int64_t)chunk_base[c];
+        const int64_t nlong = (int64_t)hist[c * PKEYS + PKEY_LONG];
+        if (lt < nlong) {                      /* long pairs lead their chunk, one per tile */
+            const int64_t i = sorted[cb + lt];
             pairs_long_item(ba, bb, desc[i], i, out, dtype, hbuf, slot);
             continue;
         }
-        const int64_t s0 = nlong + (t - nlong) * 64;
-        const int64_t s1 = s0 + 64 < total ? s0 + 64 : total;
+        const int64_t s0 = cb + nlong + (lt - nlong) * 64;
+        const int64_t ce = (int64_t)chunk_base[c + 1];
+        const int64_t s1 = s0 + 64 < ce ? s0 + 64 : ce;
         /* descending keys: the first pair of a tile has its largest word count */
-        const int w = key_words(key_of[sorted[s0]]);
+        const int w = uni(key_words(key_of[sorted[s0]]));
         if (w <= 1) pairs_tile<1>(ba, bb, desc, sorted, s0, s1, out, dtype);
         else if (w == 2) pairs_tile<2>(ba, bb, desc, sorted, s0, s1, out, dtype);
         else if (w == 3) pairs_tile<3>(ba, bb, desc, sorted, s0, s1, out, dtype);
@@ -442,6 +542,7 @@ int bl_cdist_tuned(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
     hipStream_t s = (hipStream_t)stream;
     const int wpc = prm && prm->waves_per_cu > 0 ? prm->waves_per_cu : 16;
     const int64_t tile = prm && prm->tile > 0 ? prm->tile : 8192;
+    const int strided = prm && prm->reserved[0] == 1 ? 1 : 0;
     const int words = (max_qlen + 63) >> 6;
     const int nwmax = words <= 1 ? 1 : words <= 2 ? 2 : words <= 4 ? 4 : 8;
     const size_t per_wave = (size_t)(256 * nwmax) * 8 + (size_t)CD_QCAP * sizeof(QEnt);
@@ -458,19 +559,19 @@ int bl_cdist_tuned(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
         switch (nwmax) {
         case 1:
             cdist_batched_kernel<1><<<(unsigned)blocks, 256, lds, s>>>(
-                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, d_kvec, tile, ctr);
+                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, d_kvec, tile, strided, ctr);
             break;
         case 2:
             cdist_batched_kernel<2><<<(unsigned)blocks, 256, lds, s>>>(
-                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, d_kvec, tile, ctr);
+                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, d_kvec, tile, strided, ctr);
             break;
         case 4:
             cdist_batched_kernel<4><<<(unsigned)blocks, 256, lds, s>>>(
-                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, d_kvec, tile, ctr);
+                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, d_kvec, tile, strided, ctr);
             break;
         default:
             cdist_batched_kernel<8><<<(unsigned)blocks, 256, lds, s>>>(
-                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, d_kvec, tile, ctr);
+                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, d_kvec, tile, strided, ctr);
             break;
         }
         hipError_t e = hipGetLastError();
@@ -488,13 +589,39 @@ int bl_cdist(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq, const u
                           nullptr, d_work, work_bytes, nullptr, stream);
 }
 
+static int64_t pairs_chunk(int64_t n, const bl_pairs_params *prm)
+{
+    int64_t ch = prm && prm->reserved[0] > 0 ? prm->reserved[0] : 65536;
+    const int64_t floor_ch = (n + 4095) / 4096;           /* at most 4096 chunks */
+    if (ch < floor_ch) ch = floor_ch;
+    return (ch + 255) / 256 * 256;
+}
+
+struct PairsLayout {
+    size_t head, desc, key, sorted, scratch, total;
+    int64_t chunk, nchunks;
+};
+
+static PairsLayout pairs_layout(int64_t n, int64_t max_len, const bl_pairs_params *prm)
+{
+    PairsLayout L;
+    L.chunk = pairs_chunk(n, prm);
+    L.nchunks = (n + L.chunk - 1) / L.chunk;
+    if (L.nchunks < 1) L.nchunks = 1;
+    /* head: counter | hist | cursor | chunk_cnt | tile_cnt | chunk_base | tile_base */
+    const size_t nk = (size_t)L.nchunks * PKEYS;
+    L.head = (64 + (2 * nk + 4 * ((size_t)L.nchunks + 1)) * 8 + 255) & ~(size_t)255;
+    L.desc = L.head;
+    L.key = (L.desc + (size_t)n * sizeof(PDesc) + 255) & ~(size_t)255;
+    L.sorted = (L.key + (size_t)n * sizeof(unsigned short) + 255) & ~(size_t)255;
+    L.scratch = (L.sorted + (size_t)n * sizeof(int32_t) + 255) & ~(size_t)255;
+    L.total = L.scratch + (max_len > 4096 ? (size_t)max_len * 2048 : 0) + 256;
+    return L;
+}
+
 size_t bl_pairs_workspace_size(int64_t n, int64_t max_len)
 {
-    const size_t head = 4096 + (size_t)PKEYS * 3 * 8;
-    const size_t perpair = sizeof(PDesc) + sizeof(unsigned short) + sizeof(int32_t);
-    size_t sz = head + (size_t)n * perpair + 1024;
-    if (max_len > 4096) sz += (size_t)max_len * 2048 + 256;    /* wavefront stripes */
-    return sz;
+    return pairs_layout(n, max_len, nullptr).total;
 }
 
 int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t *d_buf_b,
@@ -509,9 +636,10 @@ int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t
         return fail(BL_EINVAL, "bl_pairs: null pointer");
     if (n > 0x7fffffffLL) return fail(BL_EINVAL, "bl_pairs: at most 2^31 - 1 pairs per call");
     if (max_len > 0x7fffffffLL) return fail(BL_EINVAL, "bl_pairs: strings must be shorter than 2^31");
-    const size_t need = bl_pairs_workspace_size(n, max_len);
-    if (d_work && work_bytes < need)
-        return fail(BL_EINVAL, "bl_pairs: workspace of %zu bytes below the %zu needed", work_bytes, need);
+    const PairsLayout L = pairs_layout(n, max_len, prm);
+    if (d_work && work_bytes < L.total)
+        return fail(BL_EINVAL, "bl_pairs: workspace of %zu bytes below the %zu needed", work_bytes,
+                    L.total);
     k = clamp_k_dev(k);
     hipStream_t s = (hipStream_t)stream;
     const int bw = prm && prm->bin_width > 0 ? prm->bin_width : 16;
@@ -519,35 +647,37 @@ int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t
     const int wg = prm && (prm->block == 64 || prm->block == 128 || prm->block == 256) ? prm->block : 256;
 
     char *w = (char *)d_work;
-    if (!w) BL_HIP(hipMallocAsync((void **)&w, need, s));
-    /* layout: [counter, meta(2)] | hist | base | cursor | desc | key | sorted | scratch */
+    if (!w) BL_HIP(hipMallocAsync((void **)&w, L.total, s));
+    const size_t nk = (size_t)L.nchunks * PKEYS;
     unsigned long long *ctr = (unsigned long long *)w;
-    unsigned long long *meta = ctr + 1;
     unsigned long long *hist = (unsigned long long *)(w + 64);
-    unsigned long long *base = hist + PKEYS;
-    unsigned long long *cursor = base + PKEYS;
-    char *p = w + 4096 + (size_t)PKEYS * 3 * 8;
-    PDesc *desc = (PDesc *)p;
-    p += (size_t)n * sizeof(PDesc);
-    unsigned short *key = (unsigned short *)p;
-    p += ((size_t)n * sizeof(unsigned short) + 255) & ~(size_t)255;
-    int32_t *sorted = (int32_t *)p;
-    p += ((size_t)n * sizeof(int32_t) + 255) & ~(size_t)255;
-    int8_t *scratch = max_len > 4096 ? (int8_t *)p : nullptr;
+    unsigned long long *cursor = hist + nk;
+    unsigned long long *chunk_cnt = cursor + nk;
+    unsigned long long *tile_cnt = chunk_cnt + (L.nchunks + 1);
+    unsigned long long *chunk_base = tile_cnt + (L.nchunks + 1);
+    unsigned long long *tile_base = chunk_base + (L.nchunks + 1);
+    PDesc *desc = (PDesc *)(w + L.desc);
+    unsigned short *key = (unsigned short *)(w + L.key);
+    int32_t *sorted = (int32_t *)(w + L.sorted);
+    int8_t *scratch = max_len > 4096 ? (int8_t *)(w + L.scratch) : nullptr;
 
-    int rc = launch_clear(ctr, (int)(((w + 64 - (char *)ctr) / 8) + PKEYS * 3), s);
+    /* the counter and the histograms start at zero */
+    int rc = launch_clear(ctr, (int)(8 + nk), s);
     const unsigned cblocks = (unsigned)((n + 255) / 256);
     if (rc == BL_OK) {
         pairs_classify_kernel<<<cblocks, 256, 0, s>>>(d_buf_a, d_off_a, d_buf_b, d_off_b, n, d_out,
-                                                      dtype, trim, k, d_kvec, bw, desc, key, hist);
-        pairs_scan_kernel<<<1, 64, 0, s>>>(hist, base, cursor, meta);
-        pairs_scatter_kernel<<<cblocks, 256, 0, s>>>(key, n, cursor, sorted);
+                                                      dtype, trim, k, d_kvec, bw, L.chunk, desc,
+                                                      key, hist);
+        pairs_chunk_scan_kernel<<<(unsigned)L.nchunks, 256, 0, s>>>(hist, cursor, chunk_cnt, tile_cnt);
+        prefix_kernel<<<1, 1024, 0, s>>>(chunk_cnt, chunk_base, tile_cnt, tile_base, L.nchunks);
+        pairs_scatter_kernel<<<cblocks, 256, 0, s>>>(key, n, L.chunk, cursor, chunk_base, sorted);
         int64_t blocks = (int64_t)device_cus() * wpc * 64 / wg;
         const int64_t maxb = (n + 63) / 64 * 64 / wg + 1;
         if (blocks > maxb) blocks = maxb;
         if (scratch && blocks * (wg / 64) > 2048) blocks = 2048 / (wg / 64);
         pairs_batched_kernel<<<(unsigned)blocks, wg, 0, s>>>(d_buf_a, d_buf_b, desc, sorted, key,
-                                                             meta, d_out, dtype, ctr, scratch,
+                                                             hist, chunk_base, tile_base,
+                                                             L.nchunks, d_out, dtype, ctr, scratch,
                                                              scratch ? max_len : 0);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) rc = fail(BL_EHIP, "pairs pipeline launch: %s", hipGetErrorString(e));

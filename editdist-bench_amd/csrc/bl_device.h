@@ -370,9 +370,11 @@ BL_HD int scan_text(int m, int nw, const u8 *t, int n, int k, EqFn eq)
             }
         }
         j += cnt;
-        const int bound = diag_bound<NW>(Pv, Mv, nw, j, m, n);
-        if (j >= n) return (k >= 0 && bound > k) ? k + 1 : bound;
-        if (k >= 0 && bound > k) return k + 1;
+        if (j >= n) {
+            const int d = diag_bound<NW>(Pv, Mv, nw, j, m, n);     /* exact at j = n */
+            return (k >= 0 && d > k) ? k + 1 : d;
+        }
+        if (k >= 0 && diag_bound<NW>(Pv, Mv, nw, j, m, n) > k) return k + 1;
         c4 = c4n;
     }
 }

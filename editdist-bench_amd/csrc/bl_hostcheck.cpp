@@ -119,57 +119,92 @@ int blh_cross(const u8 *bq, const int64_t *oq, int64_t nq, const u8 *bc, const i
     return 0;
 }
 
-/* The batched pairs pipeline (classify -> bucket scan -> scatter -> tiles of
- * 64 in descending key order, long pairs first) emulated step by step with
- * the kernels' shared code, checking every index and the tile invariant
- * (no pair in a tile needs more words than the tile's first pair). Returns 0,
- * or a negative code naming the violated check. */
+/* The batched pairs pipeline emulated step by step with the kernels' shared
+ * code and the same index arithmetic: classify -> per-chunk bucket scan
+ * (descending keys) -> chunk / tile prefix sums -> scatter -> tiles taken in
+ * order, the chunk of each tile found by the same bisection. Every index is
+ * checked, and so is the tile invariant (no pair in a tile needs more words
+ * than its first pair). chunk: pairs per chunk (a multiple of 256 on the GPU;
+ * any value here). Returns 0, or a negative code naming the violated check. */
 int blh_pairs_batched(const u8 *ba, const int64_t *oa, const u8 *bb, const int64_t *ob, int64_t n,
-                      int32_t *out, int trim, int k, int bin_width)
+                      int32_t *out, int trim, int k, int bin_width, int64_t chunk)
 {
+    if (chunk < 1) return -9;
+    const int64_t nchunks = n ? (n + chunk - 1) / chunk : 1;
     std::vector<PDesc> desc((size_t)n);
     std::vector<int> key((size_t)n);
-    std::vector<uint64_t> hist(PKEYS, 0), base(PKEYS, 0), cursor(PKEYS, 0);
+    std::vector<uint64_t> hist((size_t)(nchunks * PKEYS), 0), cursor(hist.size(), 0);
+    std::vector<uint64_t> ccnt((size_t)nchunks), tcnt((size_t)nchunks);
+    std::vector<uint64_t> cbase((size_t)nchunks + 1), tbase((size_t)nchunks + 1);
     for (int64_t i = 0; i < n; i++) {
         int result = 0;
-        const int kk = classify_pair(ba, oa, bb, ob, i, trim, k, bin_width, desc[i], result);
+        const int kk = classify_pair(ba, oa, bb, ob, i, trim, k, bin_width, desc[(size_t)i], result);
         if (kk < 0 || kk >= PKEYS) return -10;
         if (kk == 0) out[i] = result;
-        key[i] = kk;
-        if (kk) hist[kk]++;
+        key[(size_t)i] = kk;
+        if (kk) hist[(size_t)((i / chunk) * PKEYS + kk)]++;
     }
-    uint64_t acc = 0;
-    for (int x = PKEYS - 1; x >= 1; x--) { base[x] = acc; cursor[x] = acc; acc += hist[x]; }
-    const int64_t total = (int64_t)acc, nlong = (int64_t)hist[PKEY_LONG];
+    for (int64_t c = 0; c < nchunks; c++) {
+        uint64_t acc = 0;
+        for (int x = PKEYS - 1; x >= 1; x--) {
+            cursor[(size_t)(c * PKEYS + x)] = acc;
+            acc += hist[(size_t)(c * PKEYS + x)];
+        }
+        ccnt[(size_t)c] = acc;
+        const uint64_t nl = hist[(size_t)(c * PKEYS + PKEY_LONG)];
+        tcnt[(size_t)c] = nl + (acc - nl + 63) / 64;
+    }
+    cbase[0] = tbase[0] = 0;
+    for (int64_t c = 0; c < nchunks; c++) {
+        cbase[(size_t)c + 1] = cbase[(size_t)c] + ccnt[(size_t)c];
+        tbase[(size_t)c + 1] = tbase[(size_t)c] + tcnt[(size_t)c];
+    }
+    const int64_t total = (int64_t)cbase[(size_t)nchunks];
     std::vector<int32_t> sorted((size_t)total, -1);
     for (int64_t i = 0; i < n; i++) {
-        if (!key[i]) continue;
-        const uint64_t pos = cursor[key[i]]++;
-        if (pos >= (uint64_t)total) return -11;
-        sorted[pos] = (int32_t)i;
+        if (!key[(size_t)i]) continue;
+        const int64_t c = i / chunk;
+        const uint64_t pos = cbase[(size_t)c] + cursor[(size_t)(c * PKEYS + key[(size_t)i])]++;
+        if (pos >= cbase[(size_t)c + 1]) return -11;
+        sorted[(size_t)pos] = (int32_t)i;
     }
-    for (int64_t s = 0; s < total; s++)
-        if (sorted[s] < 0 || sorted[s] >= n) return -12;
-    const int64_t ntiles = nlong + (total - nlong + 63) / 64;
+    for (int64_t q = 0; q < total; q++)
+        if (sorted[(size_t)q] < 0 || sorted[(size_t)q] >= n) return -12;
+    const int64_t ntiles = (int64_t)tbase[(size_t)nchunks];
+    int64_t c = 0;
     for (int64_t t = 0; t < ntiles; t++) {
-        if (t < nlong) {
-            const int64_t i = sorted[t];
-            if (key[i] != PKEY_LONG) return -13;
-            const PDesc &d = desc[i];
+        if ((int64_t)tbase[(size_t)c + 1] <= t || (int64_t)tbase[(size_t)c] > t) {
+            int64_t lo = 0, hi = nchunks;
+            while (hi - lo > 1) {
+                const int64_t mid = (lo + hi) >> 1;
+                if ((int64_t)tbase[(size_t)mid] <= t) lo = mid;
+                else hi = mid;
+            }
+            c = lo;
+        }
+        if (!((int64_t)tbase[(size_t)c] <= t && t < (int64_t)tbase[(size_t)c + 1])) return -17;
+        const int64_t lt = t - (int64_t)tbase[(size_t)c];
+        const int64_t cb = (int64_t)cbase[(size_t)c];
+        const int64_t nlong = (int64_t)hist[(size_t)(c * PKEYS + PKEY_LONG)];
+        if (lt < nlong) {
+            const int64_t i = sorted[(size_t)(cb + lt)];
+            if (key[(size_t)i] != PKEY_LONG) return -13;
+            const PDesc &d = desc[(size_t)i];
             const bool sw = d.la < 0;
             out[i] = long_sim((sw ? bb : ba) + d.a, d.la & 0x7fffffff, (sw ? ba : bb) + d.b, d.lb, 1,
                               0, d.k);
             continue;
         }
-        const int64_t s0 = nlong + (t - nlong) * 64;
-        const int64_t s1 = s0 + 64 < total ? s0 + 64 : total;
-        if (s0 >= total) return -14;
-        const int w = key_words(key[sorted[s0]]);
+        const int64_t s0 = cb + nlong + (lt - nlong) * 64;
+        const int64_t ce = (int64_t)cbase[(size_t)c + 1];
+        const int64_t s1 = s0 + 64 < ce ? s0 + 64 : ce;
+        if (s0 >= ce) return -14;
+        const int w = key_words(key[(size_t)sorted[(size_t)s0]]);
         if (w < 1 || w > 4) return -15;
-        for (int64_t s = s0; s < s1; s++) {
-            const int64_t i = sorted[s];
-            const PDesc &d = desc[i];
-            if (((d.la & 0x7fffffff) + 63) / 64 > w) return -16;
+        for (int64_t q = s0; q < s1; q++) {
+            const int64_t i = sorted[(size_t)q];
+            const PDesc &d = desc[(size_t)i];
+            if ((((d.la & 0x7fffffff) + 63) >> 6) > w) return -16;
             switch (w) {
             case 1: out[i] = desc_distance<1>(ba, bb, d); break;
             case 2: out[i] = desc_distance<2>(ba, bb, d); break;

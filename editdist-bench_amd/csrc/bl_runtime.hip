@@ -181,6 +181,38 @@ int bl_event_query(void *event)
     return bl::fail(BL_EHIP, "hipEventQuery: %s", hipGetErrorString(e));
 }
 
+int bl_graph_begin(void *stream)
+{
+    if (!stream) return bl::fail(BL_EINVAL, "bl_graph_begin: capture needs a non-default stream");
+    BL_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return BL_OK;
+}
+
+int bl_graph_end(void *stream, void **graph_exec)
+{
+    if (!graph_exec) return bl::fail(BL_EINVAL, "bl_graph_end: null output pointer");
+    hipGraph_t g = nullptr;
+    BL_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+    hipGraphExec_t ge = nullptr;
+    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return bl::fail(BL_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    *graph_exec = (void *)ge;
+    return BL_OK;
+}
+
+int bl_graph_launch(void *graph_exec, void *stream)
+{
+    BL_HIP(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream));
+    return BL_OK;
+}
+
+int bl_graph_destroy(void *graph_exec)
+{
+    if (graph_exec) BL_HIP(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+    return BL_OK;
+}
+
 int bl_event_elapsed_ms(void *start, void *stop, float *ms)
 {
     BL_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));

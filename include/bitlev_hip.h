@@ -68,6 +68,16 @@ int bl_event_sync(void *event);
 int bl_event_query(void *event);                       /* 1 done, 0 pending, <0 error */
 int bl_event_elapsed_ms(void *start, void *stop, float *ms);
 
+/* HIP graphs: capture the work issued on `stream` (not the default stream)
+ * between bl_graph_begin and bl_graph_end into an executable graph; replay it
+ * with bl_graph_launch. The batched pipelines (bl_pairs, bl_cdist) with a
+ * caller-provided workspace are capture-safe: they clear their counters with a
+ * kernel, never a memset, and allocate nothing. */
+int bl_graph_begin(void *stream);
+int bl_graph_end(void *stream, void **graph_exec);
+int bl_graph_launch(void *graph_exec, void *stream);
+int bl_graph_destroy(void *graph_exec);
+
 /* ---------------------------------------------------------------- plain form */
 /* One pairs class, one lane per pair: pair i = d_idx[t] for t < cnt, result
  * written to out[i]. cap in {16, 32, 64, 128, 256}: every listed pair has
@@ -105,14 +115,14 @@ int bl_long(int is_pair, const uint8_t *d_buf_a, const int64_t *d_off_a,
 typedef struct {
     int waves_per_cu;      /* persistent waves per compute unit (default 16) */
     int64_t tile;          /* choices per (query, block) tile (default 8192) */
-    int reserved[4];
+    int reserved[4];       /* [0]: 1 = strided lanes, 0 = queue refill (default) */
 } bl_cdist_params;
 
 typedef struct {
     int waves_per_cu;      /* persistent waves per compute unit (default 16) */
     int block;             /* workgroup size: 64, 128 or 256 (default 256) */
     int bin_width;         /* text-length bucket width in bytes (default 16) */
-    int reserved[4];
+    int reserved[4];       /* [0]: pairs per bucketing chunk (default 65536) */
 } bl_pairs_params;
 
 /* Bytes of device workspace bl_cdist needs (the work-queue counter). */

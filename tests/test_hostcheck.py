@@ -30,7 +30,7 @@ def hc():
                             ctypes.c_int, ctypes.c_int]
     L.blh_store.argtypes = [ctypes.c_int, ctypes.c_int]
     L.blh_pairs_batched.argtypes = [u8p, i64p, u8p, i64p, ctypes.c_int64, i32p, ctypes.c_int,
-                                    ctypes.c_int, ctypes.c_int]
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int64]
     L.blh_transpose8.argtypes = [ctypes.c_uint64]
     L.blh_transpose8.restype = ctypes.c_uint64
     L.blh_bop3.argtypes = [ctypes.c_uint32] * 3 + [ctypes.c_int]
@@ -83,10 +83,11 @@ def test_peq_lane_code_matches_golden(hc, golden, force_long):
                                           err_msg=f"{name} k={k}")
 
 
-@pytest.mark.parametrize("bin_width", [1, 16, 64])
-def test_batched_pairs_pipeline_emulated(hc, golden, bin_width):
-    """classify -> bucket scan -> scatter -> tiles, as the persistent kernel
-    walks them, with every index and the tile word-count invariant checked."""
+@pytest.mark.parametrize("bin_width,chunk", [(1, 7), (16, 64), (16, 256), (64, 1 << 20)])
+def test_batched_pairs_pipeline_emulated(hc, golden, bin_width, chunk):
+    """classify -> per-chunk bucket scan -> scatter -> tiles, as the persistent
+    kernel walks them, with every index and the tile word-count invariant
+    checked (small chunks: many chunks, some without long pairs or empty)."""
     for name in golden_pair_sets(golden):
         (ba, oa), (bb, ob) = pairs_of(golden, name)
         n = len(oa) - 1
@@ -94,7 +95,7 @@ def test_batched_pairs_pipeline_emulated(hc, golden, bin_width):
             for trim in (0, 1):
                 out = np.full(n, -7, np.int32)
                 rc = hc.blh_pairs_batched(_p(ba, u8p), _p(oa, i64p), _p(bb, u8p), _p(ob, i64p), n,
-                                          _p(out, i32p), trim, k, bin_width)
+                                          _p(out, i32p), trim, k, bin_width, chunk)
                 assert rc == 0, (name, k, trim, rc)
                 np.testing.assert_array_equal(out, golden[f"pairs/{name}/pairs_k/k{k}/trim{trim}"],
                                               err_msg=f"{name} k={k} trim={trim}")
