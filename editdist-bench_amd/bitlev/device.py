@@ -27,7 +27,7 @@ class DevicePool:
 
     def __init__(self):
         self._free = {}
-        self._lock = threading.Lock()
+        self._lock = threading.RLock()   # __del__ may run inside give_back via GC
         self.cached = 0
         self.live = 0
 
