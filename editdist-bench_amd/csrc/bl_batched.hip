@@ -602,15 +602,19 @@ struct PairsLayout {
     int64_t chunk, nchunks;
 };
 
-static PairsLayout pairs_layout(int64_t n, int64_t max_len, const bl_pairs_params *prm)
+/* Workspace layout. header_chunks: chunks the header is sized for (at least
+ * the chunks this call uses), so one size can serve every launch shape. */
+static PairsLayout pairs_layout(int64_t n, int64_t max_len, const bl_pairs_params *prm,
+                                int64_t header_chunks = 0)
 {
     PairsLayout L;
     L.chunk = pairs_chunk(n, prm);
     L.nchunks = (n + L.chunk - 1) / L.chunk;
     if (L.nchunks < 1) L.nchunks = 1;
+    const int64_t hc = header_chunks > L.nchunks ? header_chunks : L.nchunks;
     /* head: counter | hist | cursor | chunk_cnt | tile_cnt | chunk_base | tile_base */
-    const size_t nk = (size_t)L.nchunks * PKEYS;
-    L.head = (64 + (2 * nk + 4 * ((size_t)L.nchunks + 1)) * 8 + 255) & ~(size_t)255;
+    const size_t nk = (size_t)hc * PKEYS;
+    L.head = (64 + (2 * nk + 4 * ((size_t)hc + 1)) * 8 + 255) & ~(size_t)255;
     L.desc = L.head;
     L.key = (L.desc + (size_t)n * sizeof(PDesc) + 255) & ~(size_t)255;
     L.sorted = (L.key + (size_t)n * sizeof(unsigned short) + 255) & ~(size_t)255;
@@ -619,9 +623,16 @@ static PairsLayout pairs_layout(int64_t n, int64_t max_len, const bl_pairs_param
     return L;
 }
 
+/* Enough for any launch shape: chunks are at least 256 pairs and at most 4096. */
+static int64_t max_chunks(int64_t n)
+{
+    const int64_t c = (n + 255) / 256;
+    return c < 4096 ? (c < 1 ? 1 : c) : 4096;
+}
+
 size_t bl_pairs_workspace_size(int64_t n, int64_t max_len)
 {
-    return pairs_layout(n, max_len, nullptr).total;
+    return pairs_layout(n, max_len, nullptr, max_chunks(n)).total;
 }
 
 int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t *d_buf_b,
@@ -636,7 +647,7 @@ int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t
         return fail(BL_EINVAL, "bl_pairs: null pointer");
     if (n > 0x7fffffffLL) return fail(BL_EINVAL, "bl_pairs: at most 2^31 - 1 pairs per call");
     if (max_len > 0x7fffffffLL) return fail(BL_EINVAL, "bl_pairs: strings must be shorter than 2^31");
-    const PairsLayout L = pairs_layout(n, max_len, prm);
+    const PairsLayout L = pairs_layout(n, max_len, prm, max_chunks(n));
     if (d_work && work_bytes < L.total)
         return fail(BL_EINVAL, "bl_pairs: workspace of %zu bytes below the %zu needed", work_bytes,
                     L.total);
