@@ -53,7 +53,30 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="batched form with default launch shape")
     ap.add_argument("--retune", action="store_true", help="ignore the autotune cache")
     ap.add_argument("--no-graph", action="store_true", help="batched form without HIP graph replay")
+    ap.add_argument("--params", default=None,
+                    help="batched launch shape as JSON (skips the autotuner), e.g. "
+                         "'{\"mode\": \"strided\", \"waves_per_cu\": 32, \"tile\": 8192}'")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device 3 times (a kernel of known traffic "
+                         "for calibrating the HBM byte counters) and print the byte count")
     return ap.parse_args()
+
+
+def _log(msg):
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
+def copy_probe(mib):
+    """Three device-to-device copies of a known size on one stream."""
+    from bitlev import _capi
+    from bitlev.device import DeviceBuffer
+    nbytes = int(mib) << 20
+    a, b = DeviceBuffer(nbytes), DeviceBuffer(nbytes)
+    _capi.call("bl_memset", a.ptr, 1, nbytes, None)
+    for _ in range(3):
+        _capi.call("bl_memcpy_dtod", b.ptr, a.ptr, nbytes, None)
+    _capi.call("bl_device_sync")
+    print(json.dumps({"copy_probe_bytes": nbytes, "copies": 3}), flush=True)
 
 
 class Events:
@@ -116,6 +139,9 @@ def main():
     if not hip.available():
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     wl = W.WORKLOADS[args.workload]
     seed = args.seed if args.seed is not None else W.SEED
     k = -1 if wl["k"] is None else int(wl["k"])
@@ -130,6 +156,7 @@ def main():
     if wl["kind"] == "cdist":
         hq, hc = W.cdist(wl["gen"], wl["nq"], wl["nc"], seed)
         X, Y = hip.Packed(*hq, stream=sp), hip.Packed(*hc, stream=sp)
+        Y.stripes(sp)                      # the resident search layout of the choice set
         out = hip.DeviceArray.empty((X.n, Y.n), dt)
         plan = hip.cdist_plan(X, Y)
         pairs_total = X.n * Y.n
@@ -138,7 +165,7 @@ def main():
             return lambda: hip.run_cdist(X, Y, out, k, sp, form, plan, ws, params)
 
         def tune():
-            return autotune.tune_cdist(X, Y, out, k, sp, use_cache=not args.retune)
+            return autotune.tune_cdist(X, Y, out, k, sp, use_cache=not args.retune, log=_log)
     else:
         ha, hb = W.pairs(wl["gen"], wl["n"], seed)
         X, Y = hip.Packed(*ha, stream=sp), hip.Packed(*hb, stream=sp)
@@ -150,7 +177,7 @@ def main():
             return lambda: hip.run_pairs(X, Y, out, k, True, sp, form, plan, ws, params)
 
         def tune():
-            return autotune.tune_pairs(X, Y, out, k, True, sp, use_cache=not args.retune)
+            return autotune.tune_pairs(X, Y, out, k, True, sp, use_cache=not args.retune, log=_log)
     stream.sync()
     setup_s = time.perf_counter() - t0
 
@@ -163,7 +190,10 @@ def main():
         extra = {}
         if form == "batched":
             t1 = time.perf_counter()
-            tuned = tune() if not args.no_tune else None
+            if args.params:
+                tuned = json.loads(args.params)
+            else:
+                tuned = tune() if not args.no_tune else None
             extra["tune_s"] = round(time.perf_counter() - t1, 2)
             extra["params"] = tuned
             ws = hip.Workspace()

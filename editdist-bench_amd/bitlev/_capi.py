@@ -48,19 +48,25 @@ SIGNATURES = {
     "bl_graph_launch": [_vp, _vp],
     "bl_graph_destroy": [_vp],
     "bl_pairs_class": [_int, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _int, _int, _int, _vp],
-    "bl_cdist_class": [_int, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _int, _int, _vp],
+    "bl_stripes_layout": [_vp, _i64, _vp],
+    "bl_stripes_build": [_vp, _vp, _i64, _vp, _vp, _vp, _vp],
+    "bl_cdist_class": [_int, _vp, _vp, _vp, _i64, _vp, _vp, _int, _int, _vp],
     "bl_long": [_int, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _int, _int, _int, _i64, _vp],
-    "bl_cdist_workspace_size": [],
-    "bl_cdist": [_vp, _vp, _i64, _vp, _vp, _i64, _int, _vp, _int, _int, _vp, _sz, _vp],
-    "bl_cdist_tuned": [_vp, _vp, _i64, _vp, _vp, _i64, _int, _vp, _int, _int, _vp, _vp, _sz, _vp,
-                       _vp],
+    "bl_cdist_workspace_size": [_i64, _int],
+    "bl_cdist": [_vp, _vp, _i64, _vp, _int, _vp, _int, _int, _vp, _sz, _vp],
+    "bl_cdist_tuned": [_vp, _vp, _i64, _vp, _int, _vp, _int, _int, _vp, _vp, _sz, _vp, _vp],
     "bl_pairs_workspace_size": [_i64, _i64],
     "bl_pairs": [_vp, _vp, _vp, _vp, _i64, _i64, _vp, _int, _int, _int, _vp, _sz, _vp],
     "bl_pairs_tuned": [_vp, _vp, _vp, _vp, _i64, _i64, _vp, _int, _int, _int, _vp, _vp, _sz, _vp,
                        _vp],
 }
 _RESTYPE = {"bl_last_error": _c.c_char_p, "bl_cdist_workspace_size": _sz,
-            "bl_pairs_workspace_size": _sz}
+            "bl_pairs_workspace_size": _sz, "bl_stripes_layout": _i64}
+
+
+class Stripes(ctypes.Structure):
+    """bl_stripes (include/bitlev_hip.h)."""
+    _fields_ = [("words", _vp), ("gbase", _vp), ("len", _vp), ("n", _i64)]
 
 
 class CdistParams(ctypes.Structure):

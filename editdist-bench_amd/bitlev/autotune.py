@@ -111,7 +111,7 @@ def pairs_signature(A: hip.Packed, B: hip.Packed, k: int, dtype) -> str:
 
 
 def tune_cdist(Q: hip.Packed, C: hip.Packed, out: hip.DeviceArray, k: int = -1, stream=None,
-               sample_queries: int = 64, space=None, use_cache: bool = True, log=None):
+               sample_queries: int = 256, space=None, use_cache: bool = True, log=None):
     """Best bl_cdist launch shape for these resident inputs (dict of params)."""
     sig = cdist_signature(Q, C, k, out.dtype)
     if use_cache:

@@ -28,7 +28,7 @@ import numpy as np
 from . import _edpy
 import ctypes
 
-from ._capi import BackendUnavailable, CdistParams, PairsParams, call, lib
+from ._capi import BackendUnavailable, CdistParams, PairsParams, Stripes, call, lib
 from ._common import DTYPE_CODE, as_list, check_dtype, check_k
 from .device import DeviceArray, DeviceBuffer
 
@@ -40,6 +40,7 @@ FORM = "batched"                     # default form of pairwise() / cdist()
 __all__ = ["available", "distance", "pairwise", "cdist", "Packed", "Plan", "pairs_plan",
            "cdist_plan", "pairs_into", "cdist_into", "pairs_batched_into", "cdist_batched_into",
            "run_pairs", "run_cdist", "Workspace", "Stream", "Graph", "pair_classes", "query_classes",
+           "ChoiceStripes",
            "DeviceArray", "BackendUnavailable", "FORM"]
 
 
@@ -99,9 +100,49 @@ class Packed:
         self.d_buf = DeviceBuffer.from_host(buf, stream)
         self.d_off = DeviceBuffer.from_host(self.h_off, stream)
 
+        self._stripes = None
+
     @classmethod
     def from_strings(cls, strings, stream=None) -> "Packed":
         return cls(*pack(strings), stream=stream)
+
+    def stripes(self, stream=None) -> "ChoiceStripes":
+        """The set as search-kernel stripes (built on the device on first use,
+        on `stream`, which is then synchronised; kept with the set)."""
+        if self._stripes is None:
+            self._stripes = ChoiceStripes(self, stream)
+        return self._stripes
+
+    @property
+    def device_bytes(self) -> int:
+        b = self.d_buf.nbytes + self.d_off.nbytes
+        if self._stripes is not None:
+            b += self._stripes.nbytes
+        return b
+
+
+class ChoiceStripes:
+    """A packed set interleaved by dword in groups of 64 strings (bl_stripes):
+    the layout the cdist kernels read, so that a wave's lanes load 256
+    contiguous bytes per step."""
+
+    def __init__(self, P: Packed, stream=None):
+        n = P.n
+        gbase = np.empty((n + 63) // 64 + 1, np.int64)
+        total = call("bl_stripes_layout", P.h_off.ctypes.data, n, gbase.ctypes.data)
+        self.d_gbase = DeviceBuffer.from_host(gbase, stream)
+        self.d_words = DeviceBuffer(max(int(total), 1) * 4)
+        self.d_len = DeviceBuffer(max(n, 1) * 4)
+        call("bl_stripes_build", P.d_buf.ptr, P.d_off.ptr, n, self.d_gbase.ptr, self.d_words.ptr,
+             self.d_len.ptr, stream)
+        # built once per set; finish here so that any stream may read them next
+        call("bl_stream_sync", stream)
+        self.nbytes = self.d_gbase.nbytes + self.d_words.nbytes + self.d_len.nbytes
+        self.desc = Stripes(self.d_words.ptr, self.d_gbase.ptr, self.d_len.ptr, n)
+
+    @property
+    def ref(self):
+        return ctypes.byref(self.desc)
 
 
 def _idx_buffers(groups):
@@ -163,10 +204,11 @@ def cdist_into(Q: Packed, C: Packed, out: DeviceArray, k: int = -1, stream=None,
     if Q.n == 0 or C.n == 0:
         return plan
     dt = DTYPE_CODE[out.dtype]
+    st = C.stripes(stream)
     for nw, g, b in zip(CROSS_NWS, plan.groups, plan.bufs):
         if b is not None:
-            call("bl_cdist_class", nw, Q.d_buf.ptr, Q.d_off.ptr, b.ptr, len(g), C.d_buf.ptr,
-                 C.d_off.ptr, C.n, out.ptr, dt, int(k), stream)
+            call("bl_cdist_class", nw, Q.d_buf.ptr, Q.d_off.ptr, b.ptr, len(g), st.ref, out.ptr,
+                 dt, int(k), stream)
     if plan.rest_buf is not None:
         call("bl_long", 0, Q.d_buf.ptr, Q.d_off.ptr, C.d_buf.ptr, C.d_off.ptr, plan.rest_buf.ptr,
              len(plan.rest), C.n, out.ptr, dt, 0, int(k), plan.max_len, stream)
@@ -243,10 +285,10 @@ def cdist_batched_into(Q: Packed, C: Packed, out: DeviceArray, k: int = -1, stre
     above 512 bytes then need a single k)."""
     if Q.n == 0 or C.n == 0:
         return plan
-    ws = (workspace or _workspace(stream)).get(lib().bl_cdist_workspace_size())
     mq = min(Q.max_len, BATCHED_MAX_QUERY)
-    call("bl_cdist_tuned", Q.d_buf.ptr, Q.d_off.ptr, Q.n, C.d_buf.ptr, C.d_off.ptr, C.n, mq,
-         out.ptr, DTYPE_CODE[out.dtype], int(k), kvec.ptr if kvec else None, ws.ptr, ws.nbytes,
+    ws = (workspace or _workspace(stream)).get(lib().bl_cdist_workspace_size(Q.n, mq))
+    call("bl_cdist_tuned", Q.d_buf.ptr, Q.d_off.ptr, Q.n, C.stripes(stream).ref, mq, out.ptr,
+         DTYPE_CODE[out.dtype], int(k), kvec.ptr if kvec else None, ws.ptr, ws.nbytes,
          _cdist_params(params), stream)
     if Q.max_len > BATCHED_MAX_QUERY:
         if kvec is not None:

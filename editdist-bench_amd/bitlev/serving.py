@@ -130,14 +130,14 @@ class Server:
         with self._cv:
             if name in self.sets:
                 raise ValueError(f"set {name!r} already resident")
-            self._in_use += P.nbytes + 8 * (P.n + 1)
+            self._in_use += _resident_bytes(P)
             self.sets[name] = P
         return P
 
     def drop_set(self, name: str):
         with self._cv:
             P = self.sets.pop(name)
-            self._in_use -= P.nbytes + 8 * (P.n + 1)
+            self._in_use -= _resident_bytes(P)
             self._cv.notify_all()
 
     # ------------------------------------------------------------ submission
@@ -305,7 +305,10 @@ class _GpuExecutor:
         return _Slot(i)
 
     def resident(self, packed):
-        return hip.Packed(*packed)
+        P = hip.Packed(*packed)
+        P.stripes()                        # the search layout, built once per set
+        call("bl_device_sync")
+        return P
 
     def pairs_workspace(self, n, max_len):
         from ._capi import lib
@@ -346,6 +349,12 @@ class _GpuExecutor:
         call("bl_event_record", slot.event, slot.stream)
         call("bl_event_sync", slot.event)
         return views, out._buf
+
+
+def _resident_bytes(P) -> int:
+    """Device bytes a resident set holds (packed buffer, offsets, stripes)."""
+    b = getattr(P, "device_bytes", None)
+    return int(b) if b is not None else int(P.nbytes) + 8 * (P.n + 1)
 
 
 def _span(packed) -> int:

@@ -93,205 +93,305 @@ __global__ void clear_kernel(unsigned long long *p, int n)
 }
 
 /* ======================================================================= cdist */
-struct QEnt {
-    int64_t start;
-    int32_t n;
-    int32_t rel;
+constexpr int CD_SCAP = 128;   /* pending-choice stack entries per wave (>= 64 + 64 - 1) */
+constexpr int CD_ECAP = 128;   /* escalation stack entries per wave (>= 64 + 64 - 1) */
+constexpr int CD_DIRECT = 32;  /* in-band choices a window needs to be scanned in place */
+constexpr int CD_PARK = 16;    /* a pass parks its stragglers once at most this many remain */
+
+/* A choice waiting for a pass of the 32-row stage, with its scan state after
+ * j columns (j = 0: not started). */
+struct Pend {
+    int64_t start;             /* dword 0 of its stripe string */
+    int32_t rel, n, j;
+    u32 Pv, Mv, pad;
 };
 
-/* One (query, choice block) tile for one wave. peq: this wave's LDS table
- * (row c at peq[c * NW]); queue: this wave's LDS ring of QCAP entries. */
-template <int NW, int QCAP>
-__device__ void cdist_tile(const u64 *peq, QEnt *queue, int m, const u8 *__restrict__ bc,
-                           const int64_t *__restrict__ oc, int64_t c0, int64_t c1, void *out,
+/* One wave's slice of LDS. */
+struct WaveLds {
+    u32 lo32[256];             /* rows 0..31 of the current query's Peq table */
+    Pend pend[CD_SCAP];        /* choices waiting for a pass */
+    int32_t esc[CD_ECAP];      /* choices the 32-row stage cannot decide */
+};
+
+/* Peq tables of every query in global memory: row c of query q holds nwmax
+ * words at peq[(q * 256 + c) * nwmax]; word w bit i = (query[64w + i] == c).
+ * One workgroup per query, one thread per byte value. A query longer than
+ * 64 * nwmax (a caller error, answered -1 by the search kernel) gets zeros. */
+__global__ __launch_bounds__(256) void peq_build_kernel(const u8 *__restrict__ bq,
+                                                        const int64_t *__restrict__ oq, int nwmax,
+                                                        u64 *__restrict__ peq)
+{
+    __shared__ u8 qs[512];
+    const int64_t q = blockIdx.x;
+    const int64_t q0 = oq[q];
+    int m = (int)(oq[q + 1] - q0);
+    if (m > 64 * nwmax) m = 0;
+    for (int i = threadIdx.x; i < m; i += 256) qs[i] = bq[q0 + i];
+    __syncthreads();
+    const u32 c = threadIdx.x;
+    u64 *row = peq + ((size_t)q * 256 + c) * nwmax;
+    for (int w = 0; w < nwmax; w++) {
+        u64 word = 0;
+        const int e = m < 64 * w + 64 ? m : 64 * w + 64;
+        for (int i = 64 * w; i < e; i++) word |= (u64)(qs[i] == c) << (i - 64 * w);
+        row[w] = word;
+    }
+}
+
+/* Compacted mode, one (query, choice block) tile for one wave (c0 a multiple
+ * of 64). The 32-row stage runs in lockstep passes of up to 64 choices, so the
+ * lanes of a pass start together and read the same stripe dwords (coalesced),
+ * and a pass ends as soon as few lanes still work -- their choices are parked
+ * with their state and later continue in a pass of 64 parked choices:
+ *   * a window is the next 64 choices of the tile (one stripe group): lengths
+ *     and the first two text dwords come in with coalesced loads; the length
+ *     gap (|n - m| > k) and empty strings are answered at once;
+ *   * a window with at least CD_DIRECT in-band choices is scanned in place,
+ *     a sparser one is parked whole;
+ *   * a pass steps its lanes 4 columns at a time until at most CD_PARK are
+ *     left (none once the tile's windows are used up), then parks those;
+ *   * a choice whose next test would need rows past 32 goes on the escalation
+ *     stack, drained 64 at a time through the wider stages (peq_lane from the
+ *     64-row stage, Peq rows from the query's global table).
+ * Needs m <= 32 or k >= 0 (otherwise every choice would escalate: the caller
+ * runs strided). */
+template <int NW>
+__device__ void cdist_tile(WaveLds &L, const u64 *__restrict__ gpeq, int m,
+                           const u32 *__restrict__ words, const int64_t *__restrict__ gbase,
+                           const int32_t *__restrict__ clen, int64_t c0, int64_t c1, void *out,
                            int64_t orow, int dtype, int k)
 {
     const int lane = lane_id();
-    const int nw = NW == 1 ? 1 : (m + 63) >> 6;
-    bool active = false;
-    u64 Pv[NW], Mv[NW];
-    ByteStream ts;
-    u32 c4 = 0;
-    int j = 0, n = 0, rel = 0, dmn = 0;
-    int64_t wnext = c0;
-    int qh = 0, qt = 0;
+    const bool whole = m <= 32;
+    u8 *const out8 = (u8 *)out + orow;          /* uint8 results: the common case */
+    int64_t w = c0;
+    int ns = 0, ne = 0;
     for (;;) {
-        /* ---- refill idle lanes from the queue; top the queue up from the window */
-        for (;;) {
-            const u64 need = wave_ballot(!active);
-            const int nneed = __popcll(need);
-            if (nneed == 0) break;
-            const int avail = qt - qh;
-            if (avail >= nneed || wnext >= c1) {
-                const int r = rank_below(need);
-                if (!active && r < avail) {
-                    const QEnt e = queue[(qh + r) & (QCAP - 1)];
-                    n = e.n;
-                    rel = e.rel;
-                    dmn = m - n;
-                    ts.open(bc + e.start, n);
-                    c4 = ts.next4();
-                    j = 0;
-#pragma unroll
-                    for (int w = 0; w < NW; w++) { Pv[w] = ~(u64)0; Mv[w] = 0; }
-                    active = true;
-                }
-                qh = uni(qh + (avail < nneed ? avail : nneed));
-                break;
+        /* ---- escalations: a full wave of them, or the rest at the very end */
+        const bool tile_done = w >= c1 && ns == 0;
+        if (ne >= 64 || (tile_done && ne > 0)) {
+            const int cnt = ne < 64 ? ne : 64;
+            const int base = ne - cnt;
+            if (lane < cnt) {
+                const int64_t c = c0 + L.esc[base + lane];
+                const StripeText text{words + gbase[c >> 6] + (c & 63)};
+                store_out(out, orow + c,
+                          peq_lane<NW, 1>((const u32 *)nullptr, gpeq, m, text, clen[c], k), dtype);
             }
-            const int64_t c = wnext + lane;
-            const bool valid = c < c1;
-            int64_t o0 = 0, o1 = 0;
-            if (valid) { o0 = oc[c]; o1 = oc[c + 1]; }
-            const int nn = (int)(o1 - o0);
-            const int gap = nn > m ? nn - m : m - nn;
-            const bool decided = valid && ((k >= 0 && gap > k) || nn == 0);
-            if (decided) store_out(out, orow + c, (k >= 0 && gap > k) ? k + 1 : m, dtype);
-            const bool scan = valid && !decided;
-            const u64 sm = wave_ballot(scan);
-            if (scan) {
-                QEnt e;
-                e.start = o0;
-                e.n = nn;
-                e.rel = (int)(c - c0);
-                queue[(qt + rank_below(sm)) & (QCAP - 1)] = e;
-            }
-            qt = uni(qt + __popcll(sm));
-            wnext = uni64(wnext + 64);
+            ne = uni(base);
             wave_lds_sync();
+            continue;
         }
-        if (wave_ballot(active) == 0) break;
-        /* ---- four columns for every busy lane */
-        if (active) {
-            const u32 c4n = ts.next4();
-            const int cnt = n - j < 4 ? n - j : 4;
-            if (cnt == 4) {
-                u64 E[4][NW];
+        if (tile_done) break;
+        /* ---- the pass's lanes: the next window, or up to 64 parked choices */
+        bool act;
+        int rel, n, j;
+        const u32 *tp;
+        u32 Pv, Mv, c4, c4n;
+        if (ns < 64 && w < c1) {
+            const int64_t c = w + lane;
+            const bool valid = c < c1;
+            const u32 *gp = words + gbase[w >> 6];          /* one group: a scalar load */
+            n = valid ? clen[c] : 0;
+            tp = gp + lane;
+            rel = (int)(c - c0);
+            j = 0;
+            Pv = ~0u;
+            Mv = 0u;
+            w = uni64(w + 64);
+            const int gap = n > m ? n - m : m - n;
+            const bool decided = valid && ((k >= 0 && gap > k) || n == 0);
+            if (decided) store_out(out, orow + c, (k >= 0 && gap > k) ? k + 1 : m, dtype);
+            /* the 32-row stage cannot test even its first group: escalate now */
+            const bool direct = valid && !decided && !whole && 4 + m - n > 32;
+            act = valid && !decided && !direct;
+            const u64 dm = wave_ballot(direct);
+            if (dm) {
+                if (direct) L.esc[ne + rank_below(dm)] = rel;
+                ne = uni(ne + __popcll(dm));
+            }
+            const u64 sm = wave_ballot(act);
+            if (__popcll(sm) < CD_DIRECT) {
+                /* sparse window: park its in-band choices */
+                if (act) {
+                    Pend e;
+                    e.start = tp - words;
+                    e.rel = rel;
+                    e.n = n;
+                    e.j = 0;
+                    e.Pv = ~0u;
+                    e.Mv = 0u;
+                    e.pad = 0;
+                    L.pend[ns + rank_below(sm)] = e;
+                }
+                ns = uni(ns + __popcll(sm));
+                wave_lds_sync();
+                continue;
+            }
+            c4 = act ? tp[0] : 0u;
+            c4n = act && n > 4 ? tp[64] : 0u;
+        } else {
+            const int take = ns < 64 ? ns : 64;
+            act = lane < take;
+            rel = 0;
+            n = 1;
+            j = 0;
+            Pv = ~0u;
+            Mv = 0u;
+            tp = words;
+            if (act) {
+                const Pend e = L.pend[ns - take + lane];
+                tp = words + e.start;
+                rel = e.rel;
+                n = e.n;
+                j = e.j;
+                Pv = e.Pv;
+                Mv = e.Mv;
+            }
+            ns = uni(ns - take);
+            wave_lds_sync();                 /* entries read before their slots are reused */
+            const int dj = j >> 2, last = (n - 1) >> 2;
+            c4 = act ? tp[64 * dj] : 0u;
+            c4n = act && dj < last ? tp[64 * (dj + 1)] : 0u;
+        }
+        /* ---- lockstep groups of 4 columns until few lanes are left */
+        const int park = w < c1 ? CD_PARK : 0;
+        bool escalate = false;
+        for (;;) {
+            if (act) {
+                const int cnt = n - j < 4 ? n - j : 4;
+                const u32 cur = c4;
+                c4 = c4n;
+                /* the dword after next, for the following group (clamped to the string) */
+                const int nd = (j >> 2) + 2, last = (n - 1) >> 2;
+                c4n = nd <= last ? tp[64 * nd] : 0u;
+                u32 E[4];
 #pragma unroll
-                for (int u = 0; u < 4; u++)
+                for (int u = 0; u < 4; u++) E[u] = L.lo32[(cur >> (8 * u)) & 0xFFu];
+                u32 P1[1] = {Pv}, M1[1] = {Mv};
+                if (cnt == 4) {
 #pragma unroll
-                    for (int w = 0; w < NW; w++) E[u][w] = peq[((c4 >> (8 * u)) & 0xFFu) * NW + w];
+                    for (int u = 0; u < 4; u++) {
+                        const u32 e1[1] = {E[u]};
+                        column<u32, 1>(P1, M1, 1, e1);
+                    }
+                } else {
 #pragma unroll
-                for (int u = 0; u < 4; u++) column<NW>(Pv, Mv, nw, E[u]);
-            } else {
-#pragma unroll
-                for (int u = 0; u < 3; u++) {
-                    if (u < cnt) {
-                        u64 E[NW];
-#pragma unroll
-                        for (int w = 0; w < NW; w++) E[w] = peq[((c4 >> (8 * u)) & 0xFFu) * NW + w];
-                        column<NW>(Pv, Mv, nw, E);
+                    for (int u = 0; u < 3; u++) {
+                        if (u < cnt) {
+                            const u32 e1[1] = {E[u]};
+                            column<u32, 1>(P1, M1, 1, e1);
+                        }
+                    }
+                }
+                Pv = P1[0];
+                Mv = M1[0];
+                j += cnt;
+                /* whole: exact at j = n, bound tests with k >= 0; otherwise (k >= 0)
+                 * the bound row j + m - n is within the 32 rows by construction */
+                if (!whole || k >= 0 || j >= n) {
+                    const int bound = diag_bound<u32, 1>(P1, M1, 1, j, m, n);
+                    const bool over = k >= 0 && bound > k;
+                    if (over || (whole && j >= n)) {
+                        const int d = over ? k + 1 : bound;
+                        if (dtype == DT_UINT8) out8[c0 + rel] = (u8)(d > 255 ? 255 : d);
+                        else store_out(out, orow + c0 + rel, d, dtype);
+                        act = false;
+                    } else if (!whole && j + 4 + m - n > 32) {
+                        escalate = true;
+                        act = false;
                     }
                 }
             }
-            j += cnt;
-            if (k >= 0 || j >= n) {
-                const int bound = diag_bound<NW>(Pv, Mv, nw, j, m, m - dmn);
-                const bool over = k >= 0 && bound > k;
-                if (over || j >= n) {
-                    store_out(out, orow + c0 + rel, over ? k + 1 : bound, dtype);
-                    active = false;
-                }
-            }
-            c4 = c4n;
+            const int left = __popcll(wave_ballot(act));
+            if (left <= park) break;
         }
+        const u64 em = wave_ballot(escalate);
+        if (em) {
+            if (escalate) L.esc[ne + rank_below(em)] = rel;
+            ne = uni(ne + __popcll(em));
+        }
+        const u64 am = wave_ballot(act);
+        if (am) {
+            if (act) {
+                Pend e;
+                e.start = tp - words;
+                e.rel = rel;
+                e.n = n;
+                e.j = j;
+                e.Pv = Pv;
+                e.Mv = Mv;
+                e.pad = 0;
+                L.pend[ns + rank_below(am)] = e;
+            }
+            ns = uni(ns + __popcll(am));
+        }
+        wave_lds_sync();
     }
 }
 
-/* Strided mode: lane l takes choices c0 + l, c0 + l + 64, ... and runs each to
- * its end (the plain kernel's lane loop inside the persistent tiles). Cheaper
- * than queue refill when every choice needs about the same number of columns. */
+/* Strided mode: lane l takes choices c0 + l, c0 + l + 64, ... and runs each
+ * through all its stages (the plain kernel's lane loop inside the persistent
+ * tiles). Cheaper than the ring when every choice needs about the same work,
+ * and the only mode for exact (k < 0) queries above 32 bytes. */
 template <int NW>
-__device__ void cdist_tile_strided(const u64 *peq, int m, const u8 *__restrict__ bc,
-                                   const int64_t *__restrict__ oc, int64_t c0, int64_t c1,
+__device__ void cdist_tile_strided(const u32 *lo32, const u64 *__restrict__ gpeq, int m,
+                                   const u32 *__restrict__ words, const int64_t *__restrict__ gbase,
+                                   const int32_t *__restrict__ clen, int64_t c0, int64_t c1,
                                    void *out, int64_t orow, int dtype, int k)
 {
     for (int64_t c = c0 + lane_id(); c < c1; c += 64) {
-        const int64_t o0 = oc[c];
-        const int n = (int)(oc[c + 1] - o0);
-        store_out(out, orow + c, peq_lane<NW>(peq, m, bc + o0, n, k), dtype);
+        const StripeText text{words + gbase[c >> 6] + (c & 63)};
+        store_out(out, orow + c, peq_lane<NW, 0>(lo32, gpeq, m, text, clen[c], k), dtype);
     }
 }
 
-template <int NW>
-__device__ void build_peq_wave(u64 *peq, const u8 *__restrict__ p, int m)
-{
-    const int lane = lane_id();
-#pragma unroll
-    for (int x = 0; x < 256 * NW / 64; x++) peq[x * 64 + lane] = 0;
-    wave_lds_sync();
-    for (int i = lane; i < m; i += 64)
-        atomicOr((unsigned long long *)&peq[(int)p[i] * NW + (i >> 6)],
-                 (unsigned long long)1 << (i & 63));
-    wave_lds_sync();
-}
-
-constexpr int CD_QCAP = 128;
-
-/* Persistent cdist over queries of at most 64*NWMAX bytes. LDS per wave:
- * 256*NWMAX*8 bytes of Peq + CD_QCAP queue entries. */
+/* Persistent cdist over queries of at most 64*NWMAX bytes whose Peq tables
+ * peq_build_kernel wrote to gpeq. Waves pull (choice block, query) tiles,
+ * block-major, from the work-queue counter. */
 template <int NWMAX>
 __global__ __launch_bounds__(256) void cdist_batched_kernel(
-    const u8 *__restrict__ bq, const int64_t *__restrict__ oq, int64_t nq,
-    const u8 *__restrict__ bc, const int64_t *__restrict__ oc, int64_t nc, void *out, int dtype,
+    const int64_t *__restrict__ oq, int64_t nq, const u64 *__restrict__ gpeq,
+    const u32 *__restrict__ words, const int64_t *__restrict__ gbase,
+    const int32_t *__restrict__ clen, int64_t nc, void *out, int dtype,
     int k_all, const int32_t *__restrict__ kvec, int64_t tile, int strided,
     unsigned long long *counter)
 {
     extern __shared__ u64 lds[];
-    const int wave = threadIdx.x >> 6;
-    const int per_wave = 256 * NWMAX + CD_QCAP * (int)(sizeof(QEnt) / sizeof(u64));
-    u64 *peq = lds + wave * per_wave;
-    QEnt *queue = (QEnt *)(peq + 256 * NWMAX);
+    WaveLds &L = ((WaveLds *)lds)[threadIdx.x >> 6];
+    const int lane = lane_id();
     const int64_t nblk = (nc + tile - 1) / tile;
     const int64_t ntiles = nblk * nq;
     int64_t cur_q = -1;
-    int cur_nw = 0;
     for (;;) {
         const long long t = wave_take(counter, 1);
         if (t >= ntiles) break;
         const int64_t blk = t / nq, q = t % nq;
         const int64_t c0 = blk * tile;
         const int64_t c1 = c0 + tile < nc ? c0 + tile : nc;
-        const int64_t q0 = oq[q];
-        const int m = (int)(oq[q + 1] - q0);
+        const int m = (int)(oq[q + 1] - oq[q]);
         const int64_t orow = q * nc;
         const int k = kvec ? clamp_k_dev(kvec[q]) : k_all;
         if (m == 0 || m > 64 * NWMAX) {
             /* empty query: n, or k + 1 (csrc/gpu_kernels.cu:53-54); too long for this
              * launch: caller error, marked -1 */
-            for (int64_t c = c0 + lane_id(); c < c1; c += 64) {
-                const int nn = (int)(oc[c + 1] - oc[c]);
-                store_out(out, orow + c, m == 0 ? empty_pattern(nn, k) : -1, dtype);
-            }
+            for (int64_t c = c0 + lane; c < c1; c += 64)
+                store_out(out, orow + c, m == 0 ? empty_pattern(clen[c], k) : -1, dtype);
             continue;
         }
-        const int nwq = (m + 63) >> 6;
-        const int cls = nwq <= 1 ? 1 : nwq <= 2 ? 2 : nwq <= 4 ? 4 : 8;
-        if (q != cur_q || cls != cur_nw) {
-            if (cls == 1) build_peq_wave<1>(peq, bq + q0, m);
-            else if (NWMAX >= 2 && cls == 2) build_peq_wave<NWMAX >= 2 ? 2 : 1>(peq, bq + q0, m);
-            else if (NWMAX >= 4 && cls == 4) build_peq_wave<NWMAX >= 4 ? 4 : 1>(peq, bq + q0, m);
-            else if (NWMAX >= 8) build_peq_wave<NWMAX >= 8 ? 8 : 1>(peq, bq + q0, m);
+        const u64 *qpeq = gpeq + (size_t)q * 256 * NWMAX;
+        if (q != cur_q) {
+            wave_lds_sync();
+#pragma unroll
+            for (int x = 0; x < 4; x++) L.lo32[x * 64 + lane] = (u32)qpeq[(x * 64 + lane) * NWMAX];
+            wave_lds_sync();
             cur_q = q;
-            cur_nw = cls;
         }
-        if (strided) {
-            if (cls == 1) cdist_tile_strided<1>(peq, m, bc, oc, c0, c1, out, orow, dtype, k);
-            else if (NWMAX >= 2 && cls == 2)
-                cdist_tile_strided<NWMAX >= 2 ? 2 : 1>(peq, m, bc, oc, c0, c1, out, orow, dtype, k);
-            else if (NWMAX >= 4 && cls == 4)
-                cdist_tile_strided<NWMAX >= 4 ? 4 : 1>(peq, m, bc, oc, c0, c1, out, orow, dtype, k);
-            else if (NWMAX >= 8)
-                cdist_tile_strided<NWMAX >= 8 ? 8 : 1>(peq, m, bc, oc, c0, c1, out, orow, dtype, k);
-            continue;
-        }
-        if (cls == 1) cdist_tile<1, CD_QCAP>(peq, queue, m, bc, oc, c0, c1, out, orow, dtype, k);
-        else if (NWMAX >= 2 && cls == 2)
-            cdist_tile<NWMAX >= 2 ? 2 : 1, CD_QCAP>(peq, queue, m, bc, oc, c0, c1, out, orow, dtype, k);
-        else if (NWMAX >= 4 && cls == 4)
-            cdist_tile<NWMAX >= 4 ? 4 : 1, CD_QCAP>(peq, queue, m, bc, oc, c0, c1, out, orow, dtype, k);
-        else if (NWMAX >= 8)
-            cdist_tile<NWMAX >= 8 ? 8 : 1, CD_QCAP>(peq, queue, m, bc, oc, c0, c1, out, orow, dtype, k);
+        if (strided || (m > 32 && k < 0))
+            cdist_tile_strided<NWMAX>(L.lo32, qpeq, m, words, gbase, clen, c0, c1, out, orow,
+                                      dtype, k);
+        else
+            cdist_tile<NWMAX>(L, qpeq, m, words, gbase, clen, c0, c1, out, orow, dtype, k);
     }
 }
 
@@ -523,70 +623,92 @@ int device_cus()
 /* ======================================================================= C ABI */
 extern "C" {
 
-size_t bl_cdist_workspace_size(void) { return 256; }
+static int cdist_nwmax(int max_qlen)
+{
+    const int words = (max_qlen + 63) >> 6;
+    return words <= 1 ? 1 : words <= 2 ? 2 : words <= 4 ? 4 : 8;
+}
+
+/* Workspace: the work-queue counter (256 bytes), then the queries' Peq tables. */
+size_t bl_cdist_workspace_size(int64_t nq, int max_qlen)
+{
+    if (nq < 0 || max_qlen < 0) return 0;
+    return 256 + (size_t)nq * 256 * cdist_nwmax(max_qlen) * sizeof(u64);
+}
 
 int bl_cdist_tuned(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
-                   const uint8_t *d_buf_c, const int64_t *d_off_c, int64_t nc, int max_qlen,
-                   void *d_out, int dtype, int k, const int32_t *d_kvec, void *d_work,
-                   size_t work_bytes, const bl_cdist_params *prm, void *stream)
+                   const bl_stripes *choices, int max_qlen, void *d_out, int dtype, int k,
+                   const int32_t *d_kvec, void *d_work, size_t work_bytes,
+                   const bl_cdist_params *prm, void *stream)
 {
+    if (!choices) return fail(BL_EINVAL, "bl_cdist: null choices");
+    const int64_t nc = choices->n;
     if (nq < 0 || nc < 0 || max_qlen < 0) return fail(BL_EINVAL, "bl_cdist: negative size");
     if (nq == 0 || nc == 0) return BL_OK;
     if (dtype < BL_INT32 || dtype > BL_UINT8) return fail(BL_EINVAL, "bl_cdist: bad dtype %d", dtype);
-    if (!d_out || !d_buf_q || !d_off_q || !d_buf_c || !d_off_c)
+    if (!d_out || !d_buf_q || !d_off_q || !choices->words || !choices->gbase || !choices->len)
         return fail(BL_EINVAL, "bl_cdist: null pointer");
     if (max_qlen > 512) return fail(BL_EINVAL, "bl_cdist: queries above 512 bytes go to bl_long");
-    if (d_work && work_bytes < bl_cdist_workspace_size())
-        return fail(BL_EINVAL, "bl_cdist: workspace below %zu bytes", bl_cdist_workspace_size());
+    if (nq > 0x7fffffff) return fail(BL_EINVAL, "bl_cdist: too many queries");
+    const size_t need_ws = bl_cdist_workspace_size(nq, max_qlen);
+    if (d_work && work_bytes < need_ws)
+        return fail(BL_EINVAL, "bl_cdist: workspace of %zu bytes below the %zu needed", work_bytes,
+                    need_ws);
     k = clamp_k_dev(k);
     hipStream_t s = (hipStream_t)stream;
     const int wpc = prm && prm->waves_per_cu > 0 ? prm->waves_per_cu : 16;
-    const int64_t tile = prm && prm->tile > 0 ? prm->tile : 8192;
+    /* tiles start on stripe groups (multiples of 64 choices) */
+    const int64_t tile = (prm && prm->tile > 0 ? (prm->tile + 63) / 64 * 64 : 8192);
     const int strided = prm && prm->reserved[0] == 1 ? 1 : 0;
-    const int words = (max_qlen + 63) >> 6;
-    const int nwmax = words <= 1 ? 1 : words <= 2 ? 2 : words <= 4 ? 4 : 8;
-    const size_t per_wave = (size_t)(256 * nwmax) * 8 + (size_t)CD_QCAP * sizeof(QEnt);
-    const size_t lds = per_wave * 4;
+    const int nwmax = cdist_nwmax(max_qlen);
+    const size_t lds = sizeof(WaveLds) * 4;
     int64_t blocks = (int64_t)device_cus() * wpc / 4;
     const int64_t ntiles = ((nc + tile - 1) / tile) * nq;
     const int64_t need = (ntiles + 3) / 4;
     if (blocks > need) blocks = need;
     if (blocks < 1) blocks = 1;
-    unsigned long long *ctr = (unsigned long long *)d_work;
-    if (!ctr) BL_HIP(hipMallocAsync((void **)&ctr, 256, s));
+    unsigned char *ws = (unsigned char *)d_work;
+    if (!ws) BL_HIP(hipMallocAsync((void **)&ws, need_ws, s));
+    unsigned long long *ctr = (unsigned long long *)ws;
+    u64 *gpeq = (u64 *)(ws + 256);
     int rc = launch_clear(ctr, 1, s);
     if (rc == BL_OK) {
+        peq_build_kernel<<<(unsigned)nq, 256, 0, s>>>(d_buf_q, d_off_q, nwmax, gpeq);
         switch (nwmax) {
         case 1:
             cdist_batched_kernel<1><<<(unsigned)blocks, 256, lds, s>>>(
-                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, d_kvec, tile, strided, ctr);
+                d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, nc, d_out, dtype, k,
+                d_kvec, tile, strided, ctr);
             break;
         case 2:
             cdist_batched_kernel<2><<<(unsigned)blocks, 256, lds, s>>>(
-                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, d_kvec, tile, strided, ctr);
+                d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, nc, d_out, dtype, k,
+                d_kvec, tile, strided, ctr);
             break;
         case 4:
             cdist_batched_kernel<4><<<(unsigned)blocks, 256, lds, s>>>(
-                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, d_kvec, tile, strided, ctr);
+                d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, nc, d_out, dtype, k,
+                d_kvec, tile, strided, ctr);
             break;
         default:
             cdist_batched_kernel<8><<<(unsigned)blocks, 256, lds, s>>>(
-                d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, d_out, dtype, k, d_kvec, tile, strided, ctr);
+                d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, nc, d_out, dtype, k,
+                d_kvec, tile, strided, ctr);
             break;
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) rc = fail(BL_EHIP, "cdist_batched_kernel launch: %s", hipGetErrorString(e));
     }
-    if (!d_work) (void)hipFreeAsync(ctr, s);
+    if (!d_work) (void)hipFreeAsync(ws, s);
     return rc;
 }
 
-int bl_cdist(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq, const uint8_t *d_buf_c,
-             const int64_t *d_off_c, int64_t nc, int max_qlen, void *d_out, int dtype, int k,
-             void *d_work, size_t work_bytes, void *stream)
+int bl_cdist(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq, const bl_stripes *choices,
+             int max_qlen, void *d_out, int dtype, int k, void *d_work, size_t work_bytes,
+             void *stream)
 {
-    return bl_cdist_tuned(d_buf_q, d_off_q, nq, d_buf_c, d_off_c, nc, max_qlen, d_out, dtype, k,
-                          nullptr, d_work, work_bytes, nullptr, stream);
+    return bl_cdist_tuned(d_buf_q, d_off_q, nq, choices, max_qlen, d_out, dtype, k, nullptr, d_work,
+                          work_bytes, nullptr, stream);
 }
 
 static int64_t pairs_chunk(int64_t n, const bl_pairs_params *prm)

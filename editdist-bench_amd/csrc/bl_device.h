@@ -158,6 +158,77 @@ struct ByteStream {
     }
 };
 
+/* Text sources. A scan may restart from column 0 (row stages), so it takes a
+ * source and opens a fresh stream per stage. */
+struct ByteText {                  /* a string in a packed byte buffer */
+    const u8 *t;
+    BL_HDM ByteStream open(int n) const
+    {
+        ByteStream s;
+        s.open(t, n);
+        return s;
+    }
+};
+
+/* Choice-set stripes: the strings of group g (choices 64g .. 64g+63) are
+ * interleaved by dword, dword j of choice c at words[gbase[g] + 64 * j + c % 64],
+ * each string zero-padded to the widest of its group (gbase[g + 1] - gbase[g]
+ * = 64 * that width in dwords). A wave whose lanes hold consecutive choices
+ * reads 256 contiguous bytes per dword step. */
+struct StripeStream {
+    const u32 *p, *last;           /* next dword; the string's last dword */
+    BL_HDM u32 next4()
+    {
+        const u32 v = *(p <= last ? p : last);
+        p += 64;
+        return v;
+    }
+};
+
+/* Dword j of a string of ln bytes as the stripes hold it (zero past ln). */
+BL_HD u32 stripe_word(const u8 *s, int ln, int64_t j)
+{
+    const int64_t b0 = 4 * j;
+    if (b0 >= ln) return 0;
+    u32 v = load4(s, (int)b0, ln);
+    const int64_t r = ln - b0;
+    if (r < 4) v &= (1u << (8 * r)) - 1u;
+    return v;
+}
+
+/* Group bases of the stripes of n strings (host offsets off[0..n]) into
+ * gbase[0..ceil(n/64)]: each group is 64 x its widest string in dwords (at
+ * least 1). Returns the total dwords, or -1 for decreasing offsets. */
+static inline int64_t stripes_layout(const int64_t *off, int64_t n, int64_t *gbase)
+{
+    const int64_t groups = (n + 63) / 64;
+    int64_t acc = 0;
+    for (int64_t g = 0; g < groups; g++) {
+        int64_t widest = 0;
+        const int64_t e = (g + 1) * 64 < n ? (g + 1) * 64 : n;
+        for (int64_t c = g * 64; c < e; c++) {
+            const int64_t l = off[c + 1] - off[c];
+            if (l < 0) return -1;
+            if (l > widest) widest = l;
+        }
+        gbase[g] = acc;
+        acc += 64 * (widest > 0 ? (widest + 3) / 4 : 1);
+    }
+    gbase[groups] = acc;
+    return acc;
+}
+
+struct StripeText {
+    const u32 *first;              /* dword 0 of the string */
+    BL_HDM StripeStream open(int n) const
+    {
+        StripeStream s;
+        s.p = first;
+        s.last = first + 64 * (n > 0 ? (n - 1) >> 2 : 0);
+        return s;
+    }
+};
+
 /* ------------------------------------------------------------ output */
 BL_HD void store_out(void *out, int64_t i, int d, int dtype)
 {
@@ -167,26 +238,48 @@ BL_HD void store_out(void *out, int64_t i, int d, int dtype)
 }
 
 /* ------------------------------------------------------------ Myers column step */
-/* One text column on one 64-bit block of pattern rows. hp / hn (0 or 1) are
- * the +1 / -1 horizontal delta entering the lowest row. Ph / Mh receive the
- * deltas leaving every row (bit i = row i) before they are shifted down; a
- * following block takes bit 63 of them as its own hp / hn. */
-BL_HD void word_step(u64 &Pv, u64 &Mv, u64 Eq, u32 hp, u32 hn, u64 &Ph, u64 &Mh)
+/* Pattern rows live in blocks of 32 (u32) or 64 (u64) rows. */
+template <typename T> struct Rows;
+template <> struct Rows<u32> { static constexpr int n = 32; };
+template <> struct Rows<u64> { static constexpr int n = 64; };
+
+BL_HD int popc(u32 x)
 {
-    const u64 Xv = Eq | Mv;
-    const u64 E = Eq | (u64)hn;
-    const u64 S = (E & Pv) + Pv;
-    const u64 Xh = bop3<tt((TA ^ TB) | TC)>(S, Pv, E);          /* (S ^ Pv) | E   */
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __popc(x);
+#else
+    return __builtin_popcount(x);
+#endif
+}
+BL_HD int popc(u64 x) { return popc64(x); }
+
+/* One text column on one block of pattern rows. hp / hn (0 or 1) are the +1 /
+ * -1 horizontal delta entering the lowest row. Ph / Mh receive the deltas
+ * leaving every row (bit i = row i) before they are shifted down; a following
+ * block takes the top bit of them as its own hp / hn. Carries and shifts only
+ * move towards higher rows, so the low rows of a block never depend on the
+ * rows above them. */
+template <typename T>
+BL_HD void word_step(T &Pv, T &Mv, T Eq, u32 hp, u32 hn, T &Ph, T &Mh)
+{
+    const T Xv = Eq | Mv;
+    const T E = Eq | (T)hn;
+    const T S = (E & Pv) + Pv;
+    const T Xh = bop3<tt((TA ^ TB) | TC)>(S, Pv, E);            /* (S ^ Pv) | E   */
     Ph = bop3<tt(TA | ~(TB | TC))>(Mv, Xh, Pv);                 /* Mv | ~(Xh | Pv) */
     Mh = Pv & Xh;
-    const u64 Phs = (Ph << 1) | (u64)hp;
-    const u64 Mhs = (Mh << 1) | (u64)hn;
+    const T Phs = (T)(Ph << 1) | (T)hp;
+    const T Mhs = (T)(Mh << 1) | (T)hn;
     Pv = bop3<tt(TA | ~(TB | TC))>(Mhs, Xv, Phs);               /* Mhs | ~(Xv | Phs) */
     Mv = Phs & Xv;
 }
 
-/* Rows [0, rows) of a block: all-ones below `rows` (1 <= rows <= 64). */
-BL_HD u64 row_mask(int rows) { return rows >= 64 ? ~(u64)0 : (((u64)1 << rows) - 1); }
+/* Rows [0, r) of a block (r clamped to [0, rows of T]). */
+template <typename T>
+BL_HD T rows_below(int r)
+{
+    return r >= Rows<T>::n ? ~(T)0 : r <= 0 ? (T)0 : (T)(((T)1 << r) - 1);
+}
 
 /* ------------------------------------------------------------ bit-plane pattern */
 /* 8x8 bit transpose: byte c of the result holds, in bit r, bit c of byte r. */
@@ -243,6 +336,15 @@ BL_HD u64 eq_planes(const u64 (&pl)[8], const u32 (&m)[8])
     return ((u64)hi << 32) | lo;
 }
 
+/* Eq of the low 32 rows of a block only. */
+BL_HD u32 eq_planes_lo(const u64 (&pl)[8], const u32 (&m)[8])
+{
+    u32 lo = (u32)pl[0] ^ m[0];
+#pragma unroll
+    for (int b = 1; b < 8; b++) lo = bop3<tt(TA & ~(TB ^ TC))>(lo, (u32)pl[b], m[b]);
+    return lo;
+}
+
 /* ------------------------------------------------------------ trimming */
 /* d(xPy, xQy) == d(P, Q): drop the common prefix, then the common suffix of
  * what remains (never overlapping the prefix). 4 bytes per compare. */
@@ -285,9 +387,10 @@ BL_HD void trim_pair(const u8 *&a, int &la, const u8 *&b, int &lb)
  * This dominates the reference's bottom-row test (score - (n-1-j) > k,
  * csrc/myers.c:117, :127), so exits come earlier and results are identical:
  * when the bound exceeds k the answer is k + 1 either way. At jj = n the
- * bound is the exact distance. */
-template <int NW>
-BL_HD int diag_bound(const u64 (&Pv)[NW], const u64 (&Mv)[NW], int nw, int jj, int m, int n)
+ * bound is the exact distance. Only rows below i0 are read, so the bound is
+ * exact from any block state that holds those rows. */
+template <typename T, int NW>
+BL_HD int diag_bound(const T (&Pv)[NW], const T (&Mv)[NW], int nw, int jj, int m, int n)
 {
     const int i0 = jj + m - n;
     if (i0 <= 0) return n - m;
@@ -295,89 +398,106 @@ BL_HD int diag_bound(const u64 (&Pv)[NW], const u64 (&Mv)[NW], int nw, int jj, i
 #pragma unroll
     for (int w = 0; w < NW; w++) {
         if (w < nw) {
-            const int r = i0 - 64 * w;
-            const u64 mk = r >= 64 ? ~(u64)0 : r <= 0 ? (u64)0 : (((u64)1 << r) - 1);
-            s += popc64(Pv[w] & mk) - popc64(Mv[w] & mk);
+            const T mk = rows_below<T>(i0 - Rows<T>::n * w);
+            s += popc((T)(Pv[w] & mk)) - popc((T)(Mv[w] & mk));
         }
     }
     return s;
 }
 
-/* One text column over the first nw words, Eq words given. */
-template <int NW>
-BL_HD void column(u64 (&Pv)[NW], u64 (&Mv)[NW], int nw, const u64 (&E)[NW])
+/* One text column over the first nw blocks, Eq blocks given. */
+template <typename T, int NW>
+BL_HD void column(T (&Pv)[NW], T (&Mv)[NW], int nw, const T (&E)[NW])
 {
     u32 hp = 1, hn = 0;                    /* row 0 of a global alignment: +1 per column */
 #pragma unroll
     for (int w = 0; w < NW; w++) {
         if (NW == 1 || w < nw) {
-            u64 Ph, Mh;
-            word_step(Pv[w], Mv[w], E[w], hp, hn, Ph, Mh);
-            hp = (u32)(Ph >> 63);
-            hn = (u32)(Mh >> 63);
+            T Ph, Mh;
+            word_step<T>(Pv[w], Mv[w], E[w], hp, hn, Ph, Mh);
+            hp = (u32)(Ph >> (Rows<T>::n - 1));
+            hn = (u32)(Mh >> (Rows<T>::n - 1));
         }
     }
 }
 
-/* Scan text t (n >= 1 bytes) against a pattern of m rows held in nw <= NW
- * words; eq(c, w) yields Eq of word w for text byte c. Four columns per
- * aligned text load (the next load is issued before the current columns
- * run), the exit bound tested after each group of four.
- * EAGER: fetch the Eq words of all four columns before stepping (table
+/* Scan a text of n >= 1 bytes (from `text`) against the low R rows of a pattern of m rows,
+ * held in nw <= NW blocks of type T (R = the rows those blocks cover); eq(c, w)
+ * yields Eq of block w for text byte c. Four columns per aligned text load (the
+ * next load is issued before the current columns run), the exit bound tested
+ * after each group of four.
+ *   R >= m: the whole scan. k < 0: exact distance; k >= 0: min(d, k + 1).
+ *   R <  m (k >= 0 only): the low R rows evolve exactly as in the whole scan,
+ *          so the bound is exact while its diagonal row jj + m - n stays within
+ *          R. Returns k + 1 when it exceeds k by then, else -1 (undecided: the
+ *          caller scans again with more rows).
+ * EAGER: fetch the Eq blocks of all four columns before stepping (table
  * lookups: the four loads overlap); otherwise compute each column's Eq just
- * before its step (register bit-planes: nothing to overlap, fewer live VGPRs).
- * k < 0: exact distance; k >= 0: min(d, k + 1). */
-template <int NW, bool EAGER, typename EqFn>
-BL_HD int scan_text(int m, int nw, const u8 *t, int n, int k, EqFn eq)
+ * before its step (register bit-planes: nothing to overlap, fewer live VGPRs). */
+template <typename T, int NW, bool EAGER, typename Text, typename EqFn>
+BL_HD int scan_text(int m, int nw, int R, const Text &text, int n, int k, EqFn eq)
 {
-    u64 Pv[NW], Mv[NW];
+    T Pv[NW], Mv[NW];
 #pragma unroll
-    for (int w = 0; w < NW; w++) { Pv[w] = ~(u64)0; Mv[w] = 0; }
-    ByteStream ts;
-    ts.open(t, n);
+    for (int w = 0; w < NW; w++) { Pv[w] = ~(T)0; Mv[w] = 0; }
+    const bool whole = R >= m;
+    auto ts = text.open(n);
     u32 c4 = ts.next4();
     int j = 0;
     for (;;) {
+        /* a partial scan stops before a test it could not evaluate (it never
+         * reaches j = n: that test would need all m rows) */
+        if (!whole && j + 4 + m - n > R) return -1;
         const u32 c4n = ts.next4();
         const int cnt = n - j < 4 ? n - j : 4;
         if (cnt == 4) {
             if (EAGER) {
-                u64 E[4][NW];
+                T E[4][NW];
 #pragma unroll
                 for (int u = 0; u < 4; u++)
 #pragma unroll
                     for (int w = 0; w < NW; w++) E[u][w] = eq((c4 >> (8 * u)) & 0xFFu, w);
 #pragma unroll
-                for (int u = 0; u < 4; u++) column<NW>(Pv, Mv, nw, E[u]);
+                for (int u = 0; u < 4; u++) column<T, NW>(Pv, Mv, nw, E[u]);
             } else {
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    u64 E[NW];
+                    T E[NW];
 #pragma unroll
                     for (int w = 0; w < NW; w++) E[w] = eq((c4 >> (8 * u)) & 0xFFu, w);
-                    column<NW>(Pv, Mv, nw, E);
+                    column<T, NW>(Pv, Mv, nw, E);
                 }
             }
         } else {
 #pragma unroll
             for (int u = 0; u < 3; u++) {
                 if (u < cnt) {
-                    u64 E[NW];
+                    T E[NW];
 #pragma unroll
                     for (int w = 0; w < NW; w++) E[w] = eq((c4 >> (8 * u)) & 0xFFu, w);
-                    column<NW>(Pv, Mv, nw, E);
+                    column<T, NW>(Pv, Mv, nw, E);
                 }
             }
         }
         j += cnt;
         if (j >= n) {
-            const int d = diag_bound<NW>(Pv, Mv, nw, j, m, n);     /* exact at j = n */
+            const int d = diag_bound<T, NW>(Pv, Mv, nw, j, m, n);  /* exact at j = n */
             return (k >= 0 && d > k) ? k + 1 : d;
         }
-        if (k >= 0 && diag_bound<NW>(Pv, Mv, nw, j, m, n) > k) return k + 1;
+        if (k >= 0 && diag_bound<T, NW>(Pv, Mv, nw, j, m, n) > k) return k + 1;
         c4 = c4n;
     }
 }
+
+/* ------------------------------------------------------------ row stages */
+/* With a threshold most pairs are decided within a few columns, when the exit
+ * bound needs only the rows near the top-left corner. The lanes therefore scan
+ * in stages of growing height and stop at the first stage that decides:
+ *   32 rows (one u32 block) -> 64 rows (one u64 block) -> all rows.
+ * Each stage starts again from column 0; a stage that decides returns exactly
+ * what the whole scan returns (k + 1), and the last stage is the whole scan.
+ * Without a threshold only the last stage runs; a pattern of at most 32 rows
+ * runs entirely on u32 blocks. */
 
 /* ------------------------------------------------------------ pair lane */
 /* Distance of one pair whose shorter string (after the optional trim) has at
@@ -395,33 +515,69 @@ BL_HD int pair_lane(const u8 *a, int la, const u8 *b, int lb, int trim, int k)
     if (k >= 0 && lb - la > k) return k + 1;
     if (la == 0) return lb;
 
+    const ByteText text{b};
     u64 pl[NW][8];
+    build_planes(a, la, pl[0]);
+    auto eq32 = [&](u32 c, int) {
+        u32 m8[8];
+        text_masks(c, m8);
+        return eq_planes_lo(pl[0], m8);
+    };
+    if (la <= 32) return scan_text<u32, 1, false>(la, 1, 32, text, lb, k, eq32);
+    if (k >= 0) {
+        const int r = scan_text<u32, 1, false>(la, 1, 32, text, lb, k, eq32);
+        if (r >= 0) return r;
+    }
+    auto eq64 = [&](u32 c, int w) {
+        u32 m8[8];
+        text_masks(c, m8);
+        return eq_planes(pl[w], m8);
+    };
+    if (NW == 1 || la <= 64) return scan_text<u64, 1, false>(la, 1, 64, text, lb, k, eq64);
+    if (k >= 0) {
+        const int r = scan_text<u64, 1, false>(la, 1, 64, text, lb, k, eq64);
+        if (r >= 0) return r;
+    }
 #pragma unroll
-    for (int w = 0; w < NW; w++) {
+    for (int w = 1; w < NW; w++) {
 #pragma unroll
         for (int b8 = 0; b8 < 8; b8++) pl[w][b8] = 0;
         if (64 * w < la) build_planes(a + 64 * w, la - 64 * w, pl[w]);
     }
     const int nw = (la + 63) >> 6;
-    return scan_text<NW, false>(la, nw, b, lb, k, [&](u32 c, int w) {
-        u32 m8[8];
-        text_masks(c, m8);
-        return eq_planes(pl[w], m8);
-    });
+    return scan_text<u64, NW, false>(la, nw, 64 * nw, text, lb, k, eq64);
 }
 
 /* ------------------------------------------------------------ Peq lane */
 /* Distance of a pattern given by its Peq table (row c: NW words, peq[c*NW + w])
- * of length 1 <= m <= 64*NW against text t. No trim: the pattern is the query.
- * Semantics of text_distance_peq, csrc/gpu_myers.cuh:124-145, and run_text /
- * run_text_k, csrc/myers.c:58-77, :107-130. */
-template <int NW, typename PeqPtr>
-BL_HD int peq_lane(PeqPtr peq, int m, const u8 *t, int n, int k)
+ * of length 1 <= m <= 64*NW against a text of n bytes (a ByteText or a
+ * StripeText). No trim: the pattern is the query.
+ * lo32[c] = the low 32 bits of peq[c*NW] (a compact table for the 32-row
+ * stage). Semantics of text_distance_peq, csrc/gpu_myers.cuh:124-145, and
+ * run_text / run_text_k, csrc/myers.c:58-77, :107-130.
+ * FROM: the first stage to run (0: 32 rows, 1: 64 rows); a caller that already
+ * ran the 32-row stage passes 1 (lo32 is then not read). */
+template <int NW, int FROM, typename Lo32Ptr, typename PeqPtr, typename Text>
+BL_HD int peq_lane(Lo32Ptr lo32, PeqPtr peq, int m, const Text &t, int n, int k)
 {
     if (k >= 0 && (n > m ? n - m : m - n) > k) return k + 1;
     if (n == 0) return m;
+    if (FROM == 0) {
+        auto eq32 = [&](u32 c, int) { return (u32)lo32[c]; };
+        if (m <= 32) return scan_text<u32, 1, true>(m, 1, 32, t, n, k, eq32);
+        if (k >= 0) {
+            const int r = scan_text<u32, 1, true>(m, 1, 32, t, n, k, eq32);
+            if (r >= 0) return r;
+        }
+    }
+    auto eq64 = [&](u32 c, int w) { return (u64)peq[c * NW + w]; };
+    if (NW == 1 || m <= 64) return scan_text<u64, 1, true>(m, 1, 64, t, n, k, eq64);
+    if (k >= 0) {
+        const int r = scan_text<u64, 1, true>(m, 1, 64, t, n, k, eq64);
+        if (r >= 0) return r;
+    }
     const int nw = (m + 63) >> 6;
-    return scan_text<NW, true>(m, nw, t, n, k, [&](u32 c, int w) { return (u64)peq[c * NW + w]; });
+    return scan_text<u64, NW, true>(m, nw, 64 * nw, t, n, k, eq64);
 }
 
 /* Empty cross query: d = n, or k + 1 when k >= 0 and n > k

@@ -55,24 +55,47 @@ static int pair_host(const u8 *a, int la, const u8 *b, int lb, int trim, int k)
     return long_sim(a, la, b, lb, 1, trim, k);
 }
 
-template <int NW>
-static int peq_host_nw(const u8 *q, int m, const u8 *t, int n, int k)
+template <int NW, typename Text>
+static int peq_host_nw(const u8 *q, int m, const Text &t, int n, int k)
 {
     std::vector<u64> peq((size_t)256 * NW, 0);
+    std::vector<u32> lo32(256, 0);
     for (int i = 0; i < m; i++) peq[(size_t)q[i] * NW + (i >> 6)] |= (u64)1 << (i & 63);
-    return peq_lane<NW>(peq.data(), m, t, n, k);
+    for (int c = 0; c < 256; c++) lo32[(size_t)c] = (u32)peq[(size_t)c * NW];
+    return peq_lane<NW, 0>(lo32.data(), peq.data(), m, t, n, k);
 }
 
-static int peq_host(const u8 *q, int m, const u8 *t, int n, int k)
+/* The search lane against a text read from choice stripes (the cdist kernels'
+ * layout) or, for queries above 512 bytes, the wavefront on the packed bytes. */
+static int peq_host(const u8 *q, int m, const StripeText &st, const u8 *t, int n, int k)
 {
     if (m == 0) return empty_pattern(n, k);
     const int w = (m + 63) >> 6;
-    if (w <= 1) return peq_host_nw<1>(q, m, t, n, k);
-    if (w <= 2) return peq_host_nw<2>(q, m, t, n, k);
-    if (w <= 4) return peq_host_nw<4>(q, m, t, n, k);
-    if (w <= 8) return peq_host_nw<8>(q, m, t, n, k);
+    if (w <= 1) return peq_host_nw<1>(q, m, st, n, k);
+    if (w <= 2) return peq_host_nw<2>(q, m, st, n, k);
+    if (w <= 4) return peq_host_nw<4>(q, m, st, n, k);
+    if (w <= 8) return peq_host_nw<8>(q, m, st, n, k);
     return long_sim(q, m, t, n, 0, 0, k);
 }
+
+/* Stripes of a packed set, filled by the stripes kernel's own word rule. */
+struct HostStripes {
+    std::vector<int64_t> gbase;
+    std::vector<u32> words;
+    HostStripes(const u8 *buf, const int64_t *off, int64_t n) : gbase((size_t)((n + 63) / 64 + 1))
+    {
+        const int64_t total = stripes_layout(off, n, gbase.data());
+        words.assign((size_t)(total > 0 ? total : 0), 0xDEADBEEFu);
+        for (int64_t c = 0; c < n; c++) {
+            const int64_t g = c / 64, gb = gbase[(size_t)g];
+            const int64_t width = (gbase[(size_t)g + 1] - gb) / 64;
+            for (int64_t j = 0; j < width; j++)
+                words[(size_t)(gb + 64 * j + c % 64)] =
+                    stripe_word(buf + off[c], (int)(off[c + 1] - off[c]), j);
+        }
+    }
+    StripeText text(int64_t c) const { return StripeText{words.data() + gbase[(size_t)(c / 64)] + c % 64}; }
+};
 
 extern "C" {
 
@@ -86,7 +109,9 @@ int blh_pair(const u8 *a, int la, const u8 *b, int lb, int trim, int k)
 /* One (query, choice) through the Peq lane the cdist kernels run. */
 int blh_peq(const u8 *q, int m, const u8 *t, int n, int k)
 {
-    return peq_host(q, m, t, n, k);
+    const int64_t off[2] = {0, n};
+    const HostStripes hs(t, off, 1);
+    return peq_host(q, m, hs.text(0), t, n, k);
 }
 
 /* The wavefront path for any lengths (is_pair: trim/swap as pairs do). */
@@ -110,11 +135,13 @@ int blh_pairs(const u8 *ba, const int64_t *oa, const u8 *bb, const int64_t *ob, 
 int blh_cross(const u8 *bq, const int64_t *oq, int64_t nq, const u8 *bc, const int64_t *oc,
               int64_t nc, int32_t *out, int k, int force_long)
 {
+    const HostStripes hs(bc, oc, nc);
     for (int64_t i = 0; i < nq; i++)
         for (int64_t j = 0; j < nc; j++) {
             const u8 *q = bq + oq[i], *t = bc + oc[j];
             const int m = (int)(oq[i + 1] - oq[i]), n = (int)(oc[j + 1] - oc[j]);
-            out[i * nc + j] = force_long ? long_sim(q, m, t, n, 0, 0, k) : peq_host(q, m, t, n, k);
+            out[i * nc + j] = force_long ? long_sim(q, m, t, n, 0, 0, k)
+                                         : peq_host(q, m, hs.text(j), t, n, k);
         }
     return 0;
 }

@@ -5,7 +5,8 @@
  *                            exact trim, up to NW 64-bit words (shorter <= 64*NW).
  *                            Reference: ed_pairs_c{CAP}, csrc/gpu_kernels.cu:6-32.
  *   cdist_class_kernel<NW>   one workgroup per (query, choice block); the query's
- *                            256 x NW Peq table in LDS; lanes stride over choices.
+ *                            256 x NW Peq table (and its low-32-row copy) in LDS;
+ *                            lanes stride over choices.
  *                            Reference: ed_cross_w{NW}, csrc/gpu_kernels.cu:36-69.
  *   long_kernel              one wave per long (pattern, text), anti-diagonal
  *                            wavefront over 64-row words; replaces the reference's
@@ -21,6 +22,7 @@ using namespace bl;
 namespace {
 
 constexpr int THREADS = 256;
+static_assert(THREADS == 256, "cdist_class_kernel fills its 256-entry tables one entry per thread");
 
 static inline int clamp_k(int k) { return k < 0 ? -1 : (k > (1 << 30) ? (1 << 30) : k); }
 
@@ -46,10 +48,11 @@ __global__ __launch_bounds__(THREADS) void pairs_class_kernel(
 template <int NW>
 __global__ __launch_bounds__(THREADS) void cdist_class_kernel(
     const u8 *__restrict__ bq, const int64_t *__restrict__ oq, const int32_t *__restrict__ qidx,
-    const u8 *__restrict__ bc, const int64_t *__restrict__ oc, int64_t nc, void *out, int dtype,
-    int k)
+    const u32 *__restrict__ words, const int64_t *__restrict__ gbase,
+    const int32_t *__restrict__ clen, int64_t nc, void *out, int dtype, int k)
 {
     __shared__ u64 peq[256 * NW];
+    __shared__ u32 lo32[256];               /* rows 0..31: the 32-row stage's table */
     const int64_t q = qidx[blockIdx.x];
     const int64_t q0 = oq[q];
     const int m = (int)(oq[q + 1] - q0);
@@ -60,11 +63,13 @@ __global__ __launch_bounds__(THREADS) void cdist_class_kernel(
         atomicOr((unsigned long long *)&peq[(int)bq[q0 + i] * NW + (i >> 6)],
                  (unsigned long long)1 << (i & 63));
     __syncthreads();
+    lo32[threadIdx.x] = (u32)peq[threadIdx.x * NW];
+    __syncthreads();
     const int64_t stride = (int64_t)gridDim.y * THREADS;
     for (int64_t c = (int64_t)blockIdx.y * THREADS + threadIdx.x; c < nc; c += stride) {
-        const int64_t c0 = oc[c];
-        const int n = (int)(oc[c + 1] - c0);
-        const int d = !fits ? -1 : m == 0 ? empty_pattern(n, k) : peq_lane<NW>(peq, m, bc + c0, n, k);
+        const int n = clen[c];
+        const StripeText text{words + gbase[c >> 6] + (c & 63)};
+        const int d = !fits ? -1 : m == 0 ? empty_pattern(n, k) : peq_lane<NW, 0>(lo32, peq, m, text, n, k);
         store_out(out, q * nc + c, d, dtype);
     }
 }
@@ -165,14 +170,15 @@ int bl_pairs_class(int cap, const uint8_t *d_buf_a, const int64_t *d_off_a,
 }
 
 int bl_cdist_class(int nw, const uint8_t *d_buf_q, const int64_t *d_off_q,
-                   const int32_t *d_qidx, int64_t nqc, const uint8_t *d_buf_c,
-                   const int64_t *d_off_c, int64_t nc, void *d_out, int dtype, int k,
-                   void *stream)
+                   const int32_t *d_qidx, int64_t nqc, const bl_stripes *choices, void *d_out,
+                   int dtype, int k, void *stream)
 {
+    if (!choices) return fail(BL_EINVAL, "bl_cdist_class: null choices");
+    const int64_t nc = choices->n;
     if (nqc < 0 || nc < 0) return fail(BL_EINVAL, "bl_cdist_class: negative count");
     if (nqc == 0 || nc == 0) return BL_OK;
     if (int rc = check_common(d_out, dtype, "bl_cdist_class")) return rc;
-    if (!d_buf_q || !d_off_q || !d_qidx || !d_buf_c || !d_off_c)
+    if (!d_buf_q || !d_off_q || !d_qidx || !choices->words || !choices->gbase || !choices->len)
         return fail(BL_EINVAL, "bl_cdist_class: null input");
     if (nqc > 0x7fffffff) return fail(BL_EINVAL, "bl_cdist_class: too many queries");
     k = clamp_k(k);
@@ -182,20 +188,24 @@ int bl_cdist_class(int nw, const uint8_t *d_buf_q, const int64_t *d_off_q,
     hipStream_t s = (hipStream_t)stream;
     switch (nw) {
     case 1:
-        cdist_class_kernel<1><<<grid, THREADS, 0, s>>>(d_buf_q, d_off_q, d_qidx, d_buf_c, d_off_c,
-                                                       nc, d_out, dtype, k);
+        cdist_class_kernel<1><<<grid, THREADS, 0, s>>>(d_buf_q, d_off_q, d_qidx, choices->words,
+                                                       choices->gbase, choices->len, nc, d_out,
+                                                       dtype, k);
         break;
     case 2:
-        cdist_class_kernel<2><<<grid, THREADS, 0, s>>>(d_buf_q, d_off_q, d_qidx, d_buf_c, d_off_c,
-                                                       nc, d_out, dtype, k);
+        cdist_class_kernel<2><<<grid, THREADS, 0, s>>>(d_buf_q, d_off_q, d_qidx, choices->words,
+                                                       choices->gbase, choices->len, nc, d_out,
+                                                       dtype, k);
         break;
     case 4:
-        cdist_class_kernel<4><<<grid, THREADS, 0, s>>>(d_buf_q, d_off_q, d_qidx, d_buf_c, d_off_c,
-                                                       nc, d_out, dtype, k);
+        cdist_class_kernel<4><<<grid, THREADS, 0, s>>>(d_buf_q, d_off_q, d_qidx, choices->words,
+                                                       choices->gbase, choices->len, nc, d_out,
+                                                       dtype, k);
         break;
     case 8:
-        cdist_class_kernel<8><<<grid, THREADS, 0, s>>>(d_buf_q, d_off_q, d_qidx, d_buf_c, d_off_c,
-                                                       nc, d_out, dtype, k);
+        cdist_class_kernel<8><<<grid, THREADS, 0, s>>>(d_buf_q, d_off_q, d_qidx, choices->words,
+                                                       choices->gbase, choices->len, nc, d_out,
+                                                       dtype, k);
         break;
     default:
         return fail(BL_EINVAL, "bl_cdist_class: nw must be 1, 2, 4 or 8 (got %d)", nw);

@@ -1,0 +1,79 @@
+/*
+ * bl_stripes.hip -- choice sets laid out for the search kernels (gfx950).
+ *
+ * The search kernels give consecutive choices to consecutive lanes. In the
+ * packed layout (csrc/edpy.c:200-224: one byte buffer + int64 offsets) the
+ * lanes of a wave then read 64 different cache lines for every 4 text bytes,
+ * and the vector memory pipeline, not the arithmetic, sets the pace. Stripes
+ * interleave each group of 64 choices by dword:
+ *
+ *     dword j of choice c  at  words[gbase[c / 64] + 64 * j + c % 64]
+ *
+ * so a wave stepping its lanes' texts by one dword reads 256 contiguous bytes.
+ * Every string of a group is zero-padded to the group's widest (at least one
+ * dword), and len[c] keeps its length. Built once per resident choice set.
+ */
+#include <hip/hip_runtime.h>
+
+#include "bl_device.h"
+#include "bl_internal.h"
+
+using namespace bl;
+
+namespace {
+
+/* One wave per group of 64 choices. */
+__global__ __launch_bounds__(256) void stripes_build_kernel(const u8 *__restrict__ buf,
+                                                            const int64_t *__restrict__ off,
+                                                            int64_t n,
+                                                            const int64_t *__restrict__ gbase,
+                                                            u32 *__restrict__ words,
+                                                            int32_t *__restrict__ len)
+{
+    const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = (int)(threadIdx.x & 63);
+    if (g >= (n + 63) >> 6) return;
+    const int64_t c = g * 64 + lane;
+    const int64_t gb = gbase[g];
+    const int64_t width = (gbase[g + 1] - gb) >> 6;
+    int ln = 0;
+    const u8 *s = buf;
+    if (c < n) {
+        const int64_t o = off[c];
+        ln = (int)(off[c + 1] - o);
+        s = buf + o;
+        len[c] = ln;
+    }
+    for (int64_t j = 0; j < width; j++) words[gb + 64 * j + lane] = stripe_word(s, ln, j);
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t bl_stripes_layout(const int64_t *h_off, int64_t n, int64_t *h_gbase)
+{
+    if (n < 0) return fail(BL_EINVAL, "bl_stripes_layout: negative count");
+    if (!h_off || !h_gbase) return fail(BL_EINVAL, "bl_stripes_layout: null pointer");
+    const int64_t total = stripes_layout(h_off, n, h_gbase);
+    if (total < 0) return fail(BL_EINVAL, "bl_stripes_layout: offsets decrease");
+    return total;
+}
+
+int bl_stripes_build(const uint8_t *d_buf, const int64_t *d_off, int64_t n, const int64_t *d_gbase,
+                     uint32_t *d_words, int32_t *d_len, void *stream)
+{
+    if (n < 0) return fail(BL_EINVAL, "bl_stripes_build: negative count");
+    if (n == 0) return BL_OK;
+    if (!d_buf || !d_off || !d_gbase || !d_words || !d_len)
+        return fail(BL_EINVAL, "bl_stripes_build: null pointer");
+    const int64_t groups = (n + 63) / 64;
+    const int64_t blocks = (groups + 3) / 4;
+    if (blocks > 0x7fffffff) return fail(BL_EINVAL, "bl_stripes_build: too many strings");
+    stripes_build_kernel<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(d_buf, d_off, n, d_gbase,
+                                                                            d_words, d_len);
+    BL_LAUNCHED("stripes_build_kernel");
+    return BL_OK;
+}
+
+}  /* extern "C" */

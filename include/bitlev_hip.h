@@ -87,15 +87,40 @@ int bl_pairs_class(int cap, const uint8_t *d_buf_a, const int64_t *d_off_a,
                    const uint8_t *d_buf_b, const int64_t *d_off_b, const int32_t *d_idx,
                    int64_t cnt, void *d_out, int dtype, int trim, int k, void *stream);
 
+/* ---------------------------------------------------------------- choice stripes */
+/* A choice set laid out for the search kernels: each group of 64 choices is
+ * interleaved by dword, dword j of choice c at words[gbase[c / 64] + 64 * j +
+ * c % 64] (zero-padded to the group's widest string, at least one dword), so
+ * the lanes of a wave, holding consecutive choices, read 256 contiguous bytes
+ * per step. len[c] is the length of choice c. All pointers are device memory.
+ * The reference searches the packed buffer directly (csrc/gpu_kernels.cu:62-68,
+ * strided byte loads); the stripes are this library's own resident format. */
+typedef struct {
+    const uint32_t *words;   /* gbase[ceil(n / 64)] dwords */
+    const int64_t *gbase;    /* ceil(n / 64) + 1 group bases, in dwords */
+    const int32_t *len;      /* n lengths */
+    int64_t n;               /* number of choices */
+} bl_stripes;
+
+/* Host: group bases of the stripes of n strings with host offsets
+ * h_off[0..n] into h_gbase[0..ceil(n/64)]. Returns the number of dwords the
+ * words array needs (h_gbase[ceil(n/64)]), or a negative status. */
+int64_t bl_stripes_layout(const int64_t *h_off, int64_t n, int64_t *h_gbase);
+
+/* Device: fill words and len from packed strings (d_buf, d_off) using the
+ * group bases d_gbase that bl_stripes_layout computed for the same offsets. */
+int bl_stripes_build(const uint8_t *d_buf, const int64_t *d_off, int64_t n,
+                     const int64_t *d_gbase, uint32_t *d_words, int32_t *d_len, void *stream);
+
 /* One cdist class: queries d_qidx[0..nqc) (each at most 64*nw bytes, nw in
- * {1, 2, 4, 8}) against all nc choices; out is the row-major nq x nc matrix of
- * the call (out[q * nc + c]). One workgroup per (query, choice block), the
- * query's Peq table in LDS. Replaces ed_cross_w{1,2,4,8}
- * (csrc/gpu_kernels.cu:59-69) and their launch in bitlev/gpu.py:121-130. */
+ * {1, 2, 4, 8}) against all choices; out is the row-major nq x nc matrix of
+ * the call (out[q * nc + c], nc = choices->n). One workgroup per (query,
+ * choice block), the query's Peq table in LDS, lanes over consecutive choices
+ * of the stripes. Replaces ed_cross_w{1,2,4,8} (csrc/gpu_kernels.cu:59-69) and
+ * their launch in bitlev/gpu.py:121-130. */
 int bl_cdist_class(int nw, const uint8_t *d_buf_q, const int64_t *d_off_q,
-                   const int32_t *d_qidx, int64_t nqc, const uint8_t *d_buf_c,
-                   const int64_t *d_off_c, int64_t nc, void *d_out, int dtype, int k,
-                   void *stream);
+                   const int32_t *d_qidx, int64_t nqc, const bl_stripes *choices, void *d_out,
+                   int dtype, int k, void *stream);
 
 /* Long strings on the GPU: pairs whose shorter string exceeds 256 bytes
  * (is_pair = 1: d_idx lists pair indices; trim applies) or cdist queries
@@ -125,27 +150,34 @@ typedef struct {
     int reserved[4];       /* [0]: pairs per bucketing chunk (default 65536) */
 } bl_pairs_params;
 
-/* Bytes of device workspace bl_cdist needs (the work-queue counter). */
-size_t bl_cdist_workspace_size(void);
+/* Bytes of device workspace bl_cdist needs for nq queries of at most
+ * max_qlen bytes: the work-queue counter and one Peq table per query
+ * (256 x ceil(max_qlen / 64) words, rounded up to 1, 2, 4 or 8). 0 for a
+ * negative argument. */
+size_t bl_cdist_workspace_size(int64_t nq, int max_qlen);
 
-/* All queries (each at most max_qlen <= 512 bytes) against all nc choices in
- * one persistent launch: waves pull (choice block, query) tiles from a work
- * queue, build the query's Peq in their own LDS slice, and refill idle lanes
- * with in-band choices (length-gap and empty choices are answered without
- * occupying a lane). Row-major out[q * nc + c]. d_work: >= bl_cdist_workspace_size()
- * bytes of device memory the call may overwrite, or NULL for a stream-ordered
- * allocation. Queries above 512 bytes go to bl_long. Replaces the per-class
- * ed_cross_w{NW} launches and host split of bitlev/gpu.py:116-130. */
+/* All queries (each at most max_qlen <= 512 bytes) against all nc choices:
+ * a pre-pass writes every query's Peq table to the workspace, then one
+ * persistent launch in which waves pull (choice block, query) tiles from a work
+ * queue. Lanes scan the low 32 pattern rows first (the table's low dwords,
+ * copied into the wave's LDS slice) and refill from a per-wave ring of in-band
+ * choices as they finish (length-gap and empty choices are answered without
+ * occupying a lane); the few choices that stage cannot decide are rescanned
+ * with all rows. Row-major out[q * nc + c], nc = choices->n. d_work: >=
+ * bl_cdist_workspace_size(nq, max_qlen) bytes of device memory the call may
+ * overwrite, or NULL for a stream-ordered allocation. Queries above 512 bytes
+ * go to bl_long. Replaces the per-class ed_cross_w{NW} launches and host split
+ * of bitlev/gpu.py:116-130. */
 int bl_cdist(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
-             const uint8_t *d_buf_c, const int64_t *d_off_c, int64_t nc, int max_qlen,
-             void *d_out, int dtype, int k, void *d_work, size_t work_bytes, void *stream);
+             const bl_stripes *choices, int max_qlen, void *d_out, int dtype, int k,
+             void *d_work, size_t work_bytes, void *stream);
 /* bl_cdist with an optional per-query threshold d_kvec[q] (NULL: k for all)
  * and an explicit launch shape (NULL: defaults). Lets one launch serve
  * requests with different max_dist. */
 int bl_cdist_tuned(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
-                   const uint8_t *d_buf_c, const int64_t *d_off_c, int64_t nc, int max_qlen,
-                   void *d_out, int dtype, int k, const int32_t *d_kvec, void *d_work,
-                   size_t work_bytes, const bl_cdist_params *params, void *stream);
+                   const bl_stripes *choices, int max_qlen, void *d_out, int dtype, int k,
+                   const int32_t *d_kvec, void *d_work, size_t work_bytes,
+                   const bl_cdist_params *params, void *stream);
 
 /* Bytes of device workspace bl_pairs needs for n pairs whose strings are at
  * most max_len bytes (descriptors, bucket keys, sorted order, counters, and

@@ -147,3 +147,18 @@ def test_no_product_module_imports_the_oracle():
     for path in glob.glob(os.path.join(PKG, "bitlev", "*.py")):
         text = open(path).read()
         assert "oracle" not in text.replace("oracle.py", ""), path
+
+
+def test_stripes_layout_on_host():
+    """bl_stripes_layout: groups of 64 strings, each 64 x its widest string in
+    dwords (at least one); the last entry is the total."""
+    lens = np.array([0] * 64 + [1, 4, 5, 200] + [3] * 60 + [9], np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    g = np.empty((len(lens) + 63) // 64 + 1, np.int64)
+    total = _capi.call("bl_stripes_layout", off.ctypes.data, len(lens), g.ctypes.data)
+    assert g.tolist() == [0, 64, 64 + 64 * 50, 64 + 64 * 50 + 64 * 3]
+    assert total == g[-1]
+    bad = np.array([0, 5, 3], np.int64)
+    with pytest.raises(ValueError):
+        _capi.call("bl_stripes_layout", bad.ctypes.data, 2, g.ctypes.data)
+    assert _capi.call("bl_stripes_layout", off.ctypes.data, 0, g.ctypes.data) == 0

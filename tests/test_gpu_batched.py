@@ -57,6 +57,37 @@ def test_every_cdist_launch_shape_gives_the_same_results():
         np.testing.assert_array_equal(out.get(), O.saturate(want, np.uint8), err_msg=str(p))
 
 
+def test_cdist_stage_escalation_paths():
+    """Queue mode hands a choice the 32-row stage cannot decide to the wider
+    stages: at once when even its first test needs rows past 32 (queries above
+    36 bytes with much shorter choices), after some columns otherwise, and in
+    drains of 64 when whole tiles escalate (a threshold no prefix can exceed).
+    Queries of at most 32 bytes never escalate."""
+    rng = np.random.default_rng(12)
+    Qs = [bytes(rng.integers(0, 4, m).astype(np.uint8) + 97) for m in (5, 32, 33, 40, 64, 100, 200, 256)]
+    Cs = []
+    for i in range(3000):
+        q = Qs[i % len(Qs)]
+        if i % 3 == 0:
+            Cs.append(q[: max(0, len(q) - int(rng.integers(0, 60)))])
+        elif i % 3 == 1:
+            b = bytearray(q)
+            for _ in range(int(rng.integers(0, 6))):
+                b[int(rng.integers(0, len(b)))] = 97 + int(rng.integers(0, 4))
+            Cs.append(bytes(b))
+        else:
+            Cs.append(bytes(rng.integers(0, 4, int(rng.integers(0, 260))).astype(np.uint8) + 97))
+    pq, pc = O.pack(Qs), O.pack(Cs)
+    Q, C = hip.Packed(*pq), hip.Packed(*pc)
+    for k in (0, 2, 8, 30, 60, 300):
+        want = O.cross(pq, pc, k=k, threads=8)
+        for p in ({"mode": "queue", "tile": 512}, {"mode": "queue", "tile": 8192},
+                  {"mode": "strided", "tile": 512}):
+            out = hip.DeviceArray.empty((Q.n, C.n), np.int32)
+            hip.cdist_batched_into(Q, C, out, k, None, None, p)
+            np.testing.assert_array_equal(out.get(), want, err_msg=f"k={k} {p}")
+
+
 def test_per_item_thresholds():
     pa, pb = _mixed_pairs(2, 3000)
     A, B = hip.Packed(*pa), hip.Packed(*pb)

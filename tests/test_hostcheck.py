@@ -125,6 +125,46 @@ def test_wavefront_multi_stripe(hc):
     np.testing.assert_array_equal(hc_cross(hc, O.pack(A[:3]), O.pack(B[3:]), -1, 1), want)
 
 
+def _near(rng, a, edits, alpha):
+    b = bytearray(a)
+    for _ in range(edits):
+        op = int(rng.integers(0, 3))
+        pos = int(rng.integers(0, len(b) + 1))
+        ch = int(rng.integers(0, alpha)) + 97
+        if op == 0 and pos < len(b):
+            b[pos] = ch
+        elif op == 1:
+            b.insert(pos, ch)
+        elif pos < len(b):
+            del b[pos]
+    return bytes(b)
+
+
+def test_row_stages_at_boundaries(hc):
+    """Thresholded scans run 32 rows, then 64, then all rows; a stage that
+    cannot decide hands over when the exit bound's diagonal row leaves it.
+    Pattern lengths around every stage edge, texts shorter and longer than
+    the pattern by up to the threshold, near duplicates (which survive the
+    short stages) and unrelated strings (which exit in them)."""
+    rng = np.random.default_rng(33)
+    A, B = [], []
+    for m in (1, 31, 32, 33, 36, 60, 63, 64, 65, 68, 96, 127, 128, 129, 192, 200, 255, 256):
+        for alpha in (2, 4, 26):
+            a = bytes(rng.integers(0, alpha, m).astype(np.uint8) + 97)
+            for edits in (0, 1, 2, 5, 12, 40):
+                A.append(a)
+                B.append(_near(rng, a, edits, alpha))
+            A.append(a)
+            B.append(bytes(rng.integers(0, alpha, max(0, m + int(rng.integers(-8, 9)))).astype(np.uint8) + 97))
+    pa, pb = O.pack(A), O.pack(B)
+    for k in (0, 1, 2, 3, 4, 8, 16, 28, 31, 40, 64, 100):
+        want = O.pairs(pa, pb, k=k)
+        for trim in (0, 1):
+            np.testing.assert_array_equal(hc_pairs(hc, pa, pb, k, trim), want, err_msg=f"k={k}")
+        np.testing.assert_array_equal(np.diag(hc_cross(hc, pa, pb, k)), O.pairs(pa, pb, k=k, trim=False),
+                                      err_msg=f"cross k={k}")
+
+
 def test_random_against_oracle(hc):
     rng = np.random.default_rng(21)
     A, B = [], []
