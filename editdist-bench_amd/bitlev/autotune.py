@@ -5,10 +5,12 @@ pipeline on a sample of the batch (HIP events, median of a few runs), keeps
 the fastest and caches it per device and workload signature in a JSON file
 ($BITLEV_TUNE_CACHE, default ~/.cache/bitlev/autotune.json).
 
-cdist: lane mode (queue refill vs strided lanes), persistent waves per CU,
-choices per tile. pairs: persistent waves per CU, workgroup size, text-length
-bucket width, bucketing chunk. Every candidate computes the same results;
-only speed differs.
+cdist: lane mode (compacted lockstep passes vs strided lanes), persistent
+waves per CU, choices per tile. pairs: direct (one launch) or bucketed, and
+for bucketed persistent waves per CU, workgroup size, text-length bucket
+width, bucketing chunk; and whether to trim common affixes (which never
+changes a distance). Every candidate computes the same results; only speed
+differs.
 """
 from __future__ import annotations
 
@@ -26,7 +28,10 @@ from ._capi import call
 CDIST_SPACE = {"mode": ["strided", "queue"], "waves_per_cu": [8, 16, 32],
                "tile": [2048, 8192, 32768]}
 PAIRS_SPACE = {"waves_per_cu": [8, 16, 32], "block": [64, 256], "bin_width": [8, 16, 32],
-               "chunk": [16384, 65536, 262144]}
+               "chunk": [16384, 65536], "trim": [True, False]}
+# direct mode (one launch, no bucketing): only the launch shape matters
+PAIRS_DIRECT_SPACE = {"mode": ["direct"], "waves_per_cu": [8, 16, 32], "block": [64, 256],
+                      "trim": [True, False]}
 
 _lock = threading.Lock()
 
@@ -146,7 +151,10 @@ def tune_pairs(A: hip.Packed, B: hip.Packed, out: hip.DeviceArray, k: int = -1, 
     Bs = B if n == B.n else hip.Packed(*_head(B, n))
     ws = hip.Workspace()
     res = []
-    for p in _grid(space or PAIRS_SPACE):
+    cands = list(_grid(space)) if space else list(_grid(PAIRS_DIRECT_SPACE)) + list(_grid(PAIRS_SPACE))
+    if not trim:                       # the caller asked for no trimming: keep it off
+        cands = [p for p in cands if p.get("trim", True)]
+    for p in cands:
         ms = _time(lambda: hip.pairs_batched_into(As, Bs, out, k, trim, stream, ws, p), stream)
         res.append((ms, p))
         if log:

@@ -257,6 +257,7 @@ def _pairs_params(params):
     p.block = int(params.get("block", 0))
     p.bin_width = int(params.get("bin_width", 0))
     p.reserved[0] = int(params.get("chunk", 0))
+    p.reserved[1] = 1 if params.get("mode") == "direct" else 0
     return ctypes.byref(p)
 
 
@@ -264,11 +265,17 @@ def pairs_batched_into(A: Packed, B: Packed, out: DeviceArray, k: int = -1, trim
                        stream=None, workspace: Workspace | None = None, params=None,
                        kvec: DeviceBuffer | None = None):
     """One bl_pairs pipeline (bucketing + persistent launch) into `out`;
-    kvec: optional device int32 per-pair thresholds (overrides k)."""
+    kvec: optional device int32 per-pair thresholds (overrides k).
+    Trimming never changes a distance (d(xPy, xQy) = d(P, Q)), so where the
+    caller allows it (trim=True) the launch shape may still switch it off
+    (params["trim"] = False) when scanning the affixes is cheaper than
+    comparing them."""
     if A.n != B.n:
         raise ValueError(f"length mismatch: {A.n} vs {B.n}")
     if A.n == 0:
         return
+    if params is not None and not params.get("trim", True):
+        trim = False
     max_len = max(A.max_len, B.max_len)
     ws = (workspace or _workspace(stream)).get(lib().bl_pairs_workspace_size(A.n, max_len))
     call("bl_pairs_tuned", A.d_buf.ptr, A.d_off.ptr, B.d_buf.ptr, B.d_off.ptr, A.n, max_len,

@@ -74,6 +74,11 @@ class Request:
             raise self._error
         return self._result
 
+    def release(self):
+        """Drop this request's hold on its result (the device memory of a fused
+        output is freed once every request of the batch has let go of it)."""
+        self._result = None
+
     @property
     def latency(self):
         return None if self.t_done is None else self.t_done - self.t_submit
@@ -221,7 +226,8 @@ class Server:
             return out + inp + 4 * n + self._exec.pairs_workspace(n, mx)
         nq = sum(len(r.payload[1]) - 1 for r in reqs)
         inp = sum(_span(r.payload) + 8 * len(r.payload[1]) for r in reqs)
-        return out + inp + 4 * nq + 256
+        mq = max(_longest(r.payload) for r in reqs)
+        return out + inp + 4 * nq + self._exec.cdist_workspace(nq, mq)
 
     def _take_batch(self):
         """FIFO head and every later pending request it can fuse with."""
@@ -313,6 +319,10 @@ class _GpuExecutor:
     def pairs_workspace(self, n, max_len):
         from ._capi import lib
         return int(lib().bl_pairs_workspace_size(n, max_len))
+
+    def cdist_workspace(self, nq, max_qlen):
+        from ._capi import lib
+        return int(lib().bl_cdist_workspace_size(nq, min(max_qlen, hip.BATCHED_MAX_QUERY)))
 
     def close(self, slots):
         for s in slots:

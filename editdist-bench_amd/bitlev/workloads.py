@@ -26,6 +26,45 @@ WORKLOADS = {
 }
 
 
+# Serving mix: resident choice sets (generator 5: lengths 16-128, 'a'-'z');
+# requests half pairwise (generator 6: a of length 8-256, b = a edited at a
+# per-pair rate 5-30%) and half cdist (1-64 queries of generator 5 against one
+# set); max_dist uniform over {0, 1, 2, 3, 4, none}, dtype over
+# {uint8, uint16, int32}.
+SERVING = dict(sets=4, set_size=1_000_000, max_pairs=100_000, max_queries=64,
+               ks=(0, 1, 2, 3, 4, None), dtypes=("uint8", "uint16", "int32"))
+
+
+def serving_sets(seed: int = SEED, n_sets=None, set_size=None, threads=None):
+    """The resident choice sets of the serving workload -> [(buf, off)]."""
+    n_sets = n_sets or SERVING["sets"]
+    set_size = set_size or SERVING["set_size"]
+    return [generate(5, set_size, seed, 100 + s, 0, threads) for s in range(n_sets)]
+
+
+def serving_requests(n: int, seed: int = SEED, n_sets=None, threads=None):
+    """n request descriptions of the serving mix, each a dict with kind,
+    k (None = exact), dtype and the packed payload; cdist requests name a set
+    index."""
+    n_sets = n_sets or SERVING["sets"]
+    rng = np.random.default_rng(seed ^ 0x5E21)
+    out = []
+    for i in range(n):
+        k = SERVING["ks"][int(rng.integers(len(SERVING["ks"])))]
+        dt = SERVING["dtypes"][int(rng.integers(len(SERVING["dtypes"])))]
+        rs = seed + 7919 * (i + 1)
+        if rng.random() < 0.5:
+            npairs = int(rng.integers(1, SERVING["max_pairs"] + 1))
+            out.append(dict(kind="pairwise", k=k, dtype=dt,
+                            payload=(generate(6, npairs, rs, 0, 0, threads),
+                                     generate(6, npairs, rs, 1, 0, threads))))
+        else:
+            nq = int(rng.integers(1, SERVING["max_queries"] + 1))
+            out.append(dict(kind="cdist", k=k, dtype=dt, set=int(rng.integers(n_sets)),
+                            payload=generate(5, nq, rs, 200, 0, threads)))
+    return out
+
+
 def lib():
     global _lib
     if _lib is None:

@@ -401,24 +401,26 @@ __global__ __launch_bounds__(256) void cdist_batched_kernel(
  * string buffers (a global sort would scatter every tile over gigabytes and
  * thrash the TLB). Per chunk c the order is: long pairs, then buckets by
  * descending key. Arrays: hist / cursor [nchunks * PKEYS], chunk_base and
- * tile_base [nchunks + 1]. */
+ * tile_base [nchunks + 1]. Keys come from the offsets alone (pair_key). */
+__device__ __forceinline__ int pair_k(int k_all, const int32_t *kvec, int64_t i)
+{
+    return kvec ? clamp_k_dev(kvec[i]) : k_all;
+}
+
 __global__ __launch_bounds__(256) void pairs_classify_kernel(
-    const u8 *__restrict__ ba, const int64_t *__restrict__ oa, const u8 *__restrict__ bb,
-    const int64_t *__restrict__ ob, int64_t n, void *out, int dtype, int trim, int k_all,
-    const int32_t *__restrict__ kvec, int bin_width, int64_t chunk, PDesc *desc,
-    unsigned short *key, unsigned long long *hist)
+    const int64_t *__restrict__ oa, const int64_t *__restrict__ ob, int64_t n, void *out, int dtype,
+    int k_all, const int32_t *__restrict__ kvec, int bin_width, int64_t chunk, unsigned short *key,
+    unsigned long long *hist)
 {
     __shared__ unsigned int lh[PKEYS];
     for (int x = threadIdx.x; x < PKEYS; x += blockDim.x) lh[x] = 0;
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
-        const int k = kvec ? clamp_k_dev(kvec[i]) : k_all;
-        PDesc d;
         int result = 0;
-        const int kk = classify_pair(ba, oa, bb, ob, i, trim, k, bin_width, d, result);
+        const int kk = pair_key((int)(oa[i + 1] - oa[i]), (int)(ob[i + 1] - ob[i]),
+                                pair_k(k_all, kvec, i), bin_width, result);
         if (kk == 0) store_out(out, i, result, dtype);
-        else desc[i] = d;
         key[i] = (unsigned short)kk;
         if (kk) atomicAdd(&lh[kk], 1u);
     }
@@ -491,69 +493,92 @@ __global__ __launch_bounds__(1024) void prefix_kernel(const unsigned long long *
     }
 }
 
+/* Pair indices into bucket order. A block ranks its pairs per key in LDS and
+ * reserves each key's range of the chunk with one global atomic, so pairs that
+ * share a bucket (all of them, in a uniform batch) do not contend on one
+ * global counter. The order inside a bucket is immaterial. */
 __global__ __launch_bounds__(256) void pairs_scatter_kernel(
     const unsigned short *key, int64_t n, int64_t chunk, unsigned long long *cursor,
     const unsigned long long *chunk_base, int32_t *sorted)
 {
+    __shared__ unsigned int cnt[PKEYS];
+    __shared__ unsigned long long base[PKEYS];
+    for (int x = threadIdx.x; x < PKEYS; x += blockDim.x) cnt[x] = 0;
+    __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int kk = key[i];
-    if (!kk) return;
-    const int64_t c = i / chunk;
-    sorted[chunk_base[c] + atomicAdd(&cursor[c * PKEYS + kk], 1ull)] = (int32_t)i;
+    const int kk = i < n ? key[i] : 0;
+    const unsigned int rank = kk ? atomicAdd(&cnt[kk], 1u) : 0u;
+    __syncthreads();
+    /* a block never straddles chunks (chunk is a multiple of the block size) */
+    const int64_t c = ((int64_t)blockIdx.x * blockDim.x) / chunk;
+    for (int x = threadIdx.x; x < PKEYS; x += blockDim.x)
+        if (cnt[x]) base[x] = atomicAdd(&cursor[c * PKEYS + x], (unsigned long long)cnt[x]);
+    __syncthreads();
+    if (kk) sorted[chunk_base[c] + base[kk] + rank] = (int32_t)i;
 }
 
-template <int NW>
-__device__ void pairs_tile(const u8 *__restrict__ ba, const u8 *__restrict__ bb, const PDesc *desc,
-                           const int32_t *sorted, int64_t s0, int64_t s1, void *out, int dtype)
+/* One wave on one long pair (shorter string above 256 bytes before trimming):
+ * trim, swap and the gap / empty rules (long_prepare), then the anti-diagonal
+ * wavefront. The pair index is the same in every lane. */
+__device__ void pairs_long_item(const u8 *__restrict__ ba, const int64_t *__restrict__ oa,
+                                const u8 *__restrict__ bb, const int64_t *__restrict__ ob,
+                                int64_t i, int trim, int k, void *out, int dtype, int8_t *hbuf,
+                                int64_t slot)
 {
-    const int64_t s = s0 + lane_id();
-    if (s >= s1) return;
-    const int64_t i = sorted[s];
-    store_out(out, i, desc_distance<NW>(ba, bb, desc[i]), dtype);
-}
-
-__device__ void pairs_long_item(const u8 *__restrict__ ba, const u8 *__restrict__ bb,
-                                const PDesc &d, int64_t i, void *out, int dtype,
-                                int8_t *hbuf, int64_t slot)
-{
-    const int k = d.k;
     const int lane = lane_id();
-    const bool sw = uni(d.la) < 0;
-    const int m = uni(d.la & 0x7fffffff), n = uni(d.lb);
-    LongCtx C;
-    C.p = (sw ? bb : ba) + uni64(d.a);
-    C.m = m;
-    C.t = (sw ? ba : bb) + uni64(d.b);
-    C.n = n;
-    C.W = (m + 63) >> 6;
-    C.hbuf = hbuf;
-    const int S = (C.W + 63) >> 6;
-    int r;
-    if (S > 1 && (hbuf == nullptr || n > slot)) {
-        r = -1;                             /* scratch too small: caller error */
-    } else {
-        LongLane L;
-        for (int s = 0; s < S; s++) {
-            C.s = s;
-            C.nl = C.W - 64 * s < 64 ? C.W - 64 * s : 64;
-            long_lane_init(C, L, lane);
-            u32 prev = 0;
-            for (int step = 0; step < n + C.nl - 1; step++) {
-                u32 in = (u32)__shfl_up((int)prev, 1);
-                if (lane == 0) in = long_source(C, step);
-                prev = long_lane_step(C, L, lane, step, in);
+    const int64_t a0 = oa[i], b0 = ob[i];
+    const u8 *p = ba + a0, *t = bb + b0;
+    int m = (int)(oa[i + 1] - a0), n = (int)(ob[i + 1] - b0);
+    int r = long_prepare(p, m, t, n, 1, trim, k);
+    if (r < 0) {
+        m = uni(m);
+        n = uni(n);
+        LongCtx C;
+        C.p = p;
+        C.m = m;
+        C.t = t;
+        C.n = n;
+        C.W = (m + 63) >> 6;
+        C.hbuf = hbuf;
+        const int S = (C.W + 63) >> 6;
+        if (S > 1 && (hbuf == nullptr || n > slot)) {
+            r = -1;                             /* scratch too small: caller error */
+        } else {
+            LongLane L;
+            for (int s = 0; s < S; s++) {
+                C.s = s;
+                C.nl = C.W - 64 * s < 64 ? C.W - 64 * s : 64;
+                long_lane_init(C, L, lane);
+                u32 prev = 0;
+                for (int step = 0; step < n + C.nl - 1; step++) {
+                    u32 in = (u32)__shfl_up((int)prev, 1);
+                    if (lane == 0) in = long_source(C, step);
+                    prev = long_lane_step(C, L, lane, step, in);
+                }
+                __threadfence();
             }
-            __threadfence();
+            const int dd = m + __shfl(L.delta, (C.W - 1) - 64 * (S - 1));
+            r = (k >= 0 && dd > k) ? k + 1 : dd;
         }
-        const int dd = m + __shfl(L.delta, (C.W - 1) - 64 * (S - 1));
-        r = (k >= 0 && dd > k) ? k + 1 : dd;
     }
     if (lane == 0) store_out(out, i, r, dtype);
 }
 
+template <int NW>
+__device__ void pairs_tile(const u8 *__restrict__ ba, const int64_t *__restrict__ oa,
+                           const u8 *__restrict__ bb, const int64_t *__restrict__ ob,
+                           const int32_t *sorted, int64_t s0, int64_t s1, int trim, int k_all,
+                           const int32_t *__restrict__ kvec, void *out, int dtype)
+{
+    const int64_t s = s0 + lane_id();
+    if (s >= s1) return;
+    const int64_t i = sorted[s];
+    store_out(out, i, packed_pair<NW>(ba, oa, bb, ob, i, trim, pair_k(k_all, kvec, i)), dtype);
+}
+
 __global__ __launch_bounds__(256) void pairs_batched_kernel(
-    const u8 *__restrict__ ba, const u8 *__restrict__ bb, const PDesc *__restrict__ desc,
+    const u8 *__restrict__ ba, const int64_t *__restrict__ oa, const u8 *__restrict__ bb,
+    const int64_t *__restrict__ ob, int trim, int k_all, const int32_t *__restrict__ kvec,
     const int32_t *__restrict__ sorted, const unsigned short *__restrict__ key_of,
     const unsigned long long *__restrict__ hist, const unsigned long long *__restrict__ chunk_base,
     const unsigned long long *__restrict__ tile_base, int64_t nchunks, void *out, int dtype,
@@ -581,8 +606,8 @@ __global__ __launch_bounds__(256) void pairs_batched_kernel(
         const int64_t cb = (int64_t)chunk_base[c];
         const int64_t nlong = (int64_t)hist[c * PKEYS + PKEY_LONG];
         if (lt < nlong) {                      /* long pairs lead their chunk, one per tile */
-            const int64_t i = sorted[cb + lt];
-            pairs_long_item(ba, bb, desc[i], i, out, dtype, hbuf, slot);
+            const int64_t i = uni64(sorted[cb + lt]);
+            pairs_long_item(ba, oa, bb, ob, i, trim, pair_k(k_all, kvec, i), out, dtype, hbuf, slot);
             continue;
         }
         const int64_t s0 = cb + nlong + (lt - nlong) * 64;
@@ -590,10 +615,52 @@ __global__ __launch_bounds__(256) void pairs_batched_kernel(
         const int64_t s1 = s0 + 64 < ce ? s0 + 64 : ce;
         /* descending keys: the first pair of a tile has its largest word count */
         const int w = uni(key_words(key_of[sorted[s0]]));
-        if (w <= 1) pairs_tile<1>(ba, bb, desc, sorted, s0, s1, out, dtype);
-        else if (w == 2) pairs_tile<2>(ba, bb, desc, sorted, s0, s1, out, dtype);
-        else if (w == 3) pairs_tile<3>(ba, bb, desc, sorted, s0, s1, out, dtype);
-        else pairs_tile<4>(ba, bb, desc, sorted, s0, s1, out, dtype);
+        if (w <= 1) pairs_tile<1>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype);
+        else if (w == 2) pairs_tile<2>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype);
+        else if (w == 3) pairs_tile<3>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype);
+        else pairs_tile<4>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype);
+    }
+}
+
+/* Direct mode: one launch, no bucketing. Waves take 64 consecutive pairs; each
+ * lane runs its own pair through the lane code of its word count (up to
+ * NWMAX words), and the wave then runs the long ones (shorter string above
+ * 256 bytes, NWMAX = 4 only) one after another. Best for batches too small, or
+ * too uniform, for bucketing to pay. */
+template <int NWMAX>
+__global__ __launch_bounds__(256) void pairs_direct_kernel(
+    const u8 *__restrict__ ba, const int64_t *__restrict__ oa, const u8 *__restrict__ bb,
+    const int64_t *__restrict__ ob, int64_t n, int trim, int k_all,
+    const int32_t *__restrict__ kvec, void *out, int dtype, int8_t *scratch, int64_t slot)
+{
+    const int lane = lane_id();
+    const int64_t wave_gid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    int8_t *hbuf = scratch ? scratch + wave_gid * slot : nullptr;
+    for (int64_t b0 = wave_gid * 64; b0 < n; b0 += nwaves * 64) {
+        const int64_t i = b0 + lane;
+        bool is_long = false;
+        if (i < n) {
+            const int la = (int)(oa[i + 1] - oa[i]), lb = (int)(ob[i + 1] - ob[i]);
+            const int mn = la < lb ? la : lb;
+            const int k = pair_k(k_all, kvec, i);
+            if (mn > 64 * NWMAX) {
+                is_long = true;                  /* only when NWMAX = 4: above 256 bytes */
+            } else {
+                int d;
+                if (NWMAX == 1 || mn <= 64) d = packed_pair<1>(ba, oa, bb, ob, i, trim, k);
+                else if (NWMAX == 2 || mn <= 128) d = packed_pair<NWMAX >= 2 ? 2 : 1>(ba, oa, bb, ob, i, trim, k);
+                else d = packed_pair<NWMAX>(ba, oa, bb, ob, i, trim, k);
+                store_out(out, i, d, dtype);
+            }
+        }
+        u64 lm = wave_ballot(is_long);
+        while (lm) {
+            const int l = __builtin_ctzll(lm);
+            lm &= lm - 1;
+            const int64_t il = b0 + l;
+            pairs_long_item(ba, oa, bb, ob, il, trim, pair_k(k_all, kvec, il), out, dtype, hbuf, slot);
+        }
     }
 }
 
@@ -720,7 +787,7 @@ static int64_t pairs_chunk(int64_t n, const bl_pairs_params *prm)
 }
 
 struct PairsLayout {
-    size_t head, desc, key, sorted, scratch, total;
+    size_t head, key, sorted, scratch, total;
     int64_t chunk, nchunks;
 };
 
@@ -737,8 +804,7 @@ static PairsLayout pairs_layout(int64_t n, int64_t max_len, const bl_pairs_param
     /* head: counter | hist | cursor | chunk_cnt | tile_cnt | chunk_base | tile_base */
     const size_t nk = (size_t)hc * PKEYS;
     L.head = (64 + (2 * nk + 4 * ((size_t)hc + 1)) * 8 + 255) & ~(size_t)255;
-    L.desc = L.head;
-    L.key = (L.desc + (size_t)n * sizeof(PDesc) + 255) & ~(size_t)255;
+    L.key = L.head;
     L.sorted = (L.key + (size_t)n * sizeof(unsigned short) + 255) & ~(size_t)255;
     L.scratch = (L.sorted + (size_t)n * sizeof(int32_t) + 255) & ~(size_t)255;
     L.total = L.scratch + (max_len > 4096 ? (size_t)max_len * 2048 : 0) + 256;
@@ -778,6 +844,7 @@ int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t
     const int bw = prm && prm->bin_width > 0 ? prm->bin_width : 16;
     const int wpc = prm && prm->waves_per_cu > 0 ? prm->waves_per_cu : 16;
     const int wg = prm && (prm->block == 64 || prm->block == 128 || prm->block == 256) ? prm->block : 256;
+    const bool direct = prm && prm->reserved[1] == 1;
 
     char *w = (char *)d_work;
     if (!w) BL_HIP(hipMallocAsync((void **)&w, L.total, s));
@@ -789,31 +856,46 @@ int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t
     unsigned long long *tile_cnt = chunk_cnt + (L.nchunks + 1);
     unsigned long long *chunk_base = tile_cnt + (L.nchunks + 1);
     unsigned long long *tile_base = chunk_base + (L.nchunks + 1);
-    PDesc *desc = (PDesc *)(w + L.desc);
     unsigned short *key = (unsigned short *)(w + L.key);
     int32_t *sorted = (int32_t *)(w + L.sorted);
     int8_t *scratch = max_len > 4096 ? (int8_t *)(w + L.scratch) : nullptr;
+    const int64_t slot = scratch ? max_len : 0;
 
-    /* the counter and the histograms start at zero */
-    int rc = launch_clear(ctr, (int)(8 + nk), s);
-    const unsigned cblocks = (unsigned)((n + 255) / 256);
-    if (rc == BL_OK) {
-        pairs_classify_kernel<<<cblocks, 256, 0, s>>>(d_buf_a, d_off_a, d_buf_b, d_off_b, n, d_out,
-                                                      dtype, trim, k, d_kvec, bw, L.chunk, desc,
-                                                      key, hist);
-        pairs_chunk_scan_kernel<<<(unsigned)L.nchunks, 256, 0, s>>>(hist, cursor, chunk_cnt, tile_cnt);
-        prefix_kernel<<<1, 1024, 0, s>>>(chunk_cnt, chunk_base, tile_cnt, tile_base, L.nchunks);
-        pairs_scatter_kernel<<<cblocks, 256, 0, s>>>(key, n, L.chunk, cursor, chunk_base, sorted);
-        int64_t blocks = (int64_t)device_cus() * wpc * 64 / wg;
-        const int64_t maxb = (n + 63) / 64 * 64 / wg + 1;
-        if (blocks > maxb) blocks = maxb;
-        if (scratch && blocks * (wg / 64) > 2048) blocks = 2048 / (wg / 64);
-        pairs_batched_kernel<<<(unsigned)blocks, wg, 0, s>>>(d_buf_a, d_buf_b, desc, sorted, key,
-                                                             hist, chunk_base, tile_base,
-                                                             L.nchunks, d_out, dtype, ctr, scratch,
-                                                             scratch ? max_len : 0);
+    int rc = BL_OK;
+    int64_t blocks = (int64_t)device_cus() * wpc * 64 / wg;
+    const int64_t maxb = (n + 63) / 64 * 64 / wg + 1;
+    if (blocks > maxb) blocks = maxb;
+    if (scratch && blocks * (wg / 64) > 2048) blocks = 2048 / (wg / 64);
+    if (direct) {
+        /* the widest lane code the strings can need (shorter string <= max_len) */
+        if (max_len <= 64)
+            pairs_direct_kernel<1><<<(unsigned)blocks, wg, 0, s>>>(
+                d_buf_a, d_off_a, d_buf_b, d_off_b, n, trim, k, d_kvec, d_out, dtype, scratch, slot);
+        else if (max_len <= 128)
+            pairs_direct_kernel<2><<<(unsigned)blocks, wg, 0, s>>>(
+                d_buf_a, d_off_a, d_buf_b, d_off_b, n, trim, k, d_kvec, d_out, dtype, scratch, slot);
+        else
+            pairs_direct_kernel<4><<<(unsigned)blocks, wg, 0, s>>>(
+                d_buf_a, d_off_a, d_buf_b, d_off_b, n, trim, k, d_kvec, d_out, dtype, scratch, slot);
         hipError_t e = hipGetLastError();
-        if (e != hipSuccess) rc = fail(BL_EHIP, "pairs pipeline launch: %s", hipGetErrorString(e));
+        if (e != hipSuccess) rc = fail(BL_EHIP, "pairs_direct_kernel launch: %s", hipGetErrorString(e));
+    } else {
+        /* the counter and the histograms start at zero */
+        rc = launch_clear(ctr, (int)(8 + nk), s);
+        const unsigned cblocks = (unsigned)((n + 255) / 256);
+        if (rc == BL_OK) {
+            pairs_classify_kernel<<<cblocks, 256, 0, s>>>(d_off_a, d_off_b, n, d_out, dtype, k, d_kvec,
+                                                          bw, L.chunk, key, hist);
+            pairs_chunk_scan_kernel<<<(unsigned)L.nchunks, 256, 0, s>>>(hist, cursor, chunk_cnt,
+                                                                        tile_cnt);
+            prefix_kernel<<<1, 1024, 0, s>>>(chunk_cnt, chunk_base, tile_cnt, tile_base, L.nchunks);
+            pairs_scatter_kernel<<<cblocks, 256, 0, s>>>(key, n, L.chunk, cursor, chunk_base, sorted);
+            pairs_batched_kernel<<<(unsigned)blocks, wg, 0, s>>>(
+                d_buf_a, d_off_a, d_buf_b, d_off_b, trim, k, d_kvec, sorted, key, hist, chunk_base,
+                tile_base, L.nchunks, d_out, dtype, ctr, scratch, slot);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) rc = fail(BL_EHIP, "pairs pipeline launch: %s", hipGetErrorString(e));
+        }
     }
     if (!d_work) (void)hipFreeAsync(w, s);
     return rc;

@@ -668,62 +668,42 @@ BL_HD int long_prepare(const u8 *&p, int &m, const u8 *&t, int &n, int is_pair, 
 }
 
 /* ------------------------------------------------------------ batched pairs */
-/* Bucket keys of the batched pairs pipeline: key = 1 + (w - 1) * LB_BINS +
- * min(lb / bin_width, LB_BINS - 1) for a pattern of w <= 4 words, PKEY_LONG for
- * longer patterns, 0 for a pair answered during classification. A larger key
- * means more work; the pipeline schedules large keys first. */
+/* Bucket keys of the batched pairs pipeline, from the untrimmed lengths (so
+ * classifying reads no string bytes; trimming only shortens both strings by
+ * the same amount, so the pattern words of the untrimmed shorter string bound
+ * the trimmed ones and the length gap is unchanged):
+ * key = 1 + (w - 1) * LB_BINS + min(max(la, lb) / bin_width, LB_BINS - 1) for a
+ * shorter string of w <= 4 words, PKEY_LONG for longer ones, 0 for a pair the
+ * gap or an empty string decides. A larger key means more work; the pipeline
+ * schedules large keys first. */
 constexpr int LB_BINS = 64;
 constexpr int PKEY_LONG = 1 + 4 * LB_BINS;
 constexpr int PKEYS = PKEY_LONG + 1;
 
-struct PDesc {
-    int64_t a, b;   /* pattern / text start in their buffers (after trim and swap) */
-    int32_t la, lb; /* bit 31 of la: swapped, the pattern lives in buf_b */
-    int32_t k, pad;
-};
-
-/* Classify pair i: trim, swap, decide the trivial cases. Returns the bucket
- * key; for key 0, *result holds the answer. */
-BL_HD int classify_pair(const u8 *ba, const int64_t *oa, const u8 *bb, const int64_t *ob,
-                        int64_t i, int trim, int k, int bin_width, PDesc &d, int &result)
+/* Key of a pair of lengths la, lb; for key 0, *result holds the answer
+ * (the gap rule, csrc/myers.c:137, and the empty rule, csrc/myers.c:90-99). */
+BL_HD int pair_key(int la, int lb, int k, int bin_width, int &result)
 {
-    const u8 *a = ba + oa[i], *b = bb + ob[i];
-    int la = (int)(oa[i + 1] - oa[i]), lb = (int)(ob[i + 1] - ob[i]);
-    if (trim) trim_pair(a, la, b, lb);
-    bool swapped = false;
-    if (la > lb) {
-        const u8 *t = a; a = b; b = t;
-        const int x = la; la = lb; lb = x;
-        swapped = true;
-    }
-    if (k >= 0 && lb - la > k) { result = k + 1; return 0; }
-    if (la == 0) { result = lb; return 0; }
-    const int w = (la + 63) >> 6;
-    int key;
-    if (w > 4) key = PKEY_LONG;
-    else {
-        const int bin = lb / bin_width;
-        key = 1 + (w - 1) * LB_BINS + (bin < LB_BINS - 1 ? bin : LB_BINS - 1);
-    }
-    d.a = swapped ? (int64_t)(a - bb) : (int64_t)(a - ba);
-    d.b = swapped ? (int64_t)(b - ba) : (int64_t)(b - bb);
-    d.la = la | (swapped ? (int32_t)0x80000000 : 0);
-    d.lb = lb;
-    d.k = k;
-    d.pad = 0;
-    return key;
+    const int mn = la < lb ? la : lb, mx = la < lb ? lb : la;
+    if (k >= 0 && mx - mn > k) { result = k + 1; return 0; }
+    if (mn == 0) { result = mx; return 0; }
+    const int w = (mn + 63) >> 6;
+    if (w > 4) return PKEY_LONG;
+    const int bin = mx / bin_width;
+    return 1 + (w - 1) * LB_BINS + (bin < LB_BINS - 1 ? bin : LB_BINS - 1);
 }
 
 /* Pattern words of the tile whose largest key is `key`. */
 BL_HD int key_words(int key) { return (key - 1) / LB_BINS + 1; }
 
-/* One classified pair through the register-resident lane code. */
+/* Pair i of packed sets through the register-resident lane code (shorter
+ * string at most 64*NW bytes before trimming). */
 template <int NW>
-BL_HD int desc_distance(const u8 *ba, const u8 *bb, const PDesc &d)
+BL_HD int packed_pair(const u8 *ba, const int64_t *oa, const u8 *bb, const int64_t *ob, int64_t i,
+                      int trim, int k)
 {
-    const bool sw = d.la < 0;
-    const int la = d.la & 0x7fffffff;
-    return pair_lane<NW>((sw ? bb : ba) + d.a, la, (sw ? ba : bb) + d.b, d.lb, 0, d.k);
+    const int64_t a0 = oa[i], b0 = ob[i];
+    return pair_lane<NW>(ba + a0, (int)(oa[i + 1] - a0), bb + b0, (int)(ob[i + 1] - b0), trim, k);
 }
 
 }  // namespace bl

@@ -147,25 +147,30 @@ int blh_cross(const u8 *bq, const int64_t *oq, int64_t nq, const u8 *bc, const i
 }
 
 /* The batched pairs pipeline emulated step by step with the kernels' shared
- * code and the same index arithmetic: classify -> per-chunk bucket scan
- * (descending keys) -> chunk / tile prefix sums -> scatter -> tiles taken in
- * order, the chunk of each tile found by the same bisection. Every index is
- * checked, and so is the tile invariant (no pair in a tile needs more words
- * than its first pair). chunk: pairs per chunk (a multiple of 256 on the GPU;
- * any value here). Returns 0, or a negative code naming the violated check. */
+ * code and the same index arithmetic: keys from the offsets -> per-chunk
+ * bucket scan (descending keys) -> chunk / tile prefix sums -> scatter, each
+ * 256-pair block ranking its pairs per key and reserving the key's range with
+ * one add -> tiles taken in order, the chunk of each tile found by the same
+ * bisection. Every index is checked, and so is the tile invariant (no pair in
+ * a tile has a shorter string of more words than its first pair). chunk:
+ * pairs per chunk (a multiple of 256 on the GPU; any multiple of 256 here).
+ * Returns 0, or a negative code naming the violated check. */
 int blh_pairs_batched(const u8 *ba, const int64_t *oa, const u8 *bb, const int64_t *ob, int64_t n,
                       int32_t *out, int trim, int k, int bin_width, int64_t chunk)
 {
-    if (chunk < 1) return -9;
+    if (chunk < 256 || chunk % 256) return -9;
     const int64_t nchunks = n ? (n + chunk - 1) / chunk : 1;
-    std::vector<PDesc> desc((size_t)n);
     std::vector<int> key((size_t)n);
     std::vector<uint64_t> hist((size_t)(nchunks * PKEYS), 0), cursor(hist.size(), 0);
     std::vector<uint64_t> ccnt((size_t)nchunks), tcnt((size_t)nchunks);
     std::vector<uint64_t> cbase((size_t)nchunks + 1), tbase((size_t)nchunks + 1);
+    auto minlen = [&](int64_t i) {
+        const int64_t la = oa[i + 1] - oa[i], lb = ob[i + 1] - ob[i];
+        return (int)(la < lb ? la : lb);
+    };
     for (int64_t i = 0; i < n; i++) {
         int result = 0;
-        const int kk = classify_pair(ba, oa, bb, ob, i, trim, k, bin_width, desc[(size_t)i], result);
+        const int kk = pair_key((int)(oa[i + 1] - oa[i]), (int)(ob[i + 1] - ob[i]), k, bin_width, result);
         if (kk < 0 || kk >= PKEYS) return -10;
         if (kk == 0) out[i] = result;
         key[(size_t)i] = kk;
@@ -188,12 +193,25 @@ int blh_pairs_batched(const u8 *ba, const int64_t *oa, const u8 *bb, const int64
     }
     const int64_t total = (int64_t)cbase[(size_t)nchunks];
     std::vector<int32_t> sorted((size_t)total, -1);
-    for (int64_t i = 0; i < n; i++) {
-        if (!key[(size_t)i]) continue;
-        const int64_t c = i / chunk;
-        const uint64_t pos = cbase[(size_t)c] + cursor[(size_t)(c * PKEYS + key[(size_t)i])]++;
-        if (pos >= cbase[(size_t)c + 1]) return -11;
-        sorted[(size_t)pos] = (int32_t)i;
+    for (int64_t b0 = 0; b0 < n; b0 += 256) {             /* one scatter block */
+        const int64_t b1 = b0 + 256 < n ? b0 + 256 : n;
+        const int64_t c = b0 / chunk;
+        std::vector<uint32_t> cnt(PKEYS, 0), rank((size_t)(b1 - b0), 0);
+        for (int64_t i = b0; i < b1; i++)
+            if (key[(size_t)i]) rank[(size_t)(i - b0)] = cnt[(size_t)key[(size_t)i]]++;
+        std::vector<uint64_t> base(PKEYS, 0);
+        for (int x = 1; x < PKEYS; x++) {
+            base[(size_t)x] = cursor[(size_t)(c * PKEYS + x)];
+            cursor[(size_t)(c * PKEYS + x)] += cnt[(size_t)x];
+        }
+        for (int64_t i = b0; i < b1; i++) {
+            const int kk = key[(size_t)i];
+            if (!kk) continue;
+            const uint64_t pos = cbase[(size_t)c] + base[(size_t)kk] + rank[(size_t)(i - b0)];
+            if (pos >= cbase[(size_t)c + 1]) return -11;
+            if (sorted[(size_t)pos] != -1) return -18;
+            sorted[(size_t)pos] = (int32_t)i;
+        }
     }
     for (int64_t q = 0; q < total; q++)
         if (sorted[(size_t)q] < 0 || sorted[(size_t)q] >= n) return -12;
@@ -216,10 +234,8 @@ int blh_pairs_batched(const u8 *ba, const int64_t *oa, const u8 *bb, const int64
         if (lt < nlong) {
             const int64_t i = sorted[(size_t)(cb + lt)];
             if (key[(size_t)i] != PKEY_LONG) return -13;
-            const PDesc &d = desc[(size_t)i];
-            const bool sw = d.la < 0;
-            out[i] = long_sim((sw ? bb : ba) + d.a, d.la & 0x7fffffff, (sw ? ba : bb) + d.b, d.lb, 1,
-                              0, d.k);
+            out[i] = long_sim(ba + oa[i], (int)(oa[i + 1] - oa[i]), bb + ob[i],
+                              (int)(ob[i + 1] - ob[i]), 1, trim, k);
             continue;
         }
         const int64_t s0 = cb + nlong + (lt - nlong) * 64;
@@ -230,13 +246,12 @@ int blh_pairs_batched(const u8 *ba, const int64_t *oa, const u8 *bb, const int64
         if (w < 1 || w > 4) return -15;
         for (int64_t q = s0; q < s1; q++) {
             const int64_t i = sorted[(size_t)q];
-            const PDesc &d = desc[(size_t)i];
-            if ((((d.la & 0x7fffffff) + 63) >> 6) > w) return -16;
+            if (((minlen(i) + 63) >> 6) > w) return -16;
             switch (w) {
-            case 1: out[i] = desc_distance<1>(ba, bb, d); break;
-            case 2: out[i] = desc_distance<2>(ba, bb, d); break;
-            case 3: out[i] = desc_distance<3>(ba, bb, d); break;
-            default: out[i] = desc_distance<4>(ba, bb, d); break;
+            case 1: out[i] = packed_pair<1>(ba, oa, bb, ob, i, trim, k); break;
+            case 2: out[i] = packed_pair<2>(ba, oa, bb, ob, i, trim, k); break;
+            case 3: out[i] = packed_pair<3>(ba, oa, bb, ob, i, trim, k); break;
+            default: out[i] = packed_pair<4>(ba, oa, bb, ob, i, trim, k); break;
             }
         }
     }

@@ -147,7 +147,8 @@ typedef struct {
     int waves_per_cu;      /* persistent waves per compute unit (default 16) */
     int block;             /* workgroup size: 64, 128 or 256 (default 256) */
     int bin_width;         /* text-length bucket width in bytes (default 16) */
-    int reserved[4];       /* [0]: pairs per bucketing chunk (default 65536) */
+    int reserved[4];       /* [0]: pairs per bucketing chunk (default 65536);
+                              [1]: 1 = direct mode (one launch, no bucketing) */
 } bl_pairs_params;
 
 /* Bytes of device workspace bl_cdist needs for nq queries of at most

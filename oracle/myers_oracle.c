@@ -136,10 +136,31 @@ static int orc_diag_bound(const orc_peq *q, int m, int n, int jj)
     return s;
 }
 
-/* Which columns *cols counts: the reference's bottom-row exit test, or the
- * final-diagonal bound (same results, earlier exits). */
+/* What *cols counts: 0 = the columns the reference's bottom-row exit test
+ * evaluates; 1 = the columns the final-diagonal bound needs (same results,
+ * earlier exits); 2 = the DP cells the row-staged scan needs under the
+ * final-diagonal bound (staged_cells below). */
 static int g_count_rule = 0;
 void orc_set_count_rule(int rule) { g_count_rule = rule; }
+
+/* Cells of a scan in row stages (32 rows, then 64, then all m rows; each
+ * stage restarts at column 0 and can test the bound at column j only while
+ * its diagonal row j + m - n lies within the stage) whose bound first exceeds
+ * k after column s (s = n: never). k < 0: one whole scan of m x n cells. */
+static long long staged_cells(int m, int n, int k, int s)
+{
+    if (k < 0) return (long long)m * n;
+    long long cells = 0;
+    static const int R[2] = {32, 64};
+    for (int t = 0; t < 2; t++) {
+        if (R[t] >= m) break;
+        const int J = R[t] - (m - n);          /* last column this stage can test */
+        if (J < 1) continue;
+        if (s <= J) return cells + (long long)s * R[t];
+        cells += (long long)J * R[t];
+    }
+    return cells + (long long)s * m;
+}
 
 static int orc_scan(orc_peq *q, int m, const uint8_t *t, int n, int k, long long *cols)
 {
@@ -156,7 +177,7 @@ static int orc_scan(orc_peq *q, int m, const uint8_t *t, int n, int k, long long
         score += orc_step(&q->pv[W - 1], &q->mv[W - 1], row[W - 1], h, last);
         if (cols && *cols >= 0) {
             *cols = j + 1;
-            if (g_count_rule == 1 && k >= 0 && orc_diag_bound(q, m, n, j + 1) > k)
+            if (g_count_rule >= 1 && k >= 0 && orc_diag_bound(q, m, n, j + 1) > k)
                 *cols = -(j + 1);               /* diagonal rule fired here; keep scanning */
         }
 #if ORC_MUTANT == 4
@@ -164,11 +185,13 @@ static int orc_scan(orc_peq *q, int m, const uint8_t *t, int n, int k, long long
 #else
         if (k >= 0 && score - (n - 1 - j) > k) {
             if (cols && *cols < 0) *cols = -*cols;
+            if (cols && g_count_rule == 2) *cols = staged_cells(m, n, k, (int)*cols);
             return k + 1;
         }
 #endif
     }
     if (cols && *cols < 0) *cols = -*cols;
+    if (cols && g_count_rule == 2) *cols = staged_cells(m, n, k, (int)*cols);
     return score;
 }
 

@@ -105,9 +105,11 @@ def pairs(pa, pb, k: int = -1, trim: bool = True, threads: int = 1, cols: bool =
 
 
 def count_rule(rule: str):
-    """Column counting for cols=True: "reference" (bottom-row exit test,
-    csrc/myers.c:117, :127) or "diagonal" (final-diagonal bound)."""
-    lib().orc_set_count_rule({"reference": 0, "diagonal": 1}[rule])
+    """What cols=True counts per pair: "reference" (columns of the bottom-row
+    exit test, csrc/myers.c:117, :127), "diagonal" (columns the final-diagonal
+    bound needs) or "staged" (DP cells of the row-staged scan under the
+    final-diagonal bound: 32 rows, then 64, then all rows)."""
+    lib().orc_set_count_rule({"reference": 0, "diagonal": 1, "staged": 2}[rule])
 
 
 def cross(pq, pc, k: int = -1, threads: int = 1, cols: bool = False):

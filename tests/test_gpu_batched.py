@@ -39,7 +39,8 @@ def test_every_pairs_launch_shape_gives_the_same_results():
     want = {k: O.pairs(pa, pb, k=k, threads=8) for k in (-1, 5)}
     space = {"waves_per_cu": [4, 32], "block": [64, 256], "bin_width": [1, 64],
              "chunk": [256, 1 << 20]}
-    for p in autotune._grid(space):
+    direct = {"mode": ["direct"], "waves_per_cu": [1, 32], "block": [64, 256]}
+    for p in list(autotune._grid(space)) + list(autotune._grid(direct)):
         for k in (-1, 5):
             out = hip.DeviceArray.empty((A.n,), np.int32)
             hip.pairs_batched_into(A, B, out, k, True, None, None, p)
@@ -93,10 +94,13 @@ def test_per_item_thresholds():
     A, B = hip.Packed(*pa), hip.Packed(*pb)
     rng = np.random.default_rng(4)
     kv = rng.choice([-1, 0, 1, 3, 10], size=A.n).astype(np.int32)
-    out = hip.DeviceArray.empty((A.n,), np.int32)
-    hip.pairs_batched_into(A, B, out, -1, True, None, kvec=hip.DeviceBuffer.from_host(kv))
     exact = O.pairs(pa, pb, threads=8)
-    np.testing.assert_array_equal(out.get(), np.where(kv < 0, exact, np.minimum(exact, kv + 1)))
+    for p in (None, {"mode": "direct"}):
+        out = hip.DeviceArray.empty((A.n,), np.int32)
+        hip.pairs_batched_into(A, B, out, -1, True, None, params=p,
+                               kvec=hip.DeviceBuffer.from_host(kv))
+        np.testing.assert_array_equal(out.get(), np.where(kv < 0, exact, np.minimum(exact, kv + 1)),
+                                      err_msg=str(p))
     pq, pc = W.cdist(2, 16, 4000, seed=5)
     Q, C = hip.Packed(*pq), hip.Packed(*pc)
     kq = rng.choice([-1, 0, 2, 4], size=Q.n).astype(np.int32)

@@ -83,11 +83,12 @@ def test_peq_lane_code_matches_golden(hc, golden, force_long):
                                           err_msg=f"{name} k={k}")
 
 
-@pytest.mark.parametrize("bin_width,chunk", [(1, 7), (16, 64), (16, 256), (64, 1 << 20)])
+@pytest.mark.parametrize("bin_width,chunk", [(1, 256), (16, 512), (16, 1024), (64, 1 << 20)])
 def test_batched_pairs_pipeline_emulated(hc, golden, bin_width, chunk):
-    """classify -> per-chunk bucket scan -> scatter -> tiles, as the persistent
-    kernel walks them, with every index and the tile word-count invariant
-    checked (small chunks: many chunks, some without long pairs or empty)."""
+    """keys from the offsets -> per-chunk bucket scan -> block-ranked scatter ->
+    tiles, as the persistent kernel walks them, with every index and the tile
+    word-count invariant checked (small chunks: many chunks, some without long
+    pairs or empty)."""
     for name in golden_pair_sets(golden):
         (ba, oa), (bb, ob) = pairs_of(golden, name)
         n = len(oa) - 1

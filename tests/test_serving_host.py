@@ -35,6 +35,9 @@ class FakeExec:
     def pairs_workspace(self, n, max_len):
         return 1024 + 40 * n
 
+    def cdist_workspace(self, nq, max_qlen):
+        return 256 + 2048 * nq
+
     def close(self, slots):
         pass
 
