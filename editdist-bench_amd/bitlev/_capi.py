@@ -55,6 +55,8 @@ SIGNATURES = {
     "bl_cdist_workspace_size": [_i64, _int],
     "bl_cdist": [_vp, _vp, _i64, _vp, _int, _vp, _int, _int, _vp, _sz, _vp],
     "bl_cdist_tuned": [_vp, _vp, _i64, _vp, _int, _vp, _int, _int, _vp, _vp, _sz, _vp, _vp],
+    "bl_cdist_matches": [_vp, _vp, _i64, _vp, _int, _int, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _sz,
+                         _vp, _vp],
     "bl_pairs_workspace_size": [_i64, _i64],
     "bl_pairs": [_vp, _vp, _vp, _vp, _i64, _i64, _vp, _int, _int, _int, _vp, _sz, _vp],
     "bl_pairs_tuned": [_vp, _vp, _vp, _vp, _i64, _i64, _vp, _int, _int, _int, _vp, _vp, _sz, _vp,

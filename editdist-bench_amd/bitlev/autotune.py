@@ -5,8 +5,8 @@ pipeline on a sample of the batch (HIP events, median of a few runs), keeps
 the fastest and caches it per device and workload signature in a JSON file
 ($BITLEV_TUNE_CACHE, default ~/.cache/bitlev/autotune.json).
 
-cdist: lane mode (compacted lockstep passes vs strided lanes), persistent
-waves per CU, choices per tile. pairs: direct (one launch) or bucketed, and
+cdist: lane mode (compacted lockstep passes, strided lanes, or for k <= 3 the
+four-choice byte filter), persistent waves per CU, choices per tile. pairs: direct (one launch) or bucketed, and
 for bucketed persistent waves per CU, workgroup size, text-length bucket
 width, bucketing chunk; and whether to trim common affixes (which never
 changes a distance). Every candidate computes the same results; only speed
@@ -27,6 +27,8 @@ from ._capi import call
 
 CDIST_SPACE = {"mode": ["strided", "queue"], "waves_per_cu": [8, 16, 32],
                "tile": [2048, 8192, 32768]}
+# the four-choice byte filter, only where it can decide (k <= 3)
+CDIST_SWAR_SPACE = {"mode": ["swar"], "waves_per_cu": [8, 16, 32], "tile": [2048, 8192, 32768]}
 PAIRS_SPACE = {"waves_per_cu": [8, 16, 32], "block": [64, 256], "bin_width": [8, 16, 32],
                "chunk": [16384, 65536], "trim": [True, False]}
 # direct mode (one launch, no bucketing): only the launch shape matters
@@ -127,7 +129,10 @@ def tune_cdist(Q: hip.Packed, C: hip.Packed, out: hip.DeviceArray, k: int = -1, 
     Qs = Q if nq == Q.n else hip.Packed(*_head(Q, nq))
     ws = hip.Workspace()
     res = []
-    for p in _grid(space or CDIST_SPACE):
+    cands = list(_grid(space or CDIST_SPACE))
+    if space is None and 0 <= k <= 3:
+        cands += list(_grid(CDIST_SWAR_SPACE))
+    for p in cands:
         ms = _time(lambda: hip.cdist_batched_into(Qs, C, out, k, stream, ws, p), stream)
         res.append((ms, p))
         if log:

@@ -40,7 +40,7 @@ FORM = "batched"                     # default form of pairwise() / cdist()
 __all__ = ["available", "distance", "pairwise", "cdist", "Packed", "Plan", "pairs_plan",
            "cdist_plan", "pairs_into", "cdist_into", "pairs_batched_into", "cdist_batched_into",
            "run_pairs", "run_cdist", "Workspace", "Stream", "Graph", "pair_classes", "query_classes",
-           "ChoiceStripes",
+           "ChoiceStripes", "cdist_matches", "cdist_matches_into",
            "DeviceArray", "BackendUnavailable", "FORM"]
 
 
@@ -245,7 +245,8 @@ def _cdist_params(params):
     p = CdistParams()
     p.waves_per_cu = int(params.get("waves_per_cu", 0))
     p.tile = int(params.get("tile", 0))
-    p.reserved[0] = 1 if params.get("mode") == "strided" else 0
+    p.reserved[0] = {"strided": 1, "swar": 2}.get(params.get("mode"), 0)
+    p.reserved[1] = int(params.get("grab", 0))
     return ctypes.byref(p)
 
 
@@ -258,6 +259,7 @@ def _pairs_params(params):
     p.bin_width = int(params.get("bin_width", 0))
     p.reserved[0] = int(params.get("chunk", 0))
     p.reserved[1] = 1 if params.get("mode") == "direct" else 0
+    p.reserved[2] = int(params.get("grab", 0))
     return ctypes.byref(p)
 
 
@@ -421,6 +423,60 @@ def cdist(queries, choices, max_dist=None, dtype=np.int32, return_device: bool =
     res = _finish(out, return_device)
     del keep
     return res
+
+
+def cdist_matches_into(Q: Packed, C: Packed, k: int, stream=None,
+                       workspace: Workspace | None = None, params=None,
+                       kvec: DeviceBuffer | None = None, capacity: int | None = None):
+    """Every (q, c) with d(Q[q], C[c]) <= k as three device arrays (q int32,
+    c int64, d int32) in no particular order, through bl_cdist_matches: the
+    search without the dense Q.n x C.n result. Grows the buffers and runs
+    again when the first capacity was too small; synchronises `stream`."""
+    if k < 0 and kvec is None:
+        raise ValueError("a match list needs max_dist >= 0")
+    mq = min(Q.max_len, BATCHED_MAX_QUERY)
+    if Q.max_len > BATCHED_MAX_QUERY:
+        raise ValueError("match lists take queries of at most 512 bytes")
+    ws = (workspace or _workspace(stream)).get(lib().bl_cdist_workspace_size(Q.n, mq))
+    cap = int(capacity if capacity is not None else min(max(Q.n * C.n, 1), 1 << 20))
+    count = DeviceBuffer(8)
+    host_count = np.zeros(1, np.uint64)
+    while True:
+        dq = DeviceArray.empty((max(cap, 1),), np.int32)
+        dc = DeviceArray.empty((max(cap, 1),), np.int64)
+        dd = DeviceArray.empty((max(cap, 1),), np.int32)
+        call("bl_cdist_matches", Q.d_buf.ptr, Q.d_off.ptr, Q.n, C.stripes(stream).ref, mq, int(k),
+             kvec.ptr if kvec else None, dq.ptr, dc.ptr, dd.ptr, cap, count.ptr, ws.ptr, ws.nbytes,
+             _cdist_params(params), stream)
+        call("bl_memcpy_dtoh", host_count.ctypes.data, count.ptr, 8, stream)
+        call("bl_stream_sync", stream)
+        n = int(host_count[0])
+        if n <= cap:
+            return DeviceArray((n,), np.int32, dq._buf), DeviceArray((n,), np.int64, dc._buf), \
+                DeviceArray((n,), np.int32, dd._buf)
+        cap = n
+
+
+def cdist_matches(queries, choices, max_dist, return_device: bool = False):
+    """Every pair within max_dist: (q, c, d) arrays with d(queries[q],
+    choices[c]) = d <= max_dist, sorted by (q, c) when returned to the host
+    (device arrays come in no particular order). The reference's cdist with
+    max_dist gives the same information as a dense matrix (bitlev/gpu.py:106-138)."""
+    q, c = as_list(queries), as_list(choices)
+    k = check_k(max_dist)
+    if k < 0:
+        raise ValueError("a match list needs max_dist >= 0")
+    _require_device()
+    if len(q) == 0 or len(c) == 0:
+        z = (np.empty(0, np.int32), np.empty(0, np.int64), np.empty(0, np.int32))
+        return z
+    Q, C = Packed(*pack(q)), Packed(*pack(c))
+    dq, dc, dd = cdist_matches_into(Q, C, k)
+    if return_device:
+        return dq, dc, dd
+    hq, hc, hd = dq.get(), dc.get(), dd.get()
+    order = np.lexsort((hc, hq))
+    return hq[order], hc[order], hd[order]
 
 
 def distance(a, b, max_dist=None) -> int:

@@ -29,6 +29,8 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "bl_device.h"
 #include "bl_internal.h"
 
@@ -86,6 +88,21 @@ __device__ __forceinline__ long long wave_take(unsigned long long *counter, long
     return (long long)(((u64)hi << 32) | lo);
 }
 
+/* Work-queue tickets taken `grab` at a time: one atomic per grab (the
+ * counter is device-wide, so every atomic leaves the XCD's L2), then the
+ * wave walks its run. next / end are wave-uniform. */
+struct Tickets {
+    long long next = 0, end = 0;
+    __device__ __forceinline__ long long get(unsigned long long *counter, int grab)
+    {
+        if (next >= end) {
+            next = wave_take(counter, grab);
+            end = next + grab;
+        }
+        return next++;
+    }
+};
+
 __global__ void clear_kernel(unsigned long long *p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -93,6 +110,43 @@ __global__ void clear_kernel(unsigned long long *p, int n)
 }
 
 /* ======================================================================= cdist */
+/* Where search results go: the dense row-major q x nc matrix in one of the
+ * output dtypes (the reference's result), or a match list of (q, c, d) for
+ * every d <= k, appended in no particular order. put() receives the result
+ * min(d, k + 1) of output index q * nc + c (or -1 for a caller error). */
+struct DenseSink {
+    void *out;
+    int dtype;
+    static constexpr bool kDense = true;
+    __device__ __forceinline__ void put(int64_t idx, int d, int) const { store_out(out, idx, d, dtype); }
+};
+
+__device__ __forceinline__ int dense_dtype(const DenseSink &s) { return s.dtype; }
+__device__ __forceinline__ void *dense_out(const DenseSink &s) { return s.out; }
+
+struct MatchSink {
+    unsigned long long *count;  /* matches found (may exceed cap: the caller retries) */
+    int32_t *mq;
+    int64_t *mc;
+    int32_t *md;
+    int64_t cap, nc;
+    static constexpr bool kDense = false;
+    __device__ __forceinline__ void put(int64_t idx, int d, int k) const
+    {
+        if (d >= 0 && d <= k) {
+            const unsigned long long pos = atomicAdd(count, 1ull);
+            if (pos < (unsigned long long)cap) {
+                const int64_t q = idx / nc;
+                mq[pos] = (int32_t)q;
+                mc[pos] = idx - q * nc;
+                md[pos] = d;
+            }
+        }
+    }
+};
+__device__ __forceinline__ int dense_dtype(const MatchSink &) { return DT_INT32; }
+__device__ __forceinline__ void *dense_out(const MatchSink &) { return nullptr; }
+
 constexpr int CD_SCAP = 128;   /* pending-choice stack entries per wave (>= 64 + 64 - 1) */
 constexpr int CD_ECAP = 128;   /* escalation stack entries per wave (>= 64 + 64 - 1) */
 constexpr int CD_DIRECT = 32;  /* in-band choices a window needs to be scanned in place */
@@ -106,10 +160,17 @@ struct Pend {
     u32 Pv, Mv, pad;
 };
 
+constexpr int CD_OCAP = 320;   /* open stack entries per wave (>= 64 + 256 - 1) */
+
 /* One wave's slice of LDS. */
 struct WaveLds {
     u32 lo32[256];             /* rows 0..31 of the current query's Peq table */
-    Pend pend[CD_SCAP];        /* choices waiting for a pass */
+    u8 lo8[256];               /* rows 0..7 of it (the four-choice filter's table) */
+    union {
+        Pend pend[CD_SCAP];    /* compacted mode: choices waiting for a pass */
+        int64_t open[CD_OCAP]; /* filter mode: (query, choice) pairs to rescan, as the
+                                  output index q * nc + c, across tiles */
+    };
     int32_t esc[CD_ECAP];      /* choices the 32-row stage cannot decide */
 };
 
@@ -155,15 +216,14 @@ __global__ __launch_bounds__(256) void peq_build_kernel(const u8 *__restrict__ b
  *     64-row stage, Peq rows from the query's global table).
  * Needs m <= 32 or k >= 0 (otherwise every choice would escalate: the caller
  * runs strided). */
-template <int NW>
+template <int NW, typename Sink>
 __device__ void cdist_tile(WaveLds &L, const u64 *__restrict__ gpeq, int m,
                            const u32 *__restrict__ words, const int64_t *__restrict__ gbase,
-                           const int32_t *__restrict__ clen, int64_t c0, int64_t c1, void *out,
-                           int64_t orow, int dtype, int k)
+                           const int32_t *__restrict__ clen, int64_t c0, int64_t c1,
+                           const Sink &sink, int64_t orow, int k)
 {
     const int lane = lane_id();
     const bool whole = m <= 32;
-    u8 *const out8 = (u8 *)out + orow;          /* uint8 results: the common case */
     int64_t w = c0;
     int ns = 0, ne = 0;
     for (;;) {
@@ -175,8 +235,7 @@ __device__ void cdist_tile(WaveLds &L, const u64 *__restrict__ gpeq, int m,
             if (lane < cnt) {
                 const int64_t c = c0 + L.esc[base + lane];
                 const StripeText text{words + gbase[c >> 6] + (c & 63)};
-                store_out(out, orow + c,
-                          peq_lane<NW, 1>((const u32 *)nullptr, gpeq, m, text, clen[c], k), dtype);
+                sink.put(orow + c, peq_lane<NW, 1>((const u32 *)nullptr, gpeq, m, text, clen[c], k), k);
             }
             ne = uni(base);
             wave_lds_sync();
@@ -201,7 +260,7 @@ __device__ void cdist_tile(WaveLds &L, const u64 *__restrict__ gpeq, int m,
             w = uni64(w + 64);
             const int gap = n > m ? n - m : m - n;
             const bool decided = valid && ((k >= 0 && gap > k) || n == 0);
-            if (decided) store_out(out, orow + c, (k >= 0 && gap > k) ? k + 1 : m, dtype);
+            if (decided) sink.put(orow + c, (k >= 0 && gap > k) ? k + 1 : m, k);
             /* the 32-row stage cannot test even its first group: escalate now */
             const bool direct = valid && !decided && !whole && 4 + m - n > 32;
             act = valid && !decided && !direct;
@@ -293,9 +352,7 @@ __device__ void cdist_tile(WaveLds &L, const u64 *__restrict__ gpeq, int m,
                     const int bound = diag_bound<u32, 1>(P1, M1, 1, j, m, n);
                     const bool over = k >= 0 && bound > k;
                     if (over || (whole && j >= n)) {
-                        const int d = over ? k + 1 : bound;
-                        if (dtype == DT_UINT8) out8[c0 + rel] = (u8)(d > 255 ? 255 : d);
-                        else store_out(out, orow + c0 + rel, d, dtype);
+                        sink.put(orow + c0 + rel, over ? k + 1 : bound, k);
                         act = false;
                     } else if (!whole && j + 4 + m - n > 32) {
                         escalate = true;
@@ -334,27 +391,88 @@ __device__ void cdist_tile(WaveLds &L, const u64 *__restrict__ gpeq, int m,
  * through all its stages (the plain kernel's lane loop inside the persistent
  * tiles). Cheaper than the ring when every choice needs about the same work,
  * and the only mode for exact (k < 0) queries above 32 bytes. */
-template <int NW>
+template <int NW, typename Sink>
 __device__ void cdist_tile_strided(const u32 *lo32, const u64 *__restrict__ gpeq, int m,
                                    const u32 *__restrict__ words, const int64_t *__restrict__ gbase,
                                    const int32_t *__restrict__ clen, int64_t c0, int64_t c1,
-                                   void *out, int64_t orow, int dtype, int k)
+                                   const Sink &sink, int64_t orow, int k)
 {
     for (int64_t c = c0 + lane_id(); c < c1; c += 64) {
         const StripeText text{words + gbase[c >> 6] + (c & 63)};
-        store_out(out, orow + c, peq_lane<NW, 0>(lo32, gpeq, m, text, clen[c], k), dtype);
+        sink.put(orow + c, peq_lane<NW, 0>(lo32, gpeq, m, text, clen[c], k), k);
     }
+}
+
+/* SWAR mode, for small thresholds (k <= 3) and queries of at least 8 bytes:
+ * each lane carries four choices of a 256-choice window (w + l, w + 64 + l,
+ * w + 128 + l, w + 192 + l -- four stripe groups, so every load is coalesced)
+ * in the four bytes of one u32, each byte holding pattern rows 0..7 of its
+ * choice. Four text columns of the Myers step run on all four at once with
+ * byte-segmented arithmetic (carries and shifts kept inside each byte; row 0
+ * of every segment gets the +1 of a global alignment), then the final-diagonal
+ * bound is tested at column 4: an in-band choice has |n - m| <= k <= 3, so its
+ * diagonal row 4 + m - n lies in 1..7, within the segment, and the low rows
+ * evolve exactly as in the whole scan. Choices the bound decides get k + 1;
+ * the rest (a few percent of unrelated strings) are stacked and rescanned 64
+ * at a time by the staged lane code from column 0. */
+/* One 256-choice window of the filter. FULL: all 256 choices lie in the tile
+ * (no per-lane bounds checks). DENSE: results are stored through `ow` (the
+ * output row at the window's first choice) as OutT. The choices the filter leaves
+ * open are pushed on the wave's open stack as their output index orow + c
+ * (the caller keeps fewer than 64 there between windows, so it never holds
+ * more than 319). */
+template <bool FULL, bool DENSE, typename OutT>
+__device__ __forceinline__ void swar_window(WaveLds &L, int m, const u32 *__restrict__ words,
+                                            const int64_t *__restrict__ gbase,
+                                            const int32_t *__restrict__ lw, OutT *ow, int64_t w,
+                                            int64_t c1, int k, int64_t orow, int &ns)
+{
+    const int lane = lane_id();
+    const u32 kk1 = (u32)k + 1u;            /* k + 1 <= 4 fits every output type */
+    u32 d[4];
+    int n[4];
+    bool valid[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const u32 *gp = words + gbase[(w >> 6) + s];          /* a scalar load per group */
+        valid[s] = FULL || w + 64 * s + lane < c1;
+        n[s] = valid[s] ? lw[64 * s + lane] : 0;
+        d[s] = valid[s] ? gp[lane] : 0u;
+    }
+    int bound[4];
+    swar_bounds4(L.lo8, d, m, n, bound, 1);
+    bool open[4];
+    bool any = false;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const int gap = n[s] > m ? n[s] - m : m - n[s];
+        /* out of band (an empty choice has gap m >= 8 > k), or the bound at
+         * column 4 exceeds k: the answer is k + 1 */
+        const bool decided = gap > k || bound[s] > k;
+        /* (k + 1 is never a match: a match list gets nothing) */
+        if (DENSE && valid[s] && decided) ow[64 * s + lane] = (OutT)kk1;
+        open[s] = valid[s] && !decided;
+        any = any || open[s];
+    }
+    if (wave_ballot(any) == 0) return;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const u64 sm = wave_ballot(open[s]);
+        if (open[s]) L.open[ns + rank_below(sm)] = orow + w + 64 * s + lane;
+        ns = uni(ns + __popcll(sm));
+    }
+    wave_lds_sync();
 }
 
 /* Persistent cdist over queries of at most 64*NWMAX bytes whose Peq tables
  * peq_build_kernel wrote to gpeq. Waves pull (choice block, query) tiles,
  * block-major, from the work-queue counter. */
-template <int NWMAX>
+template <int NWMAX, typename Sink>
 __global__ __launch_bounds__(256) void cdist_batched_kernel(
     const int64_t *__restrict__ oq, int64_t nq, const u64 *__restrict__ gpeq,
     const u32 *__restrict__ words, const int64_t *__restrict__ gbase,
-    const int32_t *__restrict__ clen, int64_t nc, void *out, int dtype,
-    int k_all, const int32_t *__restrict__ kvec, int64_t tile, int strided,
+    const int32_t *__restrict__ clen, int64_t nc, Sink sink,
+    int k_all, const int32_t *__restrict__ kvec, int64_t tile, int mode, int grab,
     unsigned long long *counter)
 {
     extern __shared__ u64 lds[];
@@ -363,35 +481,85 @@ __global__ __launch_bounds__(256) void cdist_batched_kernel(
     const int64_t nblk = (nc + tile - 1) / tile;
     const int64_t ntiles = nblk * nq;
     int64_t cur_q = -1;
+    /* filter mode: choices the filter left open, as output indices, rescanned
+     * 64 at a time with the current query's tables (the stack is emptied
+     * before the wave moves to another tile) */
+    int ns = 0;
+    const u64 *qpeq = nullptr;
+    int m = 0, k = 0;
+    auto drain = [&](int cnt) {
+        const int base = ns - cnt;
+        if (lane < cnt) {
+            const int64_t idx = L.open[base + lane];
+            const int64_t c = idx % nc;
+            const StripeText text{words + gbase[c >> 6] + (c & 63)};
+            sink.put(idx, peq_lane<NWMAX, 0>(L.lo32, qpeq, m, text, clen[c], k), k);
+        }
+        ns = uni(base);
+        wave_lds_sync();
+    };
+    Tickets tk;
     for (;;) {
-        const long long t = wave_take(counter, 1);
+        const long long t = tk.get(counter, grab);
         if (t >= ntiles) break;
         const int64_t blk = t / nq, q = t % nq;
         const int64_t c0 = blk * tile;
         const int64_t c1 = c0 + tile < nc ? c0 + tile : nc;
-        const int m = (int)(oq[q + 1] - oq[q]);
+        m = (int)(oq[q + 1] - oq[q]);
         const int64_t orow = q * nc;
-        const int k = kvec ? clamp_k_dev(kvec[q]) : k_all;
+        k = kvec ? clamp_k_dev(kvec[q]) : k_all;
         if (m == 0 || m > 64 * NWMAX) {
             /* empty query: n, or k + 1 (csrc/gpu_kernels.cu:53-54); too long for this
              * launch: caller error, marked -1 */
             for (int64_t c = c0 + lane; c < c1; c += 64)
-                store_out(out, orow + c, m == 0 ? empty_pattern(clen[c], k) : -1, dtype);
+                sink.put(orow + c, m == 0 ? empty_pattern(clen[c], k) : -1, k);
             continue;
         }
-        const u64 *qpeq = gpeq + (size_t)q * 256 * NWMAX;
+        qpeq = gpeq + (size_t)q * 256 * NWMAX;
         if (q != cur_q) {
             wave_lds_sync();
 #pragma unroll
-            for (int x = 0; x < 4; x++) L.lo32[x * 64 + lane] = (u32)qpeq[(x * 64 + lane) * NWMAX];
+            for (int x = 0; x < 4; x++) {
+                const u32 v = (u32)qpeq[(x * 64 + lane) * NWMAX];
+                L.lo32[x * 64 + lane] = v;
+                L.lo8[x * 64 + lane] = (u8)v;
+            }
             wave_lds_sync();
             cur_q = q;
         }
-        if (strided || (m > 32 && k < 0))
-            cdist_tile_strided<NWMAX>(L.lo32, qpeq, m, words, gbase, clen, c0, c1, out, orow,
-                                      dtype, k);
-        else
-            cdist_tile<NWMAX>(L, qpeq, m, words, gbase, clen, c0, c1, out, orow, dtype, k);
+        /* (the filter's rescan of 8-word queries would cost the whole launch its
+         * occupancy: queries above 256 bytes use the other modes) */
+        if (NWMAX <= 4 && mode == 2 && k >= 0 && k <= 3 && m >= 8 && (c0 & 255) == 0) {
+            /* 256-choice windows through the four-choice filter; the open
+             * stack is emptied to below 64 between windows */
+            int64_t w = c0;
+            auto window = [&](auto full) {
+                constexpr bool F = decltype(full)::value;
+                int64_t w0 = w;
+                if (!Sink::kDense)
+                    swar_window<F, false>(L, m, words, gbase, clen + w0, (u8 *)nullptr, w0, c1, k,
+                                          orow, ns);
+                else if (dense_dtype(sink) == DT_UINT8)
+                    swar_window<F, true>(L, m, words, gbase, clen + w0,
+                                         (u8 *)dense_out(sink) + orow + w0, w0, c1, k, orow, ns);
+                else if (dense_dtype(sink) == DT_UINT16)
+                    swar_window<F, true>(L, m, words, gbase, clen + w0,
+                                         (uint16_t *)dense_out(sink) + orow + w0, w0, c1, k, orow, ns);
+                else
+                    swar_window<F, true>(L, m, words, gbase, clen + w0,
+                                         (int32_t *)dense_out(sink) + orow + w0, w0, c1, k, orow, ns);
+                while (ns >= 64) drain(64);
+            };
+            for (; w + 256 <= c1; w += 256) window(std::true_type());
+            if (w < c1) window(std::false_type());
+            while (ns > 0) drain(ns < 64 ? ns : 64);
+        }
+        else {
+            if (mode == 1 || (m > 32 && k < 0))
+                cdist_tile_strided<NWMAX>(L.lo32, qpeq, m, words, gbase, clen, c0, c1, sink, orow, k);
+            else
+                cdist_tile<NWMAX>(L, qpeq, m, words, gbase, clen, c0, c1, sink, orow, k);
+        }
     }
 }
 
@@ -582,14 +750,15 @@ __global__ __launch_bounds__(256) void pairs_batched_kernel(
     const int32_t *__restrict__ sorted, const unsigned short *__restrict__ key_of,
     const unsigned long long *__restrict__ hist, const unsigned long long *__restrict__ chunk_base,
     const unsigned long long *__restrict__ tile_base, int64_t nchunks, void *out, int dtype,
-    unsigned long long *counter, int8_t *scratch, int64_t slot)
+    int grab, unsigned long long *counter, int8_t *scratch, int64_t slot)
 {
     const int64_t ntiles = (int64_t)tile_base[nchunks];
     const int64_t wave_gid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     int8_t *hbuf = scratch ? scratch + wave_gid * slot : nullptr;
     int64_t c = 0;
+    Tickets tk;
     for (;;) {
-        const long long t = wave_take(counter, 1);
+        const long long t = tk.get(counter, grab);
         if (t >= ntiles) break;
         /* chunk of tile t: tile_base[c] <= t < tile_base[c + 1]; tiles are taken in
          * increasing order, so search upwards from the last chunk, then bisect */
@@ -703,30 +872,27 @@ size_t bl_cdist_workspace_size(int64_t nq, int max_qlen)
     return 256 + (size_t)nq * 256 * cdist_nwmax(max_qlen) * sizeof(u64);
 }
 
-int bl_cdist_tuned(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
-                   const bl_stripes *choices, int max_qlen, void *d_out, int dtype, int k,
-                   const int32_t *d_kvec, void *d_work, size_t work_bytes,
-                   const bl_cdist_params *prm, void *stream)
+}  /* extern "C" */
+
+/* The search launch behind bl_cdist_tuned and bl_cdist_matches. */
+template <typename Sink>
+static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
+                        const bl_stripes *choices, int max_qlen, int k, const int32_t *d_kvec,
+                        void *d_work, size_t work_bytes, const bl_cdist_params *prm, void *stream,
+                        const Sink &sink, unsigned long long *extra_clear)
 {
-    if (!choices) return fail(BL_EINVAL, "bl_cdist: null choices");
     const int64_t nc = choices->n;
-    if (nq < 0 || nc < 0 || max_qlen < 0) return fail(BL_EINVAL, "bl_cdist: negative size");
-    if (nq == 0 || nc == 0) return BL_OK;
-    if (dtype < BL_INT32 || dtype > BL_UINT8) return fail(BL_EINVAL, "bl_cdist: bad dtype %d", dtype);
-    if (!d_out || !d_buf_q || !d_off_q || !choices->words || !choices->gbase || !choices->len)
-        return fail(BL_EINVAL, "bl_cdist: null pointer");
-    if (max_qlen > 512) return fail(BL_EINVAL, "bl_cdist: queries above 512 bytes go to bl_long");
-    if (nq > 0x7fffffff) return fail(BL_EINVAL, "bl_cdist: too many queries");
     const size_t need_ws = bl_cdist_workspace_size(nq, max_qlen);
     if (d_work && work_bytes < need_ws)
-        return fail(BL_EINVAL, "bl_cdist: workspace of %zu bytes below the %zu needed", work_bytes,
+        return fail(BL_EINVAL, "%s: workspace of %zu bytes below the %zu needed", who, work_bytes,
                     need_ws);
-    k = clamp_k_dev(k);
     hipStream_t s = (hipStream_t)stream;
     const int wpc = prm && prm->waves_per_cu > 0 ? prm->waves_per_cu : 16;
-    /* tiles start on stripe groups (multiples of 64 choices) */
-    const int64_t tile = (prm && prm->tile > 0 ? (prm->tile + 63) / 64 * 64 : 8192);
-    const int strided = prm && prm->reserved[0] == 1 ? 1 : 0;
+    /* tiles start on stripe groups (multiples of 64 choices; of 256 for the
+     * four-choice filter's windows) */
+    const int64_t tile = (prm && prm->tile > 0 ? (prm->tile + 255) / 256 * 256 : 8192);
+    const int mode = prm && prm->reserved[0] >= 1 && prm->reserved[0] <= 2 ? prm->reserved[0] : 0;
+    const int grab = prm && prm->reserved[1] > 0 ? prm->reserved[1] : 1;
     const int nwmax = cdist_nwmax(max_qlen);
     const size_t lds = sizeof(WaveLds) * 4;
     int64_t blocks = (int64_t)device_cus() * wpc / 4;
@@ -739,35 +905,82 @@ int bl_cdist_tuned(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
     unsigned long long *ctr = (unsigned long long *)ws;
     u64 *gpeq = (u64 *)(ws + 256);
     int rc = launch_clear(ctr, 1, s);
+    if (rc == BL_OK && extra_clear) rc = launch_clear(extra_clear, 1, s);
     if (rc == BL_OK) {
         peq_build_kernel<<<(unsigned)nq, 256, 0, s>>>(d_buf_q, d_off_q, nwmax, gpeq);
         switch (nwmax) {
         case 1:
             cdist_batched_kernel<1><<<(unsigned)blocks, 256, lds, s>>>(
-                d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, nc, d_out, dtype, k,
-                d_kvec, tile, strided, ctr);
+                d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, nc, sink, k, d_kvec,
+                tile, mode, grab, ctr);
             break;
         case 2:
             cdist_batched_kernel<2><<<(unsigned)blocks, 256, lds, s>>>(
-                d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, nc, d_out, dtype, k,
-                d_kvec, tile, strided, ctr);
+                d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, nc, sink, k, d_kvec,
+                tile, mode, grab, ctr);
             break;
         case 4:
             cdist_batched_kernel<4><<<(unsigned)blocks, 256, lds, s>>>(
-                d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, nc, d_out, dtype, k,
-                d_kvec, tile, strided, ctr);
+                d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, nc, sink, k, d_kvec,
+                tile, mode, grab, ctr);
             break;
         default:
             cdist_batched_kernel<8><<<(unsigned)blocks, 256, lds, s>>>(
-                d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, nc, d_out, dtype, k,
-                d_kvec, tile, strided, ctr);
+                d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, nc, sink, k, d_kvec,
+                tile, mode, grab, ctr);
             break;
         }
         hipError_t e = hipGetLastError();
-        if (e != hipSuccess) rc = fail(BL_EHIP, "cdist_batched_kernel launch: %s", hipGetErrorString(e));
+        if (e != hipSuccess) rc = fail(BL_EHIP, "%s launch: %s", who, hipGetErrorString(e));
     }
     if (!d_work) (void)hipFreeAsync(ws, s);
     return rc;
+}
+
+static int cdist_check(const char *who, const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
+                       const bl_stripes *choices, int max_qlen)
+{
+    if (!choices) return fail(BL_EINVAL, "%s: null choices", who);
+    if (nq < 0 || choices->n < 0 || max_qlen < 0) return fail(BL_EINVAL, "%s: negative size", who);
+    if (nq > 0 && choices->n > 0 &&
+        (!d_buf_q || !d_off_q || !choices->words || !choices->gbase || !choices->len))
+        return fail(BL_EINVAL, "%s: null pointer", who);
+    if (max_qlen > 512) return fail(BL_EINVAL, "%s: queries above 512 bytes go to bl_long", who);
+    if (nq > 0x7fffffff) return fail(BL_EINVAL, "%s: too many queries", who);
+    return BL_OK;
+}
+
+extern "C" {
+
+int bl_cdist_tuned(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
+                   const bl_stripes *choices, int max_qlen, void *d_out, int dtype, int k,
+                   const int32_t *d_kvec, void *d_work, size_t work_bytes,
+                   const bl_cdist_params *prm, void *stream)
+{
+    if (int rc = cdist_check("bl_cdist", d_buf_q, d_off_q, nq, choices, max_qlen)) return rc;
+    if (nq == 0 || choices->n == 0) return BL_OK;
+    if (dtype < BL_INT32 || dtype > BL_UINT8) return fail(BL_EINVAL, "bl_cdist: bad dtype %d", dtype);
+    if (!d_out) return fail(BL_EINVAL, "bl_cdist: null pointer");
+    DenseSink sink{d_out, dtype};
+    return cdist_launch("bl_cdist", d_buf_q, d_off_q, nq, choices, max_qlen, clamp_k_dev(k), d_kvec,
+                        d_work, work_bytes, prm, stream, sink, nullptr);
+}
+
+int bl_cdist_matches(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
+                     const bl_stripes *choices, int max_qlen, int k, const int32_t *d_kvec,
+                     int32_t *d_q, int64_t *d_c, int32_t *d_d, int64_t capacity,
+                     unsigned long long *d_count, void *d_work, size_t work_bytes,
+                     const bl_cdist_params *prm, void *stream)
+{
+    if (int rc = cdist_check("bl_cdist_matches", d_buf_q, d_off_q, nq, choices, max_qlen)) return rc;
+    if (k < 0 && !d_kvec) return fail(BL_EINVAL, "bl_cdist_matches: needs max_dist >= 0");
+    if (!d_count || capacity < 0 || (capacity > 0 && (!d_q || !d_c || !d_d)))
+        return fail(BL_EINVAL, "bl_cdist_matches: null output or negative capacity");
+    if (nq == 0 || choices->n == 0)
+        return launch_clear(d_count, 1, (hipStream_t)stream);
+    MatchSink sink{d_count, d_q, d_c, d_d, capacity, choices->n};
+    return cdist_launch("bl_cdist_matches", d_buf_q, d_off_q, nq, choices, max_qlen, clamp_k_dev(k),
+                        d_kvec, d_work, work_bytes, prm, stream, sink, d_count);
 }
 
 int bl_cdist(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq, const bl_stripes *choices,
@@ -845,6 +1058,7 @@ int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t
     const int wpc = prm && prm->waves_per_cu > 0 ? prm->waves_per_cu : 16;
     const int wg = prm && (prm->block == 64 || prm->block == 128 || prm->block == 256) ? prm->block : 256;
     const bool direct = prm && prm->reserved[1] == 1;
+    const int grab = prm && prm->reserved[2] > 0 ? prm->reserved[2] : 1;
 
     char *w = (char *)d_work;
     if (!w) BL_HIP(hipMallocAsync((void **)&w, L.total, s));
@@ -892,7 +1106,7 @@ int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t
             pairs_scatter_kernel<<<cblocks, 256, 0, s>>>(key, n, L.chunk, cursor, chunk_base, sorted);
             pairs_batched_kernel<<<(unsigned)blocks, wg, 0, s>>>(
                 d_buf_a, d_off_a, d_buf_b, d_off_b, trim, k, d_kvec, sorted, key, hist, chunk_base,
-                tile_base, L.nchunks, d_out, dtype, ctr, scratch, slot);
+                tile_base, L.nchunks, d_out, dtype, grab, ctr, scratch, slot);
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) rc = fail(BL_EHIP, "pairs pipeline launch: %s", hipGetErrorString(e));
         }

@@ -584,6 +584,52 @@ BL_HD int peq_lane(Lo32Ptr lo32, PeqPtr peq, int m, const Text &t, int n, int k)
  * (csrc/gpu_kernels.cu:53-54, csrc/edpy.c:111-115). */
 BL_HD int empty_pattern(int n, int k) { return (k >= 0 && n > k) ? k + 1 : n; }
 
+/* ------------------------------------------------------------ 4-choice filter */
+/* Bytewise x + y with no carry between bytes (the top bit of each byte loses
+ * its carry-out, which belongs to rows beyond the byte's segment). */
+BL_HD u32 swar_add(u32 x, u32 y)
+{
+    const u32 lo = (x & 0x7F7F7F7Fu) + (y & 0x7F7F7F7Fu);
+    return lo ^ ((x ^ y) & 0x80808080u);
+}
+
+/* Four choices in the four bytes of u32 blocks, byte s holding pattern rows
+ * 0..7 of choice s (a pattern of m >= 8 rows), stepped through their first 4
+ * text columns (d[s]: their first 4 bytes, n[s] >= 4) at once: carries and
+ * shifts stay inside each byte, row 0 of every byte gets the +1 of a global
+ * alignment. lo8[stride * c] = rows 0..7 of the Peq row of byte value c.
+ * bound[s] = the final-diagonal bound D[4 + m - n[s]][4] when that row lies
+ * in 1..8 (exact: rows 0..7 evolve as in the whole scan), else -1. */
+template <typename Lo8Ptr>
+BL_HD void swar_bounds4(Lo8Ptr lo8, const u32 (&d)[4], int m, const int (&n)[4], int (&bound)[4],
+                        int stride = 4)
+{
+    u32 Pv = ~0u, Mv = 0u;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const u32 E = (u32)lo8[stride * ((d[0] >> (8 * u)) & 0xFFu)] |
+                      ((u32)lo8[stride * ((d[1] >> (8 * u)) & 0xFFu)] << 8) |
+                      ((u32)lo8[stride * ((d[2] >> (8 * u)) & 0xFFu)] << 16) |
+                      ((u32)lo8[stride * ((d[3] >> (8 * u)) & 0xFFu)] << 24);
+        const u32 Xv = E | Mv;
+        const u32 S = swar_add(E & Pv, Pv);
+        const u32 Xh = bop3<tt((TA ^ TB) | TC)>(S, Pv, E);          /* (S ^ Pv) | E    */
+        const u32 Ph = bop3<tt(TA | ~(TB | TC))>(Mv, Xh, Pv);       /* Mv | ~(Xh | Pv) */
+        const u32 Mh = Pv & Xh;
+        const u32 Phs = (Ph << 1) | 0x01010101u;                    /* +1 into row 0 */
+        const u32 Mhs = (Mh << 1) & 0xFEFEFEFEu;
+        Pv = bop3<tt(TA | ~(TB | TC))>(Mhs, Xv, Phs);               /* Mhs | ~(Xv | Phs) */
+        Mv = Phs & Xv;
+    }
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const int i0 = 4 + m - n[s];
+        const u32 mk = i0 >= 8 ? 0xFFu : ((1u << (i0 > 0 ? i0 : 0)) - 1u);
+        const int b = 4 + popc((u32)((Pv >> (8 * s)) & mk)) - popc((u32)((Mv >> (8 * s)) & mk));
+        bound[s] = (i0 >= 1 && i0 <= 8) ? b : -1;
+    }
+}
+
 /* ------------------------------------------------------------ long strings */
 /* Pairs whose shorter string exceeds 256 bytes, and cross queries longer than
  * 512 bytes, run here instead of on the CPU (the reference falls back to its C

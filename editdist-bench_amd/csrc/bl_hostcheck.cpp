@@ -258,6 +258,22 @@ int blh_pairs_batched(const u8 *ba, const int64_t *oa, const u8 *bb, const int64
     return 0;
 }
 
+/* The 4-choice byte-segment filter: bounds of choices t[0..3] (first 4 bytes
+ * each, lengths n[0..3]) against a pattern of m >= 8 bytes. */
+void blh_swar_bounds(const u8 *q, int m, const u8 *t, const int *n, int *bound)
+{
+    std::vector<u32> lo32(256, 0);
+    for (int i = 0; i < m && i < 32; i++) lo32[q[i]] |= 1u << i;
+    u32 d[4];
+    int nn[4], b[4];
+    for (int s = 0; s < 4; s++) {
+        memcpy(&d[s], t + 4 * s, 4);
+        nn[s] = n[s];
+    }
+    swar_bounds4((const u8 *)lo32.data(), d, m, nn, b);
+    for (int s = 0; s < 4; s++) bound[s] = b[s];
+}
+
 /* Output conversion exactly as the kernels store it. */
 int blh_store(int d, int dtype)
 {

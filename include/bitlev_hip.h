@@ -140,7 +140,10 @@ int bl_long(int is_pair, const uint8_t *d_buf_a, const int64_t *d_off_a,
 typedef struct {
     int waves_per_cu;      /* persistent waves per compute unit (default 16) */
     int64_t tile;          /* choices per (query, block) tile (default 8192) */
-    int reserved[4];       /* [0]: 1 = strided lanes, 0 = queue refill (default) */
+    int reserved[4];       /* [0]: lane mode -- 0 compacted passes (default), 1 strided
+                              lanes, 2 four-choice byte filter for k <= 3 (queries of
+                              8 to 256 bytes; other queries run compacted);
+                              [1]: tiles per work-queue ticket (default 1) */
 } bl_cdist_params;
 
 typedef struct {
@@ -148,7 +151,8 @@ typedef struct {
     int block;             /* workgroup size: 64, 128 or 256 (default 256) */
     int bin_width;         /* text-length bucket width in bytes (default 16) */
     int reserved[4];       /* [0]: pairs per bucketing chunk (default 65536);
-                              [1]: 1 = direct mode (one launch, no bucketing) */
+                              [1]: 1 = direct mode (one launch, no bucketing);
+                              [2]: tiles per work-queue ticket (default 1) */
 } bl_pairs_params;
 
 /* Bytes of device workspace bl_cdist needs for nq queries of at most
@@ -179,6 +183,21 @@ int bl_cdist_tuned(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
                    const bl_stripes *choices, int max_qlen, void *d_out, int dtype, int k,
                    const int32_t *d_kvec, void *d_work, size_t work_bytes,
                    const bl_cdist_params *params, void *stream);
+
+/* Threshold match list instead of the dense matrix: every (query q, choice c)
+ * with d(q, c) <= k is appended as d_q[i] = q, d_c[i] = c, d_d[i] = d, in no
+ * particular order, for i < min(*d_count, capacity); *d_count receives the
+ * number of matches (more than capacity: call again with a larger buffer).
+ * k >= 0 (or a per-query d_kvec; queries with a negative threshold add
+ * nothing). Same launch as bl_cdist_tuned, with no nq x nc write: the four-
+ * choice filter and the band exits decide almost every cell without any
+ * store. d_count is device memory, cleared by the call. No counterpart in the
+ * reference; it names this output as its next step (README.md:319-322). */
+int bl_cdist_matches(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
+                     const bl_stripes *choices, int max_qlen, int k, const int32_t *d_kvec,
+                     int32_t *d_q, int64_t *d_c, int32_t *d_d, int64_t capacity,
+                     unsigned long long *d_count, void *d_work, size_t work_bytes,
+                     const bl_cdist_params *params, void *stream);
 
 /* Bytes of device workspace bl_pairs needs for n pairs whose strings are at
  * most max_len bytes (descriptors, bucket keys, sorted order, counters, and

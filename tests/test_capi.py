@@ -162,3 +162,8 @@ def test_stripes_layout_on_host():
     with pytest.raises(ValueError):
         _capi.call("bl_stripes_layout", bad.ctypes.data, 2, g.ctypes.data)
     assert _capi.call("bl_stripes_layout", off.ctypes.data, 0, g.ctypes.data) == 0
+
+
+def test_match_list_needs_a_threshold():
+    with pytest.raises(ValueError):
+        hip.cdist_matches(["abc"], ["abd"], None)

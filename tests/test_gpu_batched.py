@@ -47,14 +47,15 @@ def test_every_pairs_launch_shape_gives_the_same_results():
             np.testing.assert_array_equal(out.get(), want[k], err_msg=str(p))
 
 
-def test_every_cdist_launch_shape_gives_the_same_results():
-    pq, pc = W.cdist(4, 12, 5000, seed=3)
+@pytest.mark.parametrize("gen,k", [(4, 8), (2, 2), (2, 0)])
+def test_every_cdist_launch_shape_gives_the_same_results(gen, k):
+    pq, pc = W.cdist(gen, 12, 5000, seed=3)
     Q, C = hip.Packed(*pq), hip.Packed(*pc)
-    want = O.cross(pq, pc, k=8, threads=8)
-    space = {"mode": ["strided", "queue"], "waves_per_cu": [4, 32], "tile": [64, 1000, 8192]}
+    want = O.cross(pq, pc, k=k, threads=8)
+    space = {"mode": ["strided", "queue", "swar"], "waves_per_cu": [4, 32], "tile": [64, 1000, 8192]}
     for p in autotune._grid(space):
         out = hip.DeviceArray.empty((Q.n, C.n), np.uint8)
-        hip.cdist_batched_into(Q, C, out, 8, None, None, p)
+        hip.cdist_batched_into(Q, C, out, k, None, None, p)
         np.testing.assert_array_equal(out.get(), O.saturate(want, np.uint8), err_msg=str(p))
 
 
@@ -80,13 +81,61 @@ def test_cdist_stage_escalation_paths():
             Cs.append(bytes(rng.integers(0, 4, int(rng.integers(0, 260))).astype(np.uint8) + 97))
     pq, pc = O.pack(Qs), O.pack(Cs)
     Q, C = hip.Packed(*pq), hip.Packed(*pc)
-    for k in (0, 2, 8, 30, 60, 300):
+    for k in (0, 1, 2, 3, 8, 30, 60, 300):
         want = O.cross(pq, pc, k=k, threads=8)
         for p in ({"mode": "queue", "tile": 512}, {"mode": "queue", "tile": 8192},
-                  {"mode": "strided", "tile": 512}):
+                  {"mode": "strided", "tile": 512}, {"mode": "swar", "tile": 320},
+                  {"mode": "swar", "tile": 8192}):
             out = hip.DeviceArray.empty((Q.n, C.n), np.int32)
             hip.cdist_batched_into(Q, C, out, k, None, None, p)
             np.testing.assert_array_equal(out.get(), want, err_msg=f"k={k} {p}")
+
+
+def test_filter_mode_with_every_choice_open():
+    """The four-choice filter leaves every choice open when all are within k
+    of the query (here: copies with at most one substitution, k = 3): the
+    rescan stack fills 64 at a time in every window; mixed with per-query
+    thresholds that send some queries through the compacted mode (k > 3) and
+    queries too short for the filter."""
+    rng = np.random.default_rng(17)
+    Qs = [bytes(rng.integers(97, 123, m).astype(np.uint8)) for m in (64, 40, 8, 7, 100)]
+    Cs = []
+    for i in range(4000):
+        b = bytearray(Qs[0])
+        if i % 2:
+            b[int(rng.integers(0, 64))] = 97 + int(rng.integers(0, 26))
+        Cs.append(bytes(b[: 64 - (i % 3)]))
+    pq, pc = O.pack(Qs), O.pack(Cs)
+    Q, C = hip.Packed(*pq), hip.Packed(*pc)
+    exact = O.cross(pq, pc, threads=8)
+    for kq in ([3, 3, 3, 3, 3], [3, 9, 2, 1, 0], [0, 1, 2, 3, -1]):
+        kv = np.array(kq, np.int32)
+        want = np.where(kv[:, None] < 0, exact, np.minimum(exact, kv[:, None] + 1))
+        for p in ({"mode": "swar", "tile": 256}, {"mode": "swar", "tile": 8192},
+                  {"mode": "swar", "tile": 1024, "waves_per_cu": 1}):
+            out = hip.DeviceArray.empty((Q.n, C.n), np.int32)
+            hip.cdist_batched_into(Q, C, out, -1, None, None, p, kvec=hip.DeviceBuffer.from_host(kv))
+            np.testing.assert_array_equal(out.get(), want, err_msg=f"{kq} {p}")
+
+
+@pytest.mark.parametrize("mode", ["swar", "queue", "strided"])
+def test_match_list_equals_the_dense_result(mode):
+    """bl_cdist_matches returns exactly the cells the dense cdist puts at or
+    below k (here with per-query thresholds, one of them negative: that query
+    adds nothing), whatever the lane mode, and grows a too-small buffer."""
+    pq, pc = W.cdist(2, 16, 6000, seed=9)
+    Q, C = hip.Packed(*pq), hip.Packed(*pc)
+    exact = O.cross(pq, pc, threads=8)
+    kv = np.array([2, 3, 0, 1, 8, -1, 2, 2, 3, 4, 0, 1, 2, 3, 5, 40], np.int32)
+    for cap in (None, 1):
+        dq, dc, dd = hip.cdist_matches_into(Q, C, -1, params={"mode": mode},
+                                            kvec=hip.DeviceBuffer.from_host(kv), capacity=cap)
+        got = sorted(zip(dq.get().tolist(), dc.get().tolist(), dd.get().tolist()))
+        qi, ci = np.nonzero((exact <= kv[:, None]) & (kv[:, None] >= 0))
+        want = sorted(zip(qi.tolist(), ci.tolist(), exact[qi, ci].tolist()))
+        assert got == want, (mode, cap, len(got), len(want))
+    q, c, d = hip.cdist_matches([b"kitten", b"flaw"], [b"sitting", b"lawn", b"kitten"], 2)
+    assert (q.tolist(), c.tolist(), d.tolist()) == ([0, 1], [2, 1], [0, 2])
 
 
 def test_per_item_thresholds():

@@ -166,6 +166,55 @@ def test_row_stages_at_boundaries(hc):
                                       err_msg=f"cross k={k}")
 
 
+def _dp_cell(a: bytes, b: bytes, i: int, j: int) -> int:
+    """Wagner-Fischer D[i][j] of pattern a against text b."""
+    prev = list(range(i + 1))
+    for jj in range(1, j + 1):
+        cur = [jj] + [0] * i
+        for ii in range(1, i + 1):
+            cur[ii] = min(prev[ii] + 1, cur[ii - 1] + 1, prev[ii - 1] + (a[ii - 1] != b[jj - 1]))
+        prev = cur
+    return prev[i]
+
+
+def test_swar_filter_bounds_equal_the_dp(hc):
+    """The 4-choice filter's bound for each byte segment is exactly the DP cell
+    D[4 + m - n][4] (its row within 1..8), whatever the other three choices
+    are: no carry or shift leaks between segments."""
+    hc.blh_swar_bounds.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
+                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    rng = np.random.default_rng(44)
+    checked = 0
+    for trial in range(3000):
+        alpha = int(rng.choice([2, 3, 4, 26, 256]))
+        m = int(rng.integers(8, 70))
+        q = bytes(rng.integers(0, alpha, m).astype(np.uint8))
+        ns, texts = [], []
+        for s in range(4):
+            n = int(np.clip(m + int(rng.integers(-5, 6)), 4, None))
+            if rng.random() < 0.5:
+                t = bytearray(q[:n] + bytes(max(0, n - m)))
+                for _ in range(int(rng.integers(0, 4))):
+                    t[int(rng.integers(0, len(t)))] = int(rng.integers(0, alpha))
+                t = bytes(t[:n])
+            else:
+                t = bytes(rng.integers(0, alpha, n).astype(np.uint8))
+            ns.append(n)
+            texts.append(t)
+        packed = b"".join(t[:4] for t in texts)
+        nn = (ctypes.c_int * 4)(*ns)
+        out = (ctypes.c_int * 4)()
+        hc.blh_swar_bounds(q, m, packed, nn, out)
+        for s in range(4):
+            i0 = 4 + m - ns[s]
+            if 1 <= i0 <= 8:
+                assert out[s] == _dp_cell(q, texts[s], i0, 4), (trial, s, m, ns[s])
+                checked += 1
+            else:
+                assert out[s] == -1
+    assert checked > 5000
+
+
 def test_random_against_oracle(hc):
     rng = np.random.default_rng(21)
     A, B = [], []
