@@ -226,6 +226,10 @@ def main():
             "kernel_ms_min": float(np.min(per)),
             **extra,
         }
+    if wl["kind"] == "cdist" and k >= 0 and "batched" in measured:
+        measured_matches = matches_leg(hip, _capi, X, Y, k, tuned, ev, args)
+    else:
+        measured_matches = None
     # the faster form is what a caller of this build gets (both are reported)
     best = max(measured, key=lambda f: measured[f]["value"])
     main_m = measured[best]
@@ -253,6 +257,7 @@ def main():
         "plain": rounded(measured["plain"]) if "plain" in measured else None,
         "batched": rounded(measured["batched"]) if "batched" in measured else None,
         "batched_over_plain": round(speedup, 3) if speedup else None,
+        "matches": measured_matches,
         "device": info,
         "setup_s": round(setup_s, 2),
     }
@@ -387,6 +392,31 @@ def _serving_oracle(reqs, sizes, got, sets):
         ok = ok and bool(np.array_equal(O.saturate(want, np.dtype(r["dtype"])), got[i]))
         checked += 1
     return ok, checked
+
+
+def matches_leg(hip, capi, X, Y, k, params, ev, args):
+    """The same search with the threshold match-list output (bl_cdist_matches):
+    no dense nq x nc result, only the (q, c, d) of cells at or below k."""
+    dq, dc, dd = hip.cdist_matches_into(X, Y, k, None, None, params)
+    n = dq.shape[0]
+    cap = max(n, 1)
+    bq, bc, bd = (hip.DeviceArray.empty((cap,), t) for t in (np.int32, np.int64, np.int32))
+    count = hip.DeviceBuffer(8)
+    ws = hip.Workspace()
+    mq = min(X.max_len, hip.BATCHED_MAX_QUERY)
+    w = ws.get(capi.lib().bl_cdist_workspace_size(X.n, mq))
+    sp = ev.stream
+    pr = hip._cdist_params(params)
+
+    def step():
+        capi.call("bl_cdist_matches", X.d_buf.ptr, X.d_off.ptr, X.n, Y.stripes(sp).ref, mq, int(k),
+                  None, bq.ptr, bc.ptr, bd.ptr, cap, count.ptr, w.ptr, w.nbytes, pr, sp)
+
+    wall, per = timed_steps(step, ev, args.steps, args.warmup)
+    ms = wall * 1e3 / args.steps
+    return {"value": round(X.n * Y.n / (ms / 1e3) / 1e6, 3), "ms_per_step": round(ms, 4),
+            "kernel_ms_mean": round(float(np.mean(per)), 4), "matches": int(n),
+            "output_bytes": int(n * 16)}
 
 
 def rounded(d):

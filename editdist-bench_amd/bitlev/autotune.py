@@ -30,7 +30,7 @@ CDIST_SPACE = {"mode": ["strided", "queue"], "waves_per_cu": [8, 16, 32],
 # the four-choice byte filter, only where it can decide (k <= 3)
 CDIST_SWAR_SPACE = {"mode": ["swar"], "waves_per_cu": [8, 16, 32], "tile": [2048, 8192, 32768]}
 PAIRS_SPACE = {"waves_per_cu": [8, 16, 32], "block": [64, 256], "bin_width": [8, 16, 32],
-               "chunk": [16384, 65536], "trim": [True, False]}
+               "chunk": [16384, 65536, 262144], "trim": [True, False]}
 # direct mode (one launch, no bucketing): only the launch shape matters
 PAIRS_DIRECT_SPACE = {"mode": ["direct"], "waves_per_cu": [8, 16, 32], "block": [64, 256],
                       "trim": [True, False]}

@@ -17,8 +17,8 @@
  *      key from (pattern words, text length), privatised histogram in LDS;
  *   2. scan: bucket bases, most expensive bucket first (longest-job-first);
  *   3. scatter: pair indices into bucket order;
- *   4. persistent kernel: waves pull 64-pair tiles; a tile's pattern word count
- *      is uniform, so its lanes run the same register-resident code with
+ *   4. persistent kernel: waves pull 64-pair tiles; a tile's pattern size in
+ *      32-row units is uniform, so its lanes run the same register-resident code with
  *      text lengths within one bucket width. Long pairs (pattern > 256 bytes)
  *      take the wave-cooperative wavefront inside the same launch.
  * Reference: ed_pairs_c{CAP}, csrc/gpu_kernels.cu:6-32, bitlev/gpu.py:71-103.
@@ -732,7 +732,7 @@ __device__ void pairs_long_item(const u8 *__restrict__ ba, const int64_t *__rest
     if (lane == 0) store_out(out, i, r, dtype);
 }
 
-template <int NW>
+template <int W32>
 __device__ void pairs_tile(const u8 *__restrict__ ba, const int64_t *__restrict__ oa,
                            const u8 *__restrict__ bb, const int64_t *__restrict__ ob,
                            const int32_t *sorted, int64_t s0, int64_t s1, int trim, int k_all,
@@ -741,10 +741,12 @@ __device__ void pairs_tile(const u8 *__restrict__ ba, const int64_t *__restrict_
     const int64_t s = s0 + lane_id();
     if (s >= s1) return;
     const int64_t i = sorted[s];
-    store_out(out, i, packed_pair<NW>(ba, oa, bb, ob, i, trim, pair_k(k_all, kvec, i)), dtype);
+    store_out(out, i, packed_pair_w32<W32>(ba, oa, bb, ob, i, trim, pair_k(k_all, kvec, i)), dtype);
 }
 
-__global__ __launch_bounds__(256) void pairs_batched_kernel(
+/* (at most 128 VGPRs: four waves per SIMD; without the cap the scheduler
+ * interleaves the unrolled columns of the widest tiles into ~200 registers) */
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void pairs_batched_kernel(
     const u8 *__restrict__ ba, const int64_t *__restrict__ oa, const u8 *__restrict__ bb,
     const int64_t *__restrict__ ob, int trim, int k_all, const int32_t *__restrict__ kvec,
     const int32_t *__restrict__ sorted, const unsigned short *__restrict__ key_of,
@@ -782,12 +784,18 @@ __global__ __launch_bounds__(256) void pairs_batched_kernel(
         const int64_t s0 = cb + nlong + (lt - nlong) * 64;
         const int64_t ce = (int64_t)chunk_base[c + 1];
         const int64_t s1 = s0 + 64 < ce ? s0 + 64 : ce;
-        /* descending keys: the first pair of a tile has its largest word count */
-        const int w = uni(key_words(key_of[sorted[s0]]));
-        if (w <= 1) pairs_tile<1>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype);
-        else if (w == 2) pairs_tile<2>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype);
-        else if (w == 3) pairs_tile<3>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype);
-        else pairs_tile<4>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype);
+        /* descending keys: the first pair of a tile has its largest unit count */
+        const int w = uni(key_units(key_of[sorted[s0]]));
+        switch (w) {
+        case 1: pairs_tile<1>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype); break;
+        case 2: pairs_tile<2>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype); break;
+        case 3: pairs_tile<3>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype); break;
+        case 4: pairs_tile<4>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype); break;
+        case 5: pairs_tile<5>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype); break;
+        case 6: pairs_tile<6>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype); break;
+        case 7: pairs_tile<7>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype); break;
+        default: pairs_tile<8>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype); break;
+        }
     }
 }
 

@@ -268,8 +268,10 @@ BL_HD void word_step(T &Pv, T &Mv, T Eq, u32 hp, u32 hn, T &Ph, T &Mh)
     const T Xh = bop3<tt((TA ^ TB) | TC)>(S, Pv, E);            /* (S ^ Pv) | E   */
     Ph = bop3<tt(TA | ~(TB | TC))>(Mv, Xh, Pv);                 /* Mv | ~(Xh | Pv) */
     Mh = Pv & Xh;
-    const T Phs = (T)(Ph << 1) | (T)hp;
-    const T Mhs = (T)(Mh << 1) | (T)hn;
+    /* (the shifted-in bit 0 is free, so "+" equals "|"; on 64-bit blocks it
+     * maps to one v_lshl_add_u64) */
+    const T Phs = (T)(Ph << 1) + (T)hp;
+    const T Mhs = (T)(Mh << 1) + (T)hn;
     Pv = bop3<tt(TA | ~(TB | TC))>(Mhs, Xv, Phs);               /* Mhs | ~(Xv | Phs) */
     Mv = Phs & Xv;
 }
@@ -548,6 +550,126 @@ BL_HD int pair_lane(const u8 *a, int la, const u8 *b, int lb, int trim, int k)
     return scan_text<u64, NW, false>(la, nw, 64 * nw, text, lb, k, eq64);
 }
 
+/* ------------------------------------------------------------ 32-row units */
+/* The whole scan over NW64 64-row blocks plus, with TAIL, one 32-row block on
+ * top of them (rows 64*NW64 .. 64*NW64 + 31), so a pattern of 32*(2*NW64 + 1)
+ * rows costs no more than its 32-row units. eq64(c, w) and eq32(c) give the
+ * Eq blocks of text byte c. Results as scan_text with R >= m. */
+template <int NW64, bool TAIL, typename Text, typename Eq64, typename Eq32>
+BL_HD int scan_mixed(int m, const Text &text, int n, int k, Eq64 eq64, Eq32 eq32)
+{
+    static_assert(NW64 >= 1, "at least one 64-row block");
+    u64 Pv[NW64], Mv[NW64];
+#pragma unroll
+    for (int w = 0; w < NW64; w++) { Pv[w] = ~(u64)0; Mv[w] = 0; }
+    u32 Pt = ~0u, Mt = 0u;
+    auto col = [&](u32 c) {
+        u32 hp = 1, hn = 0;
+#pragma unroll
+        for (int w = 0; w < NW64; w++) {
+            u64 Ph, Mh;
+            word_step<u64>(Pv[w], Mv[w], eq64(c, w), hp, hn, Ph, Mh);
+            hp = (u32)(Ph >> 63);
+            hn = (u32)(Mh >> 63);
+        }
+        if (TAIL) {
+            u32 Ph, Mh;
+            word_step<u32>(Pt, Mt, eq32(c), hp, hn, Ph, Mh);
+        }
+    };
+    auto bound = [&](int jj) {
+        const int i0 = jj + m - n;
+        if (i0 <= 0) return n - m;
+        int s = jj;
+#pragma unroll
+        for (int w = 0; w < NW64; w++) {
+            const u64 mk = rows_below<u64>(i0 - 64 * w);
+            s += popc((u64)(Pv[w] & mk)) - popc((u64)(Mv[w] & mk));
+        }
+        if (TAIL) {
+            const u32 mk = rows_below<u32>(i0 - 64 * NW64);
+            s += popc((u32)(Pt & mk)) - popc((u32)(Mt & mk));
+        }
+        return s;
+    };
+    auto ts = text.open(n);
+    u32 c4 = ts.next4();
+    int j = 0;
+    for (;;) {
+        const u32 c4n = ts.next4();
+        const int cnt = n - j < 4 ? n - j : 4;
+        if (cnt == 4) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) col((c4 >> (8 * u)) & 0xFFu);
+        } else {
+#pragma unroll
+            for (int u = 0; u < 3; u++)
+                if (u < cnt) col((c4 >> (8 * u)) & 0xFFu);
+        }
+        j += cnt;
+        if (j >= n) {
+            const int d = bound(j);                     /* exact at j = n */
+            return (k >= 0 && d > k) ? k + 1 : d;
+        }
+        if (k >= 0 && bound(j) > k) return k + 1;
+        c4 = c4n;
+    }
+}
+
+/* pair_lane for a shorter string (before trimming) of at most 32*W32 bytes,
+ * scanned in 32-row units: the same stages, then the whole scan over W32/2
+ * 64-row blocks and, for odd W32, a 32-row tail block. */
+template <int W32>
+BL_HD int pair_lane_w32(const u8 *a, int la, const u8 *b, int lb, int trim, int k)
+{
+    constexpr int NW64 = W32 / 2 > 0 ? W32 / 2 : 1;
+    constexpr bool TAIL = (W32 & 1) && W32 > 1;
+    constexpr int NPL = NW64 + (TAIL ? 1 : 0);
+    if (trim) trim_pair(a, la, b, lb);
+    if (la > lb) {
+        const u8 *t = a; a = b; b = t;
+        const int x = la; la = lb; lb = x;
+    }
+    if (k >= 0 && lb - la > k) return k + 1;
+    if (la == 0) return lb;
+
+    const ByteText text{b};
+    u64 pl[NPL][8];
+    build_planes(a, la, pl[0]);
+    auto eq32 = [&](u32 c, int) {
+        u32 m8[8];
+        text_masks(c, m8);
+        return eq_planes_lo(pl[0], m8);
+    };
+    if (W32 == 1 || la <= 32) return scan_text<u32, 1, false>(la, 1, 32, text, lb, k, eq32);
+    if (k >= 0) {
+        const int r = scan_text<u32, 1, false>(la, 1, 32, text, lb, k, eq32);
+        if (r >= 0) return r;
+    }
+    auto eq64 = [&](u32 c, int w) {
+        u32 m8[8];
+        text_masks(c, m8);
+        return eq_planes(pl[w], m8);
+    };
+    if (W32 <= 2 || la <= 64) return scan_text<u64, 1, false>(la, 1, 64, text, lb, k, eq64);
+    if (k >= 0) {
+        const int r = scan_text<u64, 1, false>(la, 1, 64, text, lb, k, eq64);
+        if (r >= 0) return r;
+    }
+#pragma unroll
+    for (int w = 1; w < NPL; w++) {
+#pragma unroll
+        for (int b8 = 0; b8 < 8; b8++) pl[w][b8] = 0;
+        if (64 * w < la) build_planes(a + 64 * w, la - 64 * w, pl[w]);
+    }
+    auto eqt = [&](u32 c) {
+        u32 m8[8];
+        text_masks(c, m8);
+        return eq_planes_lo(pl[NPL - 1], m8);
+    };
+    return scan_mixed<NW64, TAIL>(la, text, lb, k, eq64, eqt);
+}
+
 /* ------------------------------------------------------------ Peq lane */
 /* Distance of a pattern given by its Peq table (row c: NW words, peq[c*NW + w])
  * of length 1 <= m <= 64*NW against a text of n bytes (a ByteText or a
@@ -719,11 +841,12 @@ BL_HD int long_prepare(const u8 *&p, int &m, const u8 *&t, int &n, int is_pair, 
  * the same amount, so the pattern words of the untrimmed shorter string bound
  * the trimmed ones and the length gap is unchanged):
  * key = 1 + (w - 1) * LB_BINS + min(max(la, lb) / bin_width, LB_BINS - 1) for a
- * shorter string of w <= 4 words, PKEY_LONG for longer ones, 0 for a pair the
+ * shorter string of w <= 8 units of 32 bytes, PKEY_LONG for longer ones
+ * (above 256 bytes), 0 for a pair the
  * gap or an empty string decides. A larger key means more work; the pipeline
  * schedules large keys first. */
 constexpr int LB_BINS = 64;
-constexpr int PKEY_LONG = 1 + 4 * LB_BINS;
+constexpr int PKEY_LONG = 1 + 8 * LB_BINS;
 constexpr int PKEYS = PKEY_LONG + 1;
 
 /* Key of a pair of lengths la, lb; for key 0, *result holds the answer
@@ -733,14 +856,14 @@ BL_HD int pair_key(int la, int lb, int k, int bin_width, int &result)
     const int mn = la < lb ? la : lb, mx = la < lb ? lb : la;
     if (k >= 0 && mx - mn > k) { result = k + 1; return 0; }
     if (mn == 0) { result = mx; return 0; }
-    const int w = (mn + 63) >> 6;
-    if (w > 4) return PKEY_LONG;
+    const int w = (mn + 31) >> 5;                 /* 32-row units of the pattern */
+    if (w > 8) return PKEY_LONG;
     const int bin = mx / bin_width;
     return 1 + (w - 1) * LB_BINS + (bin < LB_BINS - 1 ? bin : LB_BINS - 1);
 }
 
-/* Pattern words of the tile whose largest key is `key`. */
-BL_HD int key_words(int key) { return (key - 1) / LB_BINS + 1; }
+/* 32-row units of the tile whose largest key is `key`. */
+BL_HD int key_units(int key) { return (key - 1) / LB_BINS + 1; }
 
 /* Pair i of packed sets through the register-resident lane code (shorter
  * string at most 64*NW bytes before trimming). */
@@ -750,6 +873,16 @@ BL_HD int packed_pair(const u8 *ba, const int64_t *oa, const u8 *bb, const int64
 {
     const int64_t a0 = oa[i], b0 = ob[i];
     return pair_lane<NW>(ba + a0, (int)(oa[i + 1] - a0), bb + b0, (int)(ob[i + 1] - b0), trim, k);
+}
+
+/* The same in 32-row units (shorter string at most 32*W32 bytes before trimming). */
+template <int W32>
+BL_HD int packed_pair_w32(const u8 *ba, const int64_t *oa, const u8 *bb, const int64_t *ob,
+                          int64_t i, int trim, int k)
+{
+    const int64_t a0 = oa[i], b0 = ob[i];
+    return pair_lane_w32<W32>(ba + a0, (int)(oa[i + 1] - a0), bb + b0, (int)(ob[i + 1] - b0), trim,
+                              k);
 }
 
 }  // namespace bl

@@ -106,6 +106,23 @@ int blh_pair(const u8 *a, int la, const u8 *b, int lb, int trim, int k)
     return pair_host(a, la, b, lb, trim, k);
 }
 
+/* One pair through the 32-row-unit lane code of the batched tiles; w32 must
+ * cover the shorter untrimmed string (1..8). */
+int blh_pair_w32(const u8 *a, int la, const u8 *b, int lb, int trim, int k, int w32)
+{
+    switch (w32) {
+    case 1: return pair_lane_w32<1>(a, la, b, lb, trim, k);
+    case 2: return pair_lane_w32<2>(a, la, b, lb, trim, k);
+    case 3: return pair_lane_w32<3>(a, la, b, lb, trim, k);
+    case 4: return pair_lane_w32<4>(a, la, b, lb, trim, k);
+    case 5: return pair_lane_w32<5>(a, la, b, lb, trim, k);
+    case 6: return pair_lane_w32<6>(a, la, b, lb, trim, k);
+    case 7: return pair_lane_w32<7>(a, la, b, lb, trim, k);
+    case 8: return pair_lane_w32<8>(a, la, b, lb, trim, k);
+    default: return -1;
+    }
+}
+
 /* One (query, choice) through the Peq lane the cdist kernels run. */
 int blh_peq(const u8 *q, int m, const u8 *t, int n, int k)
 {
@@ -152,7 +169,7 @@ int blh_cross(const u8 *bq, const int64_t *oq, int64_t nq, const u8 *bc, const i
  * 256-pair block ranking its pairs per key and reserving the key's range with
  * one add -> tiles taken in order, the chunk of each tile found by the same
  * bisection. Every index is checked, and so is the tile invariant (no pair in
- * a tile has a shorter string of more words than its first pair). chunk:
+ * a tile has a shorter string of more 32-row units than its first pair). chunk:
  * pairs per chunk (a multiple of 256 on the GPU; any multiple of 256 here).
  * Returns 0, or a negative code naming the violated check. */
 int blh_pairs_batched(const u8 *ba, const int64_t *oa, const u8 *bb, const int64_t *ob, int64_t n,
@@ -242,16 +259,20 @@ int blh_pairs_batched(const u8 *ba, const int64_t *oa, const u8 *bb, const int64
         const int64_t ce = (int64_t)cbase[(size_t)c + 1];
         const int64_t s1 = s0 + 64 < ce ? s0 + 64 : ce;
         if (s0 >= ce) return -14;
-        const int w = key_words(key[(size_t)sorted[(size_t)s0]]);
-        if (w < 1 || w > 4) return -15;
+        const int w = key_units(key[(size_t)sorted[(size_t)s0]]);
+        if (w < 1 || w > 8) return -15;
         for (int64_t q = s0; q < s1; q++) {
             const int64_t i = sorted[(size_t)q];
-            if (((minlen(i) + 63) >> 6) > w) return -16;
+            if (((minlen(i) + 31) >> 5) > w) return -16;
             switch (w) {
-            case 1: out[i] = packed_pair<1>(ba, oa, bb, ob, i, trim, k); break;
-            case 2: out[i] = packed_pair<2>(ba, oa, bb, ob, i, trim, k); break;
-            case 3: out[i] = packed_pair<3>(ba, oa, bb, ob, i, trim, k); break;
-            default: out[i] = packed_pair<4>(ba, oa, bb, ob, i, trim, k); break;
+            case 1: out[i] = packed_pair_w32<1>(ba, oa, bb, ob, i, trim, k); break;
+            case 2: out[i] = packed_pair_w32<2>(ba, oa, bb, ob, i, trim, k); break;
+            case 3: out[i] = packed_pair_w32<3>(ba, oa, bb, ob, i, trim, k); break;
+            case 4: out[i] = packed_pair_w32<4>(ba, oa, bb, ob, i, trim, k); break;
+            case 5: out[i] = packed_pair_w32<5>(ba, oa, bb, ob, i, trim, k); break;
+            case 6: out[i] = packed_pair_w32<6>(ba, oa, bb, ob, i, trim, k); break;
+            case 7: out[i] = packed_pair_w32<7>(ba, oa, bb, ob, i, trim, k); break;
+            default: out[i] = packed_pair_w32<8>(ba, oa, bb, ob, i, trim, k); break;
             }
         }
     }

@@ -166,6 +166,36 @@ def test_row_stages_at_boundaries(hc):
                                       err_msg=f"cross k={k}")
 
 
+def test_32_row_unit_lanes(hc):
+    """The batched tiles' lane code in 32-row units (u64 blocks plus a u32 tail
+    block for an odd unit count) equals the oracle for every unit count that
+    covers the shorter string, exact and thresholded, trimmed or not."""
+    hc.blh_pair_w32.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    rng = np.random.default_rng(55)
+    cases = []
+    for mn in (1, 5, 31, 32, 33, 63, 64, 65, 95, 96, 97, 127, 128, 129, 160, 161, 192, 193, 224,
+               225, 255, 256):
+        for alpha in (4, 256):
+            a = bytes(rng.integers(0, alpha, mn).astype(np.uint8))
+            b = bytearray(a)
+            for _ in range(int(rng.integers(0, 12))):
+                b[int(rng.integers(0, len(b)))] = int(rng.integers(0, alpha))
+            b = bytes(b) + bytes(rng.integers(0, alpha, int(rng.integers(0, 9))).astype(np.uint8))
+            cases.append((a, b))
+            cases.append((b, a))
+            cases.append((a, bytes(rng.integers(0, alpha, mn + 3).astype(np.uint8))))
+    pa, pb = O.pack([c[0] for c in cases]), O.pack([c[1] for c in cases])
+    for k in (-1, 0, 3, 20, 100):
+        for trim in (0, 1):
+            want = O.pairs(pa, pb, k=k, trim=bool(trim))
+            for i, (a, b) in enumerate(cases):
+                need = (min(len(a), len(b)) + 31) // 32
+                for w in sorted({max(need, 1), 8}):
+                    got = hc.blh_pair_w32(a, len(a), b, len(b), trim, k, w)
+                    assert got == want[i], (i, len(a), len(b), k, trim, w)
+
+
 def _dp_cell(a: bytes, b: bytes, i: int, j: int) -> int:
     """Wagner-Fischer D[i][j] of pattern a against text b."""
     prev = list(range(i + 1))
