@@ -1,8 +1,10 @@
 """Autotuner for the batched persistent kernels.
 
 For a resident workload it times every candidate launch shape of the batched
-pipeline on a sample of the batch (HIP events, median of a few runs), keeps
-the fastest and caches it per device and workload signature in a JSON file
+pipeline on a sample of the batch (HIP events, median of a few runs), times
+the few fastest again on the whole batch (a sample's length mix and its
+tail of short tiles can rank close shapes differently), keeps the fastest and
+caches it per device and workload signature in a JSON file
 ($BITLEV_TUNE_CACHE, default ~/.cache/bitlev/autotune.json).
 
 cdist: lane mode (compacted lockstep passes, strided lanes, or for k <= 3 the
@@ -102,6 +104,18 @@ def _time(fn, stream, reps=3):
         call("bl_event_destroy", e1)
 
 
+def _refine(res, full_fn, stream, refine, log, tag):
+    """Re-time the `refine` fastest sampled candidates on the whole batch."""
+    top = sorted(res, key=lambda r: r[0])[:max(1, refine)]
+    out = []
+    for _, p in top:
+        ms = _time(lambda: full_fn(p), stream)
+        out.append((ms, p))
+        if log:
+            log(f"{tag} full {p} {ms:.3f} ms")
+    return min(out, key=lambda r: r[0])
+
+
 def _length_sig(packed: hip.Packed):
     L = packed.lengths
     if len(L) == 0:
@@ -118,7 +132,8 @@ def pairs_signature(A: hip.Packed, B: hip.Packed, k: int, dtype) -> str:
 
 
 def tune_cdist(Q: hip.Packed, C: hip.Packed, out: hip.DeviceArray, k: int = -1, stream=None,
-               sample_queries: int = 256, space=None, use_cache: bool = True, log=None):
+               sample_queries: int = 256, space=None, use_cache: bool = True, log=None,
+               refine: int = 3):
     """Best bl_cdist launch shape for these resident inputs (dict of params)."""
     sig = cdist_signature(Q, C, k, out.dtype)
     if use_cache:
@@ -137,14 +152,18 @@ def tune_cdist(Q: hip.Packed, C: hip.Packed, out: hip.DeviceArray, k: int = -1, 
         res.append((ms, p))
         if log:
             log(f"cdist {p} {ms:.3f} ms")
-    ms, best = min(res, key=lambda r: r[0])
-    _store(sig, {"params": best, "ms_sample": ms, "sample_queries": nq})
+    if nq < Q.n and refine > 0:
+        ms, best = _refine(res, lambda p: hip.cdist_batched_into(Q, C, out, k, stream, ws, p),
+                           stream, refine, log, "cdist")
+    else:
+        ms, best = min(res, key=lambda r: r[0])
+    _store(sig, {"params": best, "ms": ms, "sample_queries": nq})
     return best
 
 
 def tune_pairs(A: hip.Packed, B: hip.Packed, out: hip.DeviceArray, k: int = -1, trim=True,
                stream=None, sample_pairs: int = 1 << 20, space=None, use_cache: bool = True,
-               log=None):
+               log=None, refine: int = 3):
     """Best bl_pairs launch shape for these resident inputs (dict of params)."""
     sig = pairs_signature(A, B, k, out.dtype)
     if use_cache:
@@ -157,15 +176,19 @@ def tune_pairs(A: hip.Packed, B: hip.Packed, out: hip.DeviceArray, k: int = -1, 
     ws = hip.Workspace()
     res = []
     cands = list(_grid(space)) if space else list(_grid(PAIRS_DIRECT_SPACE)) + list(_grid(PAIRS_SPACE))
-    if not trim:                       # the caller asked for no trimming: keep it off
+    if not trim:                       # trimming is off anyway: drop the duplicate shapes
         cands = [p for p in cands if p.get("trim", True)]
     for p in cands:
         ms = _time(lambda: hip.pairs_batched_into(As, Bs, out, k, trim, stream, ws, p), stream)
         res.append((ms, p))
         if log:
             log(f"pairs {p} {ms:.3f} ms")
-    ms, best = min(res, key=lambda r: r[0])
-    _store(sig, {"params": best, "ms_sample": ms, "sample_pairs": n})
+    if n < A.n and refine > 0:
+        ms, best = _refine(res, lambda p: hip.pairs_batched_into(A, B, out, k, trim, stream, ws, p),
+                           stream, refine, log, "pairs")
+    else:
+        ms, best = min(res, key=lambda r: r[0])
+    _store(sig, {"params": best, "ms": ms, "sample_pairs": n})
     return best
 
 

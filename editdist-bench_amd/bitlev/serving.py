@@ -120,7 +120,7 @@ class Server:
         self._in_use = 0
         self._stop = False
         self.stats = collections.Counter()
-        self._exec = executor or _GpuExecutor()
+        self._exec = executor or _GpuExecutor(slots)
         self._slots = [self._exec.new_slot(i) for i in range(slots)]
         self._threads = [threading.Thread(target=self._schedule, name="bitlev-sched", daemon=True)]
         for t in self._threads:
@@ -305,7 +305,17 @@ class Server:
 
 
 class _GpuExecutor:
-    """Runs a fused batch on a slot's stream with the batched kernels."""
+    """Runs a fused batch on a slot's stream with the batched kernels. Each
+    launch takes 1/slots of the GPU's wave slots, so batches of different
+    slots run side by side instead of a long one holding every CU while the
+    others wait; searches use the four-choice filter where a query's
+    max_dist allows it (the launch falls back per query otherwise)."""
+
+    def __init__(self, slots: int = 4):
+        wpc = max(4, 32 // max(1, int(slots)))
+        self.cdist_params = {"mode": "swar", "waves_per_cu": wpc, "tile": 8192}
+        self.pairs_params = {"waves_per_cu": wpc, "block": 256, "bin_width": 16, "chunk": 65536,
+                             "trim": True}
 
     def new_slot(self, i):
         return _Slot(i)
@@ -340,6 +350,7 @@ class _GpuExecutor:
             kvec = DeviceBuffer.from_host(kv, slot.stream)
             out = DeviceArray.empty((A.n,), dt)
             hip.pairs_batched_into(A, B, out, -1, batch[0].trim, slot.stream, slot.workspace,
+                                   params=self.pairs_params,
                                    kvec=kvec)
             offs = np.cumsum([0] + [r.pairs for r in batch])
             views = [DeviceArray((r.pairs,), dt, out._buf, out._off + int(o) * dt.itemsize)
@@ -352,7 +363,8 @@ class _GpuExecutor:
             Q = hip.Packed(*pq, stream=slot.stream)
             kvec = DeviceBuffer.from_host(kv, slot.stream)
             out = DeviceArray.empty((Q.n, C.n), dt)
-            hip.cdist_batched_into(Q, C, out, -1, slot.stream, slot.workspace, kvec=kvec)
+            hip.cdist_batched_into(Q, C, out, -1, slot.stream, slot.workspace, self.cdist_params,
+                                   kvec=kvec)
             offs = np.cumsum([0] + nqs)
             views = [DeviceArray((q, C.n), dt, out._buf, out._off + int(o) * C.n * dt.itemsize)
                      for q, o in zip(nqs, offs)]
