@@ -10,9 +10,10 @@ caches it per device and workload signature in a JSON file
 cdist: lane mode (compacted lockstep passes, strided lanes, or for k <= 3 the
 four-choice byte filter), persistent waves per CU, choices per tile. pairs: direct (one launch) or bucketed, and
 for bucketed persistent waves per CU, workgroup size, text-length bucket
-width, bucketing chunk; and whether to trim common affixes (which never
-changes a distance). Every candidate computes the same results; only speed
-differs.
+width, bucketing chunk; whether to trim common affixes (which never changes a
+distance); then, on the two best bucketed shapes, the smallest tile (in
+32-row units) that runs as a diagonal-band tile. Every candidate computes the
+same results; only speed differs.
 """
 from __future__ import annotations
 
@@ -36,6 +37,8 @@ PAIRS_SPACE = {"waves_per_cu": [8, 16, 32], "block": [64, 256], "bin_width": [8,
 # direct mode (one launch, no bucketing): only the launch shape matters
 PAIRS_DIRECT_SPACE = {"mode": ["direct"], "waves_per_cu": [8, 16, 32], "block": [64, 256],
                       "trim": [True, False]}
+# band tiles from this many 32-row units up (tried on the best bucketed shapes)
+PAIRS_BAND_UNITS = [2, 3, 4, 5]
 
 _lock = threading.Lock()
 
@@ -104,15 +107,26 @@ def _time(fn, stream, reps=3):
         call("bl_event_destroy", e1)
 
 
-def _refine(res, full_fn, stream, refine, log, tag):
-    """Re-time the `refine` fastest sampled candidates on the whole batch."""
+def _refine(res, full_fn, stream, refine, log, tag, wpc_values):
+    """Re-time the `refine` fastest sampled candidates on the whole batch, then
+    the winner at its other waves-per-CU values (a sample holds too few tiles
+    per wave to rank occupancy the way the whole batch does)."""
     top = sorted(res, key=lambda r: r[0])[:max(1, refine)]
     out = []
-    for _, p in top:
+
+    def run(p):
         ms = _time(lambda: full_fn(p), stream)
         out.append((ms, p))
         if log:
             log(f"{tag} full {p} {ms:.3f} ms")
+
+    for _, p in top:
+        run(p)
+    best = min(out, key=lambda r: r[0])[1]
+    for w in wpc_values:
+        q = dict(best, waves_per_cu=w)
+        if all(q != p for _, p in out):
+            run(q)
     return min(out, key=lambda r: r[0])
 
 
@@ -154,7 +168,7 @@ def tune_cdist(Q: hip.Packed, C: hip.Packed, out: hip.DeviceArray, k: int = -1, 
             log(f"cdist {p} {ms:.3f} ms")
     if nq < Q.n and refine > 0:
         ms, best = _refine(res, lambda p: hip.cdist_batched_into(Q, C, out, k, stream, ws, p),
-                           stream, refine, log, "cdist")
+                           stream, refine, log, "cdist", CDIST_SPACE["waves_per_cu"])
     else:
         ms, best = min(res, key=lambda r: r[0])
     _store(sig, {"params": best, "ms": ms, "sample_queries": nq})
@@ -178,14 +192,22 @@ def tune_pairs(A: hip.Packed, B: hip.Packed, out: hip.DeviceArray, k: int = -1, 
     cands = list(_grid(space)) if space else list(_grid(PAIRS_DIRECT_SPACE)) + list(_grid(PAIRS_SPACE))
     if not trim:                       # trimming is off anyway: drop the duplicate shapes
         cands = [p for p in cands if p.get("trim", True)]
-    for p in cands:
+    def run(p):
         ms = _time(lambda: hip.pairs_batched_into(As, Bs, out, k, trim, stream, ws, p), stream)
         res.append((ms, p))
         if log:
             log(f"pairs {p} {ms:.3f} ms")
+
+    for p in cands:
+        run(p)
+    if space is None:
+        bucketed = sorted((r for r in res if r[1].get("mode") != "direct"), key=lambda r: r[0])
+        for _, p in bucketed[:2]:
+            for b in PAIRS_BAND_UNITS:
+                run(dict(p, band=b))
     if n < A.n and refine > 0:
         ms, best = _refine(res, lambda p: hip.pairs_batched_into(A, B, out, k, trim, stream, ws, p),
-                           stream, refine, log, "pairs")
+                           stream, refine, log, "pairs", PAIRS_SPACE["waves_per_cu"])
     else:
         ms, best = min(res, key=lambda r: r[0])
     _store(sig, {"params": best, "ms": ms, "sample_pairs": n})

@@ -260,6 +260,7 @@ def _pairs_params(params):
     p.reserved[0] = int(params.get("chunk", 0))
     p.reserved[1] = 1 if params.get("mode") == "direct" else 0
     p.reserved[2] = int(params.get("grab", 0))
+    p.reserved[3] = int(params.get("band", 0))
     return ctypes.byref(p)
 
 

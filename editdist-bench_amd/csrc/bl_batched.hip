@@ -744,6 +744,100 @@ __device__ void pairs_tile(const u8 *__restrict__ ba, const int64_t *__restrict_
     store_out(out, i, packed_pair_w32<W32>(ba, oa, bb, ob, i, trim, pair_k(k_all, kvec, i)), dtype);
 }
 
+__device__ __forceinline__ int wave_min(int v)
+{
+    for (int o = 32; o; o >>= 1) {
+        const int x = __shfl_xor(v, o);
+        v = x < v ? x : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int wave_max(int v)
+{
+    for (int o = 32; o; o >>= 1) {
+        const int x = __shfl_xor(v, o);
+        v = x > v ? x : v;
+    }
+    return v;
+}
+
+/* Pair indices a wave's band tiles could not decide, scanned in full 64 at a
+ * time by the same wave (its LDS stack), so those scans overlap the other
+ * waves' band tiles. */
+constexpr int PB_ESC = 128;
+
+/* A band tile: every lane scans its pair in a diagonal window at one offset
+ * for the whole tile, chosen from the lanes' length differences (band_offset);
+ * 32 rows when every threshold is at most 14, else 64. A lane whose distance
+ * the window does not reach pushes its pair onto the wave's stack. */
+__device__ void pairs_tile_band(const u8 *__restrict__ ba, const int64_t *__restrict__ oa,
+                                const u8 *__restrict__ bb, const int64_t *__restrict__ ob,
+                                const int32_t *sorted, int64_t s0, int64_t s1, int trim, int k_all,
+                                const int32_t *__restrict__ kvec, void *out, int dtype, int32_t *esc,
+                                int &nesc)
+{
+    const int64_t s = s0 + lane_id();
+    const bool act = s < s1;
+    int64_t i = 0;
+    int k = 0, res = -1, la = 0, lb = 0;
+    const u8 *a = ba, *b = bb;
+    if (act) {
+        i = sorted[s];
+        k = pair_k(k_all, kvec, i);
+        const int64_t a0 = oa[i], b0 = ob[i];
+        a = ba + a0;
+        b = bb + b0;
+        la = (int)(oa[i + 1] - a0);
+        lb = (int)(ob[i + 1] - b0);
+        res = band_prepare(a, la, b, lb, trim, k);
+    }
+    const bool need = act && res < 0;
+    const int dmin = uni(wave_min(need ? la - lb : 0x7fffffff));
+    const int dmax = uni(wave_max(need ? la - lb : -0x7fffffff));
+    const bool narrow = wave_ballot(need && !(k >= 0 && k <= 14)) == 0;
+    if (need) {
+        if (narrow) res = band_pair<u32>(a, la, b, lb, k, band_offset<32>(dmin, dmax));
+        else res = band_pair<u64>(a, la, b, lb, k, band_offset<64>(dmin, dmax));
+    }
+    if (act && res >= 0) store_out(out, i, res, dtype);
+    const u64 em = wave_ballot(act && res < 0);
+    if (em) {
+        if (act && res < 0) esc[nesc + rank_below(em)] = (int32_t)i;
+        nesc += (int)__popcll(em);
+        wave_lds_sync();
+    }
+}
+
+/* The top cnt (<= 64) pairs of the wave's stack through the whole scan, in the
+ * 32-row units of the widest of them. */
+__device__ void pairs_esc_flush(const u8 *__restrict__ ba, const int64_t *__restrict__ oa,
+                                const u8 *__restrict__ bb, const int64_t *__restrict__ ob,
+                                int trim, int k_all, const int32_t *__restrict__ kvec, void *out,
+                                int dtype, const int32_t *esc, int &nesc, int cnt)
+{
+    const int base = nesc - cnt;
+    int w = 0;
+    if (lane_id() < cnt) {
+        const int64_t i = esc[base + lane_id()];
+        const int la = (int)(oa[i + 1] - oa[i]), lb = (int)(ob[i + 1] - ob[i]);
+        w = ((la < lb ? la : lb) + 31) >> 5;
+    }
+    w = uni(wave_max(w));
+    switch (w) {
+    case 1: pairs_tile<1>(ba, oa, bb, ob, esc, base, base + cnt, trim, k_all, kvec, out, dtype); break;
+    case 2: pairs_tile<2>(ba, oa, bb, ob, esc, base, base + cnt, trim, k_all, kvec, out, dtype); break;
+    case 3: pairs_tile<3>(ba, oa, bb, ob, esc, base, base + cnt, trim, k_all, kvec, out, dtype); break;
+    case 4: pairs_tile<4>(ba, oa, bb, ob, esc, base, base + cnt, trim, k_all, kvec, out, dtype); break;
+    case 5: pairs_tile<5>(ba, oa, bb, ob, esc, base, base + cnt, trim, k_all, kvec, out, dtype); break;
+    case 6: pairs_tile<6>(ba, oa, bb, ob, esc, base, base + cnt, trim, k_all, kvec, out, dtype); break;
+    case 7: pairs_tile<7>(ba, oa, bb, ob, esc, base, base + cnt, trim, k_all, kvec, out, dtype); break;
+    default: pairs_tile<8>(ba, oa, bb, ob, esc, base, base + cnt, trim, k_all, kvec, out, dtype); break;
+    }
+    nesc = base;
+    wave_lds_sync();                             /* reads done before the next pushes */
+}
+
 /* (at most 128 VGPRs: four waves per SIMD; without the cap the scheduler
  * interleaves the unrolled columns of the widest tiles into ~200 registers) */
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void pairs_batched_kernel(
@@ -752,8 +846,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void p
     const int32_t *__restrict__ sorted, const unsigned short *__restrict__ key_of,
     const unsigned long long *__restrict__ hist, const unsigned long long *__restrict__ chunk_base,
     const unsigned long long *__restrict__ tile_base, int64_t nchunks, void *out, int dtype,
-    int grab, unsigned long long *counter, int8_t *scratch, int64_t slot)
+    int grab, unsigned long long *counter, int8_t *scratch, int64_t slot, int band)
 {
+    __shared__ int32_t esc_all[4][PB_ESC];
+    int32_t *esc = esc_all[threadIdx.x >> 6];
+    int nesc = 0;
     const int64_t ntiles = (int64_t)tile_base[nchunks];
     const int64_t wave_gid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     int8_t *hbuf = scratch ? scratch + wave_gid * slot : nullptr;
@@ -786,6 +883,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void p
         const int64_t s1 = s0 + 64 < ce ? s0 + 64 : ce;
         /* descending keys: the first pair of a tile has its largest unit count */
         const int w = uni(key_units(key_of[sorted[s0]]));
+        if (band > 0 && w >= band) {
+            pairs_tile_band(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype, esc, nesc);
+            if (nesc >= 64)
+                pairs_esc_flush(ba, oa, bb, ob, trim, k_all, kvec, out, dtype, esc, nesc, 64);
+            continue;
+        }
         switch (w) {
         case 1: pairs_tile<1>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype); break;
         case 2: pairs_tile<2>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype); break;
@@ -797,6 +900,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void p
         default: pairs_tile<8>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype); break;
         }
     }
+    while (nesc > 0)
+        pairs_esc_flush(ba, oa, bb, ob, trim, k_all, kvec, out, dtype, esc, nesc, nesc < 64 ? nesc : 64);
 }
 
 /* Direct mode: one launch, no bucketing. Waves take 64 consecutive pairs; each
@@ -1067,6 +1172,7 @@ int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t
     const int wg = prm && (prm->block == 64 || prm->block == 128 || prm->block == 256) ? prm->block : 256;
     const bool direct = prm && prm->reserved[1] == 1;
     const int grab = prm && prm->reserved[2] > 0 ? prm->reserved[2] : 1;
+    const int band = prm && prm->reserved[3] > 0 ? prm->reserved[3] : 0;
 
     char *w = (char *)d_work;
     if (!w) BL_HIP(hipMallocAsync((void **)&w, L.total, s));
@@ -1114,7 +1220,7 @@ int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t
             pairs_scatter_kernel<<<cblocks, 256, 0, s>>>(key, n, L.chunk, cursor, chunk_base, sorted);
             pairs_batched_kernel<<<(unsigned)blocks, wg, 0, s>>>(
                 d_buf_a, d_off_a, d_buf_b, d_off_b, trim, k, d_kvec, sorted, key, hist, chunk_base,
-                tile_base, L.nchunks, d_out, dtype, grab, ctr, scratch, slot);
+                tile_base, L.nchunks, d_out, dtype, grab, ctr, scratch, slot, band);
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) rc = fail(BL_EHIP, "pairs pipeline launch: %s", hipGetErrorString(e));
         }

@@ -670,6 +670,250 @@ BL_HD int pair_lane_w32(const u8 *a, int la, const u8 *b, int lb, int trim, int 
     return scan_mixed<NW64, TAIL>(la, text, lb, k, eq64, eqt);
 }
 
+/* ------------------------------------------------------------ diagonal band */
+/* An alignment of cost <= k passes only through cells (i, j) with
+ *     |i - j| + |(m - n) - (i - j)| <= k
+ * (it needs |i - j| edits to reach the cell and |(m - n) - (i - j)| more to
+ * leave it), i.e. through the diagonals i - j in [min(0, dl) - e, max(0, dl) + e],
+ * dl = m - n, e = floor((k - |dl|) / 2): at most k + 1 diagonals (Ukkonen's band).
+ *
+ * A band scan keeps WIN pattern rows (one T word) in a window that slides down
+ * one row per text column, covering the diagonals [d0, d0 + WIN - 2] of every
+ * column plus one row below them. Values outside the window are replaced by
+ * upper bounds (+1 horizontal delta above it, +1 vertical delta below it, the
+ * same conventions as the DP's own first row and column), so every value the
+ * scan holds is >= the true one, and a cell whose optimal path stays inside
+ * the window is exact. With the band of k inside the window that covers an
+ * optimal alignment whenever d <= k: the scan returns d exactly then, and a
+ * value > k otherwise; the final-diagonal exit bound stays valid as well (its
+ * cell is on the final diagonal, inside the band). Cost per column: one T
+ * step plus Eq of WIN / 32 + 1 plane words, whatever m is. */
+
+/* v_alignbit_b32: the low 32 bits of (hi:lo) >> (sh & 31). */
+BL_HD u32 funnel32(u32 hi, u32 lo, u32 sh)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_alignbit(hi, lo, sh);
+#else
+    return (u32)((((u64)hi << 32) | lo) >> (sh & 31));
+#endif
+}
+
+/* The largest threshold a window of WIN rows at diagonal offset d0 decides for
+ * a pair with m - n = dl (-1: none). */
+template <int WIN>
+BL_HD int band_reach(int dl, int d0)
+{
+    const int lo = dl < 0 ? dl : 0, hi = dl > 0 ? dl : 0;
+    const int e1 = lo - d0, e2 = d0 + WIN - 2 - hi;
+    const int e = e1 < e2 ? e1 : e2;
+    return e < 0 ? -1 : (dl < 0 ? -dl : dl) + 2 * e + 1;
+}
+
+/* The offset that maximises the least reach over dl in [dl_min, dl_max]. */
+template <int WIN>
+BL_HD int band_offset(int dl_min, int dl_max)
+{
+    const int t = (dl_min < 0 ? dl_min : 0) + (dl_max > 0 ? dl_max : 0) - (WIN - 2);
+    return t >= 0 ? t / 2 : -((1 - t) / 2);          /* floor(t / 2) */
+}
+
+/* Raw bytes of pattern rows [32q, 32q + 32) as 8 dwords (bytes at or past m
+ * unspecified; none at all when 32q >= m): 9 aligned dword loads, all within
+ * the dwords that hold a byte of p[0 .. m). */
+BL_HD void word_bytes(const u8 *p, int m, int q, u32 (&raw)[8])
+{
+    if (32 * q >= m) {
+#pragma unroll
+        for (int g = 0; g < 8; g++) raw[g] = 0;
+        return;
+    }
+    const u8 *s0 = p + 32 * q;
+    const u32 sh = (u32)((uintptr_t)s0 & 3);
+    const u8 *al = s0 - sh;
+    const u8 *e = p + (m - 1);
+    const u8 *last = e - ((uintptr_t)e & 3);
+    u32 d[9];
+#pragma unroll
+    for (int g = 0; g < 9; g++) d[g] = al + 4 * g <= last ? ld32(al + 4 * g) : 0u;
+#pragma unroll
+    for (int g = 0; g < 8; g++) raw[g] = align_bytes(d[g + 1], d[g], sh);
+}
+
+/* Bit-planes of those rows: bit r of pw[b] = bit b of row 32q + r (rows at or
+ * past m hold unspecified bits, which never reach the rows below them). */
+BL_HD void word_planes(const u32 (&raw)[8], u32 (&pw)[8])
+{
+#pragma unroll
+    for (int b = 0; b < 8; b++) pw[b] = 0;
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        const u64 y = transpose8(((u64)raw[2 * g + 1] << 32) | raw[2 * g]);
+#pragma unroll
+        for (int b = 0; b < 8; b++) pw[b] |= (u32)((y >> (8 * b)) & 0xFFu) << (8 * g);
+    }
+}
+
+/* Eq of one plane word: AND over b of ~(plane_b ^ m_b) (m[0] inverted). */
+BL_HD u32 eq_word(const u32 (&pw)[8], const u32 (&m)[8])
+{
+    u32 e = pw[0] ^ m[0];
+#pragma unroll
+    for (int b = 1; b < 8; b++) e = bop3<tt(TA & ~(TB ^ TC))>(e, pw[b], m[b]);
+    return e;
+}
+
+/* Band scan of pattern p (m >= 1 rows, bytes in memory) against a text of
+ * n >= 1 bytes, window T (32 or 64 rows) at diagonal offset d0, for a threshold
+ * kq <= band_reach<WIN>(m - n, d0): returns d if d <= kq, else kq + 1. early:
+ * test the final-diagonal bound every 4 columns and stop once it exceeds kq.
+ * The window's plane words are built from the pattern bytes as it slides (the
+ * bytes of the next word are loaded one word ahead). Groups of four columns
+ * that neither start sliding nor enter a new plane word -- all but about one
+ * in eight -- run without those checks; the rest go column by column. */
+template <typename T, typename Text>
+BL_HD int band_scan(const u8 *p, int m, const Text &text, int n, int d0, int kq, bool early)
+{
+    constexpr int WIN = Rows<T>::n;
+    constexpr int NE = WIN / 32 + 1;             /* plane words under one window */
+    u32 cur[NE][8], raw[8];
+#pragma unroll
+    for (int w = 0; w < NE; w++) {
+        word_bytes(p, m, w, raw);
+        word_planes(raw, cur[w]);
+    }
+    word_bytes(p, m, NE, raw);
+    T Pv = ~(T)0, Mv = 0;
+    int V = 0;                /* value of the window's base row s at column j */
+    int s = 0, q = 0;         /* window = rows s+1 .. s+WIN; cur[0] = plane word q */
+    int j = 0;
+    /* Eq of text byte c on the current window */
+    auto eqwin = [&](u32 c) {
+        u32 m8[8];
+        text_masks(c, m8);
+        u32 E[NE];
+#pragma unroll
+        for (int w = 0; w < NE; w++) E[w] = eq_word(cur[w], m8);
+        const u32 sh = (u32)s;
+        if (WIN == 32) return (T)funnel32(E[1], E[0], sh);
+        return (T)(((u64)funnel32(E[NE - 1], E[1], sh) << 32) | funnel32(E[1], E[0], sh));
+    };
+    /* one column (byte c), the window staying put */
+    auto step = [&](u32 c) {
+        T Ph, Mh;
+        word_step<T>(Pv, Mv, eqwin(c), 1u, 0u, Ph, Mh);
+        V += 1;
+    };
+    /* one column, then the window one row down. The new deltas shifted down a
+     * row undo the step's own up-shift of Ph / Mh, and the row that leaves the
+     * window always has Pv = 0, Mv = Xv bit 0 (hp = 1 enters below it), so:
+     *   Pv' = Mh | ~((Xv >> 1) | Ph) | top,  Mv' = Ph & (Xv >> 1),
+     *   V' = V + 1 - (Xv & 1). */
+    auto step_slide = [&](u32 c) {
+        const T Eq = eqwin(c);
+        const T Xv = Eq | Mv;
+        const T S = (T)((Eq & Pv) + Pv);
+        const T Xh = bop3<tt((TA ^ TB) | TC)>(S, Pv, Eq);          /* (S ^ Pv) | Eq */
+        const T Ph = bop3<tt(TA | ~(TB | TC))>(Mv, Xh, Pv);        /* Mv | ~(Xh | Pv) */
+        const T Mh = Pv & Xh;
+        const T X1 = (T)(Xv >> 1);
+        V += 1 - (int)(u32)(Xv & 1u);
+        Pv = bop3<tt(TA | ~(TB | TC))>(Mh, X1, Ph) | (T)((T)1 << (WIN - 1));
+        Mv = Ph & X1;
+        s += 1;
+    };
+    /* the window entered the next plane word */
+    auto refresh = [&]() {
+#pragma unroll
+        for (int w = 0; w + 1 < NE; w++)
+#pragma unroll
+            for (int b = 0; b < 8; b++) cur[w][b] = cur[w + 1][b];
+        word_planes(raw, cur[NE - 1]);
+        q += 1;
+        word_bytes(p, m, q + NE, raw);
+    };
+    auto value = [&](int i) {                    /* s <= i <= s + WIN */
+        const T mk = rows_below<T>(i - s);
+        return V + popc((T)(Pv & mk)) - popc((T)(Mv & mk));
+    };
+    auto over = [&]() {                          /* early exit: bound above kq */
+        const int i0 = j + m - n;
+        return (i0 <= 0 ? n - m : value(i0)) > kq;
+    };
+    auto ts = text.open(n);
+    u32 c4 = ts.next4();
+    for (;;) {
+        /* The two inner loops leave the plane words alone, so nothing but the
+         * scan state is carried around them. */
+        while (n - j >= 4 && j + 4 + d0 - 1 <= 0) {          /* no slide in these 4 */
+            const u32 c4n = ts.next4();
+#pragma unroll
+            for (int u = 0; u < 4; u++) step((c4 >> (8 * u)) & 0xFFu);
+            j += 4;
+            if (early && over()) return kq + 1;
+            c4 = c4n;
+        }
+        while (n - j >= 4 && j + d0 - 1 >= 0 && (s & 31) <= 27) {  /* 4 slides, one word */
+            const u32 c4n = ts.next4();
+#pragma unroll
+            for (int u = 0; u < 4; u++) step_slide((c4 >> (8 * u)) & 0xFFu);
+            j += 4;
+            if (early && over()) return kq + 1;
+            c4 = c4n;
+        }
+        /* one group column by column: the slide starts, the window enters the
+         * next plane word, or fewer than 4 columns are left */
+        const u32 c4n = ts.next4();
+        const int cnt = n - j < 4 ? n - j : 4;
+        for (int u = 0; u < cnt; u++) {
+            j += 1;
+            if (j + d0 - 1 > s) {
+                step_slide((c4 >> (8 * u)) & 0xFFu);
+                if ((s & 31) == 0) refresh();
+            } else {
+                step((c4 >> (8 * u)) & 0xFFu);
+            }
+        }
+        if (j >= n) {
+            const int d = value(m);
+            return d <= kq ? d : kq + 1;
+        }
+        if (early && over()) return kq + 1;
+        c4 = c4n;
+    }
+}
+
+/* A pair before its band scan: trim, swap (pattern a = the shorter string),
+ * then the gap / empty rules. Returns the result when these decide it, else
+ * -1 (a scan is needed; m - n = la - lb <= 0). */
+BL_HD int band_prepare(const u8 *&a, int &la, const u8 *&b, int &lb, int trim, int k)
+{
+    if (trim) trim_pair(a, la, b, lb);
+    if (la > lb) {
+        const u8 *t = a; a = b; b = t;
+        const int x = la; la = lb; lb = x;
+    }
+    if (k >= 0 && lb - la > k) return k + 1;
+    if (la == 0) return lb;
+    return -1;
+}
+
+/* A prepared pair through band_scan at the caller's offset d0: the pair's
+ * result (min(d, k + 1), or d for k < 0), or -1 when d is above what the
+ * window reaches (the caller then scans the whole pattern). */
+template <typename T>
+BL_HD int band_pair(const u8 *a, int la, const u8 *b, int lb, int k, int d0)
+{
+    constexpr int WIN = Rows<T>::n;
+    const int reach = band_reach<WIN>(la - lb, d0);
+    if (reach < 0) return -1;
+    const int kq = k >= 0 && k < reach ? k : reach;
+    const ByteText text{b};
+    const int r = band_scan<T>(a, la, text, lb, d0, kq, k >= 0);
+    if (r <= kq) return r;
+    return k >= 0 && kq == k ? k + 1 : -1;
+}
+
 /* ------------------------------------------------------------ Peq lane */
 /* Distance of a pattern given by its Peq table (row c: NW words, peq[c*NW + w])
  * of length 1 <= m <= 64*NW against a text of n bytes (a ByteText or a

@@ -97,70 +97,19 @@ struct HostStripes {
     StripeText text(int64_t c) const { return StripeText{words.data() + gbase[(size_t)(c / 64)] + c % 64}; }
 };
 
-extern "C" {
-
-/* One pair through the lane code the pairs kernels run (class by the shorter
- * length: <= 64, <= 128, <= 256 in registers; longer via the wavefront). */
-int blh_pair(const u8 *a, int la, const u8 *b, int lb, int trim, int k)
+static int w32_pair(int w, const u8 *ba, const int64_t *oa, const u8 *bb, const int64_t *ob,
+                    int64_t i, int trim, int k)
 {
-    return pair_host(a, la, b, lb, trim, k);
-}
-
-/* One pair through the 32-row-unit lane code of the batched tiles; w32 must
- * cover the shorter untrimmed string (1..8). */
-int blh_pair_w32(const u8 *a, int la, const u8 *b, int lb, int trim, int k, int w32)
-{
-    switch (w32) {
-    case 1: return pair_lane_w32<1>(a, la, b, lb, trim, k);
-    case 2: return pair_lane_w32<2>(a, la, b, lb, trim, k);
-    case 3: return pair_lane_w32<3>(a, la, b, lb, trim, k);
-    case 4: return pair_lane_w32<4>(a, la, b, lb, trim, k);
-    case 5: return pair_lane_w32<5>(a, la, b, lb, trim, k);
-    case 6: return pair_lane_w32<6>(a, la, b, lb, trim, k);
-    case 7: return pair_lane_w32<7>(a, la, b, lb, trim, k);
-    case 8: return pair_lane_w32<8>(a, la, b, lb, trim, k);
-    default: return -1;
+    switch (w) {
+    case 1: return packed_pair_w32<1>(ba, oa, bb, ob, i, trim, k);
+    case 2: return packed_pair_w32<2>(ba, oa, bb, ob, i, trim, k);
+    case 3: return packed_pair_w32<3>(ba, oa, bb, ob, i, trim, k);
+    case 4: return packed_pair_w32<4>(ba, oa, bb, ob, i, trim, k);
+    case 5: return packed_pair_w32<5>(ba, oa, bb, ob, i, trim, k);
+    case 6: return packed_pair_w32<6>(ba, oa, bb, ob, i, trim, k);
+    case 7: return packed_pair_w32<7>(ba, oa, bb, ob, i, trim, k);
+    default: return packed_pair_w32<8>(ba, oa, bb, ob, i, trim, k);
     }
-}
-
-/* One (query, choice) through the Peq lane the cdist kernels run. */
-int blh_peq(const u8 *q, int m, const u8 *t, int n, int k)
-{
-    const int64_t off[2] = {0, n};
-    const HostStripes hs(t, off, 1);
-    return peq_host(q, m, hs.text(0), t, n, k);
-}
-
-/* The wavefront path for any lengths (is_pair: trim/swap as pairs do). */
-int blh_long(const u8 *p, int m, const u8 *t, int n, int is_pair, int trim, int k)
-{
-    return long_sim(p, m, t, n, is_pair, trim, k);
-}
-
-/* Packed batches, int32 results. */
-int blh_pairs(const u8 *ba, const int64_t *oa, const u8 *bb, const int64_t *ob, int64_t n,
-              int32_t *out, int trim, int k, int force_long)
-{
-    for (int64_t i = 0; i < n; i++) {
-        const u8 *a = ba + oa[i], *b = bb + ob[i];
-        const int la = (int)(oa[i + 1] - oa[i]), lb = (int)(ob[i + 1] - ob[i]);
-        out[i] = force_long ? long_sim(a, la, b, lb, 1, trim, k) : pair_host(a, la, b, lb, trim, k);
-    }
-    return 0;
-}
-
-int blh_cross(const u8 *bq, const int64_t *oq, int64_t nq, const u8 *bc, const int64_t *oc,
-              int64_t nc, int32_t *out, int k, int force_long)
-{
-    const HostStripes hs(bc, oc, nc);
-    for (int64_t i = 0; i < nq; i++)
-        for (int64_t j = 0; j < nc; j++) {
-            const u8 *q = bq + oq[i], *t = bc + oc[j];
-            const int m = (int)(oq[i + 1] - oq[i]), n = (int)(oc[j + 1] - oc[j]);
-            out[i * nc + j] = force_long ? long_sim(q, m, t, n, 0, 0, k)
-                                         : peq_host(q, m, hs.text(j), t, n, k);
-        }
-    return 0;
 }
 
 /* The batched pairs pipeline emulated step by step with the kernels' shared
@@ -172,8 +121,10 @@ int blh_cross(const u8 *bq, const int64_t *oq, int64_t nq, const u8 *bc, const i
  * a tile has a shorter string of more 32-row units than its first pair). chunk:
  * pairs per chunk (a multiple of 256 on the GPU; any multiple of 256 here).
  * Returns 0, or a negative code naming the violated check. */
-int blh_pairs_batched(const u8 *ba, const int64_t *oa, const u8 *bb, const int64_t *ob, int64_t n,
-                      int32_t *out, int trim, int k, int bin_width, int64_t chunk)
+template <typename BandTile>
+static int blh_pairs_batched_core(const u8 *ba, const int64_t *oa, const u8 *bb,
+                                  const int64_t *ob, int64_t n, int32_t *out, int trim, int k,
+                                  int bin_width, int64_t chunk, int band, BandTile band_tile)
 {
     if (chunk < 256 || chunk % 256) return -9;
     const int64_t nchunks = n ? (n + chunk - 1) / chunk : 1;
@@ -261,22 +212,168 @@ int blh_pairs_batched(const u8 *ba, const int64_t *oa, const u8 *bb, const int64
         if (s0 >= ce) return -14;
         const int w = key_units(key[(size_t)sorted[(size_t)s0]]);
         if (w < 1 || w > 8) return -15;
+        for (int64_t q = s0; q < s1; q++)
+            if (((minlen(sorted[(size_t)q]) + 31) >> 5) > w) return -16;
+        if (band > 0 && w >= band) {
+            band_tile(std::vector<int32_t>(sorted.begin() + s0, sorted.begin() + s1));
+            continue;
+        }
         for (int64_t q = s0; q < s1; q++) {
             const int64_t i = sorted[(size_t)q];
-            if (((minlen(i) + 31) >> 5) > w) return -16;
-            switch (w) {
-            case 1: out[i] = packed_pair_w32<1>(ba, oa, bb, ob, i, trim, k); break;
-            case 2: out[i] = packed_pair_w32<2>(ba, oa, bb, ob, i, trim, k); break;
-            case 3: out[i] = packed_pair_w32<3>(ba, oa, bb, ob, i, trim, k); break;
-            case 4: out[i] = packed_pair_w32<4>(ba, oa, bb, ob, i, trim, k); break;
-            case 5: out[i] = packed_pair_w32<5>(ba, oa, bb, ob, i, trim, k); break;
-            case 6: out[i] = packed_pair_w32<6>(ba, oa, bb, ob, i, trim, k); break;
-            case 7: out[i] = packed_pair_w32<7>(ba, oa, bb, ob, i, trim, k); break;
-            default: out[i] = packed_pair_w32<8>(ba, oa, bb, ob, i, trim, k); break;
-            }
+            out[i] = w32_pair(w, ba, oa, bb, ob, i, trim, k);
         }
     }
     return 0;
+}
+
+extern "C" {
+
+/* One pair through the lane code the pairs kernels run (class by the shorter
+ * length: <= 64, <= 128, <= 256 in registers; longer via the wavefront). */
+int blh_pair(const u8 *a, int la, const u8 *b, int lb, int trim, int k)
+{
+    return pair_host(a, la, b, lb, trim, k);
+}
+
+/* One pair through the 32-row-unit lane code of the batched tiles; w32 must
+ * cover the shorter untrimmed string (1..8). */
+int blh_pair_w32(const u8 *a, int la, const u8 *b, int lb, int trim, int k, int w32)
+{
+    switch (w32) {
+    case 1: return pair_lane_w32<1>(a, la, b, lb, trim, k);
+    case 2: return pair_lane_w32<2>(a, la, b, lb, trim, k);
+    case 3: return pair_lane_w32<3>(a, la, b, lb, trim, k);
+    case 4: return pair_lane_w32<4>(a, la, b, lb, trim, k);
+    case 5: return pair_lane_w32<5>(a, la, b, lb, trim, k);
+    case 6: return pair_lane_w32<6>(a, la, b, lb, trim, k);
+    case 7: return pair_lane_w32<7>(a, la, b, lb, trim, k);
+    case 8: return pair_lane_w32<8>(a, la, b, lb, trim, k);
+    default: return -1;
+    }
+}
+
+/* One pair through the band lane (win 32 or 64 rows at diagonal offset d0):
+ * the pair's result, or -1 when the window cannot decide it. *reach receives
+ * the threshold the window decides for this pair after trimming (-2 when the
+ * gap / empty rules decided it without a scan). */
+int blh_band(const u8 *a, int la, const u8 *b, int lb, int trim, int k, int d0, int win,
+             int *reach)
+{
+    const int r = band_prepare(a, la, b, lb, trim, k);
+    if (r >= 0) {
+        *reach = -2;
+        return r;
+    }
+    if (win == 32) {
+        *reach = band_reach<32>(la - lb, d0);
+        return band_pair<u32>(a, la, b, lb, k, d0);
+    }
+    *reach = band_reach<64>(la - lb, d0);
+    return band_pair<u64>(a, la, b, lb, k, d0);
+}
+
+/* The window offset a tile uses for pairs with m - n in [dl_min, dl_max]. */
+int blh_band_offset(int dl_min, int dl_max, int win)
+{
+    return win == 32 ? band_offset<32>(dl_min, dl_max) : band_offset<64>(dl_min, dl_max);
+}
+
+/* One (query, choice) through the Peq lane the cdist kernels run. */
+int blh_peq(const u8 *q, int m, const u8 *t, int n, int k)
+{
+    const int64_t off[2] = {0, n};
+    const HostStripes hs(t, off, 1);
+    return peq_host(q, m, hs.text(0), t, n, k);
+}
+
+/* The wavefront path for any lengths (is_pair: trim/swap as pairs do). */
+int blh_long(const u8 *p, int m, const u8 *t, int n, int is_pair, int trim, int k)
+{
+    return long_sim(p, m, t, n, is_pair, trim, k);
+}
+
+/* Packed batches, int32 results. */
+int blh_pairs(const u8 *ba, const int64_t *oa, const u8 *bb, const int64_t *ob, int64_t n,
+              int32_t *out, int trim, int k, int force_long)
+{
+    for (int64_t i = 0; i < n; i++) {
+        const u8 *a = ba + oa[i], *b = bb + ob[i];
+        const int la = (int)(oa[i + 1] - oa[i]), lb = (int)(ob[i + 1] - ob[i]);
+        out[i] = force_long ? long_sim(a, la, b, lb, 1, trim, k) : pair_host(a, la, b, lb, trim, k);
+    }
+    return 0;
+}
+
+int blh_cross(const u8 *bq, const int64_t *oq, int64_t nq, const u8 *bc, const int64_t *oc,
+              int64_t nc, int32_t *out, int k, int force_long)
+{
+    const HostStripes hs(bc, oc, nc);
+    for (int64_t i = 0; i < nq; i++)
+        for (int64_t j = 0; j < nc; j++) {
+            const u8 *q = bq + oq[i], *t = bc + oc[j];
+            const int m = (int)(oq[i + 1] - oq[i]), n = (int)(oc[j + 1] - oc[j]);
+            out[i * nc + j] = force_long ? long_sim(q, m, t, n, 0, 0, k)
+                                         : peq_host(q, m, hs.text(j), t, n, k);
+        }
+    return 0;
+}
+
+/* band > 0: tiles of at least `band` 32-row units run as band tiles (one
+ * window offset per tile from its lanes' length differences, 32 rows when
+ * every threshold is <= 14, else 64, as pairs_tile_band does); undecided pairs
+ * go through the whole scan 64 at a time in the units of the widest. (The
+ * kernel's waves keep them on per-wave stacks; which pairs share a scan never
+ * changes a result.) *escalated receives their number. */
+int blh_pairs_batched_band(const u8 *ba, const int64_t *oa, const u8 *bb, const int64_t *ob,
+                           int64_t n, int32_t *out, int trim, int k, int bin_width, int64_t chunk,
+                           int band, int64_t *escalated)
+{
+    std::vector<int64_t> esc;
+    auto band_tile = [&](const std::vector<int32_t> &idx) {
+        struct L { int64_t i; const u8 *a, *b; int la, lb, res; };
+        std::vector<L> ln;
+        int dmin = 0x7fffffff, dmax = -0x7fffffff;
+        bool narrow = true;
+        for (int32_t i : idx) {
+            L x{i, ba + oa[i], bb + ob[i], (int)(oa[i + 1] - oa[i]), (int)(ob[i + 1] - ob[i]), 0};
+            x.res = band_prepare(x.a, x.la, x.b, x.lb, trim, k);
+            if (x.res < 0) {
+                const int dl = x.la - x.lb;
+                dmin = dl < dmin ? dl : dmin;
+                dmax = dl > dmax ? dl : dmax;
+                if (!(k >= 0 && k <= 14)) narrow = false;
+            }
+            ln.push_back(x);
+        }
+        for (L &x : ln) {
+            if (x.res < 0)
+                x.res = narrow ? band_pair<u32>(x.a, x.la, x.b, x.lb, k, band_offset<32>(dmin, dmax))
+                               : band_pair<u64>(x.a, x.la, x.b, x.lb, k, band_offset<64>(dmin, dmax));
+            if (x.res >= 0) out[x.i] = x.res;
+            else esc.push_back(x.i);
+        }
+    };
+    const int rc = blh_pairs_batched_core(ba, oa, bb, ob, n, out, trim, k, bin_width, chunk, band,
+                                          band_tile);
+    for (size_t t0 = 0; t0 < esc.size(); t0 += 64) {
+        const size_t t1 = t0 + 64 < esc.size() ? t0 + 64 : esc.size();
+        int w = 0;
+        for (size_t e = t0; e < t1; e++) {
+            const int64_t i = esc[e];
+            const int64_t la = oa[i + 1] - oa[i], lb = ob[i + 1] - ob[i];
+            const int u = (int)(((la < lb ? la : lb) + 31) >> 5);
+            w = u > w ? u : w;
+        }
+        for (size_t e = t0; e < t1; e++) out[esc[e]] = w32_pair(w, ba, oa, bb, ob, esc[e], trim, k);
+    }
+    if (escalated) *escalated = (int64_t)esc.size();
+    return rc;
+}
+
+int blh_pairs_batched(const u8 *ba, const int64_t *oa, const u8 *bb, const int64_t *ob, int64_t n,
+                      int32_t *out, int trim, int k, int bin_width, int64_t chunk)
+{
+    return blh_pairs_batched_band(ba, oa, bb, ob, n, out, trim, k, bin_width, chunk, 0, nullptr);
 }
 
 /* The 4-choice byte-segment filter: bounds of choices t[0..3] (first 4 bytes

@@ -152,7 +152,12 @@ typedef struct {
     int bin_width;         /* text-length bucket width in bytes (default 16) */
     int reserved[4];       /* [0]: pairs per bucketing chunk (default 65536);
                               [1]: 1 = direct mode (one launch, no bucketing);
-                              [2]: tiles per work-queue ticket (default 1) */
+                              [2]: tiles per work-queue ticket (default 1);
+                              [3]: tiles of at least this many 32-row units (1..8)
+                              scan a diagonal band first (Ukkonen: an alignment of
+                              cost <= k stays within k + 1 diagonals), pairs the
+                              band cannot decide then the whole pattern (default
+                              0: no band tiles) */
 } bl_pairs_params;
 
 /* Bytes of device workspace bl_cdist needs for nq queries of at most

@@ -38,13 +38,46 @@ def test_every_pairs_launch_shape_gives_the_same_results():
     A, B = hip.Packed(*pa), hip.Packed(*pb)
     want = {k: O.pairs(pa, pb, k=k, threads=8) for k in (-1, 5)}
     space = {"waves_per_cu": [4, 32], "block": [64, 256], "bin_width": [1, 64],
-             "chunk": [256, 1 << 20]}
+             "chunk": [256, 1 << 20], "band": [0, 1]}
     direct = {"mode": ["direct"], "waves_per_cu": [1, 32], "block": [64, 256]}
     for p in list(autotune._grid(space)) + list(autotune._grid(direct)):
         for k in (-1, 5):
             out = hip.DeviceArray.empty((A.n,), np.int32)
             hip.pairs_batched_into(A, B, out, k, True, None, None, p)
             np.testing.assert_array_equal(out.get(), want[k], err_msg=str(p))
+
+
+@pytest.mark.parametrize("band", [1, 3, 5])
+def test_band_tiles_on_uneven_pairs(band):
+    """Band tiles on the uneven workload's kind of pairs (lengths 1-256, bytes
+    0-255, edit rates 5-30%) and on harder ones (rates up to 45%, so the window
+    leaves some pairs to the whole scan), exact, thresholded with the narrow
+    window (k <= 14) and the wide one, per-item thresholds, trimmed or not."""
+    A, B = W.pairs(3, 20000, seed=5)
+    rng = np.random.default_rng(8)
+    Ah, Bh = [], []
+    for _ in range(4000):
+        a = rng.integers(0, 256, int(rng.integers(1, 257))).astype(np.uint8)
+        b = a.copy()
+        idx = rng.random(len(b)) < rng.uniform(0.3, 0.45)
+        b[idx] = rng.integers(0, 256, int(idx.sum())).astype(np.uint8)
+        cut = int(rng.integers(0, 30))
+        Ah.append(bytes(a))
+        Bh.append(bytes(b[cut:]) + bytes(rng.integers(0, 256, int(rng.integers(0, 30))).astype(np.uint8)))
+    p = {"waves_per_cu": 16, "block": 256, "bin_width": 8, "chunk": 16384, "band": band}
+    for pa, pb in ((A, B), (O.pack(Ah), O.pack(Bh))):
+        X, Y = hip.Packed(*pa), hip.Packed(*pb)
+        exact = O.pairs(pa, pb, threads=8)
+        for k in (-1, 3, 14, 40):
+            for trim in (True, False):
+                out = hip.DeviceArray.empty((X.n,), np.int32)
+                hip.pairs_batched_into(X, Y, out, k, trim, None, None, p)
+                want = exact if k < 0 else np.minimum(exact, k + 1)
+                np.testing.assert_array_equal(out.get(), want, err_msg=f"k={k} trim={trim}")
+        kv = np.random.default_rng(9).choice([-1, 0, 2, 14, 30], size=X.n).astype(np.int32)
+        out = hip.DeviceArray.empty((X.n,), np.int32)
+        hip.pairs_batched_into(X, Y, out, -1, True, None, None, p, kvec=hip.DeviceBuffer.from_host(kv))
+        np.testing.assert_array_equal(out.get(), np.where(kv < 0, exact, np.minimum(exact, kv + 1)))
 
 
 @pytest.mark.parametrize("gen,k", [(4, 8), (2, 2), (2, 0)])
@@ -188,3 +221,27 @@ def test_graph_replays_match_eager():
         s.sync()
         np.testing.assert_array_equal(o1.get(), eager1, err_msg=f"replay {rep}")
         np.testing.assert_array_equal(o2.get(), eager2, err_msg=f"replay {rep}")
+
+
+def test_autotuner_picks_and_caches(tmp_path, monkeypatch):
+    """The tuners time their candidates (band tiles included for pairs), return
+    a shape that gives the oracle's results, and answer the next call from
+    their cache file."""
+    monkeypatch.setenv("BITLEV_TUNE_CACHE", str(tmp_path / "tune.json"))
+    pa, pb = W.pairs(3, 30000, seed=6)
+    A, B = hip.Packed(*pa), hip.Packed(*pb)
+    out = hip.DeviceArray.empty((A.n,), np.uint16)
+    seen = []
+    p = autotune.tune_pairs(A, B, out, -1, True, None, sample_pairs=8192, log=seen.append)
+    assert any("'band'" in line for line in seen)
+    hip.pairs_batched_into(A, B, out, -1, True, None, None, p)
+    np.testing.assert_array_equal(out.get(), O.saturate(O.pairs(pa, pb, threads=8), np.uint16))
+    assert autotune.tune_pairs(A, B, out, -1, True, None, log=seen.append) == p
+    pq, pc = W.cdist(2, 300, 20000, seed=7)
+    Q, C = hip.Packed(*pq), hip.Packed(*pc)
+    outc = hip.DeviceArray.empty((Q.n, C.n), np.uint8)
+    pc_ = autotune.tune_cdist(Q, C, outc, 2, None, sample_queries=64)
+    hip.cdist_batched_into(Q, C, outc, 2, None, None, pc_)
+    np.testing.assert_array_equal(outc.get(), O.saturate(O.cross(pq, pc, k=2, threads=8), np.uint8))
+    assert autotune.tune_cdist(Q, C, outc, 2, None) == pc_
+    assert (tmp_path / "tune.json").exists()

@@ -102,6 +102,53 @@ def test_batched_pairs_pipeline_emulated(hc, golden, bin_width, chunk):
                                               err_msg=f"{name} k={k} trim={trim}")
 
 
+@pytest.mark.parametrize("band", [1, 3, 5])
+def test_batched_pairs_band_tiles_emulated(hc, golden, band):
+    """Band tiles in the emulated pipeline: one window offset per tile from its
+    lanes' length differences, the 32-row window when every threshold is <= 14,
+    undecided pairs onto the stack and through the whole scan 64 at a time.
+    Uneven lengths with edit rates up to 0.45 (so some distances exceed what a
+    window reaches), exact and thresholded, plus the golden pair sets."""
+    hc.blh_pairs_batched_band.argtypes = [u8p, i64p, u8p, i64p, ctypes.c_int64, i32p, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_int64)]
+    rng = np.random.default_rng(77)
+    A, B = [], []
+    for _ in range(6000):
+        la = int(rng.integers(1, 257))
+        a = rng.integers(0, 256 if rng.random() < 0.7 else 4, la).astype(np.uint8)
+        rate = float(rng.uniform(0.0, 0.45))
+        b = []
+        for ch in a.tolist():
+            u = rng.random()
+            if u >= rate:
+                b.append(ch)
+            elif u < rate / 3:
+                b.append(int(rng.integers(0, 256)))
+            elif u < 2 * rate / 3:
+                b += [ch, int(rng.integers(0, 256))]
+        A.append(bytes(a))
+        B.append(bytes(b[:300]))
+    sets = [(O.pack(A), O.pack(B), "uneven")]
+    sets += [(*pairs_of(golden, name), name) for name in golden_pair_sets(golden)]
+    esc = ctypes.c_int64()
+    escalated = 0
+    for pa, pb, name in sets:
+        (ba, oa), (bb, ob) = pa, pb
+        n = len(oa) - 1
+        for k in (-1, 0, 3, 14, 15, 40):
+            for trim in (0, 1):
+                want = O.pairs(pa, pb, k=k, trim=bool(trim))
+                out = np.full(n, -7, np.int32)
+                rc = hc.blh_pairs_batched_band(_p(ba, u8p), _p(oa, i64p), _p(bb, u8p), _p(ob, i64p),
+                                               n, _p(out, i32p), trim, k, 8, 1024, band,
+                                               ctypes.byref(esc))
+                assert rc == 0, (name, k, trim, rc)
+                np.testing.assert_array_equal(out, want, err_msg=f"{name} k={k} trim={trim}")
+                escalated += esc.value
+    assert escalated > 100
+
+
 def test_wavefront_multi_stripe(hc):
     """Patterns above 4096 bytes take several 64-word stripes and the bottom-row
     scratch between them; compare with the oracle (no word limit)."""
@@ -243,6 +290,83 @@ def test_swar_filter_bounds_equal_the_dp(hc):
             else:
                 assert out[s] == -1
     assert checked > 5000
+
+
+def _band_cases(rng, count):
+    """Pairs whose optimal alignments run along the band's edges: edits
+    bunched at one end (insertions at the front push the path off the main
+    diagonal before it returns), spread evenly, or none, plus unrelated strings."""
+    cases = []
+    for _ in range(count):
+        alpha = int(rng.choice([2, 4, 26, 256]))
+        m = int(rng.integers(1, 300))
+        a = bytes(rng.integers(0, alpha, m).astype(np.uint8))
+        kind = int(rng.integers(0, 4))
+        if kind == 0:
+            g = int(rng.integers(0, 40))
+            b = bytes(rng.integers(0, alpha, g).astype(np.uint8)) + a
+            if rng.random() < 0.5:
+                b = b[: max(0, len(b) - int(rng.integers(0, 40)))]
+        elif kind == 1:
+            b = bytearray(a)
+            for _ in range(int(rng.integers(0, max(1, m // 3)))):
+                op, pos = int(rng.integers(0, 3)), int(rng.integers(0, len(b) + 1))
+                ch = int(rng.integers(0, alpha))
+                if op == 0 and pos < len(b):
+                    b[pos] = ch
+                elif op == 1:
+                    b.insert(pos, ch)
+                elif pos < len(b):
+                    del b[pos]
+            b = bytes(b)
+        elif kind == 2:
+            b = a
+        else:
+            b = bytes(rng.integers(0, alpha, max(0, m + int(rng.integers(-20, 21)))).astype(np.uint8))
+        cases.append((a, b) if rng.random() < 0.5 else (b, a))
+    return cases
+
+
+def test_band_lane(hc):
+    """The band lane (a 32- or 64-row window sliding down the diagonals
+    [d0, d0 + win - 2], planes built as it goes) returns the exact result
+    whenever the window reaches d (or decides the threshold), -1 -- hand over
+    to the whole scan -- exactly when it does not, and never a wrong value,
+    at offsets that suit the pair, offsets chosen for other lanes of a tile,
+    and offsets that barely reach; thresholded scans stop early with k + 1."""
+    hc.blh_band.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                            ctypes.POINTER(ctypes.c_int)]
+    hc.blh_band_offset.argtypes = [ctypes.c_int] * 3
+    rng = np.random.default_rng(66)
+    cases = _band_cases(rng, 700)
+    pa, pb = O.pack([c[0] for c in cases]), O.pack([c[1] for c in cases])
+    exact = O.pairs(pa, pb, k=-1, trim=False)
+    reach = ctypes.c_int()
+    decided = handed = 0
+    for k in (-1, 0, 2, 8, 30, 62, 100):
+        for trim in (0, 1):
+            want = O.pairs(pa, pb, k=k, trim=bool(trim))
+            for i, (a, b) in enumerate(cases):
+                dl = -abs(len(a) - len(b))
+                for win in (32, 64):
+                    best = hc.blh_band_offset(dl, dl, win)
+                    offs = {best, best - 2, best + 2, hc.blh_band_offset(dl - 9, dl, win),
+                            hc.blh_band_offset(dl, 0, win), dl, 2 - win}
+                    for d0 in offs:
+                        got = hc.blh_band(a, len(a), b, len(b), trim, k, d0, win, ctypes.byref(reach))
+                        r, d = reach.value, int(exact[i])
+                        if r == -2:
+                            assert got == want[i], (i, k, trim)
+                            continue
+                        undecidable = r < 0 or (d > r and (k < 0 or k > r))
+                        assert (got == -1) == undecidable, (i, len(a), len(b), k, trim, win, d0, r, d, got)
+                        if got != -1:
+                            assert got == want[i], (i, len(a), len(b), k, trim, win, d0, r, d, got)
+                            decided += 1
+                        else:
+                            handed += 1
+    assert decided > 20000 and handed > 2000, (decided, handed)
 
 
 def test_random_against_oracle(hc):
