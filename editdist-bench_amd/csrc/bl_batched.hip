@@ -575,10 +575,10 @@ __device__ __forceinline__ int pair_k(int k_all, const int32_t *kvec, int64_t i)
     return kvec ? clamp_k_dev(kvec[i]) : k_all;
 }
 
-__global__ __launch_bounds__(256) void pairs_classify_kernel(
+__global__ __launch_bounds__(1024) void pairs_classify_kernel(
     const int64_t *__restrict__ oa, const int64_t *__restrict__ ob, int64_t n, void *out, int dtype,
-    int k_all, const int32_t *__restrict__ kvec, int bin_width, int64_t chunk, unsigned short *key,
-    unsigned long long *hist)
+    int k_all, const int32_t *__restrict__ kvec, int bin_width, int band, int64_t chunk,
+    unsigned short *key, unsigned long long *hist)
 {
     __shared__ unsigned int lh[PKEYS];
     for (int x = threadIdx.x; x < PKEYS; x += blockDim.x) lh[x] = 0;
@@ -587,7 +587,7 @@ __global__ __launch_bounds__(256) void pairs_classify_kernel(
     if (i < n) {
         int result = 0;
         const int kk = pair_key((int)(oa[i + 1] - oa[i]), (int)(ob[i + 1] - ob[i]),
-                                pair_k(k_all, kvec, i), bin_width, result);
+                                pair_k(k_all, kvec, i), bin_width, band, result);
         if (kk == 0) store_out(out, i, result, dtype);
         key[i] = (unsigned short)kk;
         if (kk) atomicAdd(&lh[kk], 1u);
@@ -665,7 +665,7 @@ __global__ __launch_bounds__(1024) void prefix_kernel(const unsigned long long *
  * reserves each key's range of the chunk with one global atomic, so pairs that
  * share a bucket (all of them, in a uniform batch) do not contend on one
  * global counter. The order inside a bucket is immaterial. */
-__global__ __launch_bounds__(256) void pairs_scatter_kernel(
+__global__ __launch_bounds__(1024) void pairs_scatter_kernel(
     const unsigned short *key, int64_t n, int64_t chunk, unsigned long long *cursor,
     const unsigned long long *chunk_base, int32_t *sorted)
 {
@@ -846,7 +846,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void p
     const int32_t *__restrict__ sorted, const unsigned short *__restrict__ key_of,
     const unsigned long long *__restrict__ hist, const unsigned long long *__restrict__ chunk_base,
     const unsigned long long *__restrict__ tile_base, int64_t nchunks, void *out, int dtype,
-    int grab, unsigned long long *counter, int8_t *scratch, int64_t slot, int band)
+    int grab, unsigned long long *counter, int8_t *scratch, int64_t slot)
 {
     __shared__ int32_t esc_all[4][PB_ESC];
     int32_t *esc = esc_all[threadIdx.x >> 6];
@@ -881,15 +881,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void p
         const int64_t s0 = cb + nlong + (lt - nlong) * 64;
         const int64_t ce = (int64_t)chunk_base[c + 1];
         const int64_t s1 = s0 + 64 < ce ? s0 + 64 : ce;
-        /* descending keys: the first pair of a tile has its largest unit count */
-        const int w = uni(key_units(key_of[sorted[s0]]));
-        if (band > 0 && w >= band) {
+        /* descending keys: the first pair of a tile has its largest key (a band
+         * key, or the largest unit count of a whole-pattern tile) */
+        const int key0 = uni(key_of[sorted[s0]]);
+        if (key_band(key0)) {
             pairs_tile_band(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype, esc, nesc);
             if (nesc >= 64)
                 pairs_esc_flush(ba, oa, bb, ob, trim, k_all, kvec, out, dtype, esc, nesc, 64);
             continue;
         }
-        switch (w) {
+        switch (key_units(key0)) {
         case 1: pairs_tile<1>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype); break;
         case 2: pairs_tile<2>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype); break;
         case 3: pairs_tile<3>(ba, oa, bb, ob, sorted, s0, s1, trim, k_all, kvec, out, dtype); break;
@@ -1109,7 +1110,7 @@ static int64_t pairs_chunk(int64_t n, const bl_pairs_params *prm)
     int64_t ch = prm && prm->reserved[0] > 0 ? prm->reserved[0] : 65536;
     const int64_t floor_ch = (n + 4095) / 4096;           /* at most 4096 chunks */
     if (ch < floor_ch) ch = floor_ch;
-    return (ch + 255) / 256 * 256;
+    return (ch + 1023) / 1024 * 1024;                     /* whole bucketing blocks */
 }
 
 struct PairsLayout {
@@ -1137,10 +1138,10 @@ static PairsLayout pairs_layout(int64_t n, int64_t max_len, const bl_pairs_param
     return L;
 }
 
-/* Enough for any launch shape: chunks are at least 256 pairs and at most 4096. */
+/* Enough for any launch shape: chunks are at least 1024 pairs and at most 4096. */
 static int64_t max_chunks(int64_t n)
 {
-    const int64_t c = (n + 255) / 256;
+    const int64_t c = (n + 1023) / 1024;
     return c < 4096 ? (c < 1 ? 1 : c) : 4096;
 }
 
@@ -1210,17 +1211,17 @@ int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t
     } else {
         /* the counter and the histograms start at zero */
         rc = launch_clear(ctr, (int)(8 + nk), s);
-        const unsigned cblocks = (unsigned)((n + 255) / 256);
+        const unsigned cblocks = (unsigned)((n + 1023) / 1024);  /* 1024 pairs per block */
         if (rc == BL_OK) {
-            pairs_classify_kernel<<<cblocks, 256, 0, s>>>(d_off_a, d_off_b, n, d_out, dtype, k, d_kvec,
-                                                          bw, L.chunk, key, hist);
+            pairs_classify_kernel<<<cblocks, 1024, 0, s>>>(d_off_a, d_off_b, n, d_out, dtype, k, d_kvec,
+                                                          bw, band, L.chunk, key, hist);
             pairs_chunk_scan_kernel<<<(unsigned)L.nchunks, 256, 0, s>>>(hist, cursor, chunk_cnt,
                                                                         tile_cnt);
             prefix_kernel<<<1, 1024, 0, s>>>(chunk_cnt, chunk_base, tile_cnt, tile_base, L.nchunks);
-            pairs_scatter_kernel<<<cblocks, 256, 0, s>>>(key, n, L.chunk, cursor, chunk_base, sorted);
+            pairs_scatter_kernel<<<cblocks, 1024, 0, s>>>(key, n, L.chunk, cursor, chunk_base, sorted);
             pairs_batched_kernel<<<(unsigned)blocks, wg, 0, s>>>(
                 d_buf_a, d_off_a, d_buf_b, d_off_b, trim, k, d_kvec, sorted, key, hist, chunk_base,
-                tile_base, L.nchunks, d_out, dtype, grab, ctr, scratch, slot, band);
+                tile_base, L.nchunks, d_out, dtype, grab, ctr, scratch, slot);
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) rc = fail(BL_EHIP, "pairs pipeline launch: %s", hipGetErrorString(e));
         }

@@ -840,30 +840,34 @@ BL_HD int band_scan(const u8 *p, int m, const Text &text, int n, int d0, int kq,
         const int i0 = j + m - n;
         return (i0 <= 0 ? n - m : value(i0)) > kq;
     };
+    /* text dwords two groups ahead: a band column costs few instructions, so
+     * one group does not cover a memory round trip */
     auto ts = text.open(n);
-    u32 c4 = ts.next4();
+    u32 c4 = ts.next4(), c4n = ts.next4();
     for (;;) {
         /* The two inner loops leave the plane words alone, so nothing but the
          * scan state is carried around them. */
         while (n - j >= 4 && j + 4 + d0 - 1 <= 0) {          /* no slide in these 4 */
-            const u32 c4n = ts.next4();
+            const u32 c4nn = ts.next4();
 #pragma unroll
             for (int u = 0; u < 4; u++) step((c4 >> (8 * u)) & 0xFFu);
             j += 4;
             if (early && over()) return kq + 1;
             c4 = c4n;
+            c4n = c4nn;
         }
         while (n - j >= 4 && j + d0 - 1 >= 0 && (s & 31) <= 27) {  /* 4 slides, one word */
-            const u32 c4n = ts.next4();
+            const u32 c4nn = ts.next4();
 #pragma unroll
             for (int u = 0; u < 4; u++) step_slide((c4 >> (8 * u)) & 0xFFu);
             j += 4;
             if (early && over()) return kq + 1;
             c4 = c4n;
+            c4n = c4nn;
         }
         /* one group column by column: the slide starts, the window enters the
          * next plane word, or fewer than 4 columns are left */
-        const u32 c4n = ts.next4();
+        const u32 c4nn = ts.next4();
         const int cnt = n - j < 4 ? n - j : 4;
         for (int u = 0; u < cnt; u++) {
             j += 1;
@@ -880,6 +884,7 @@ BL_HD int band_scan(const u8 *p, int m, const Text &text, int n, int d0, int kq,
         }
         if (early && over()) return kq + 1;
         c4 = c4n;
+        c4n = c4nn;
     }
 }
 
@@ -1084,29 +1089,39 @@ BL_HD int long_prepare(const u8 *&p, int &m, const u8 *&t, int &n, int is_pair, 
  * classifying reads no string bytes; trimming only shortens both strings by
  * the same amount, so the pattern words of the untrimmed shorter string bound
  * the trimmed ones and the length gap is unchanged):
- * key = 1 + (w - 1) * LB_BINS + min(max(la, lb) / bin_width, LB_BINS - 1) for a
- * shorter string of w <= 8 units of 32 bytes, PKEY_LONG for longer ones
- * (above 256 bytes), 0 for a pair the
- * gap or an empty string decides. A larger key means more work; the pipeline
- * schedules large keys first. */
+ *   1 + (w - 1) * LB_BINS + min(max(la, lb) / bin_width, LB_BINS - 1)
+ *       for a shorter string of w <= 8 units of 32 bytes (whole-pattern tiles);
+ *   PKEY_BAND0 + that length bin * GAP_BINS + |la - lb| / 4
+ *       instead when band > 0, w >= band and the gap is at most 63 (band tiles:
+ *       one window offset serves a tile well when its gaps are close);
+ *   PKEY_LONG for a shorter string above 256 bytes;
+ *   0 for a pair the gap or an empty string decides.
+ * Larger keys mean more work; the pipeline schedules large keys first. */
 constexpr int LB_BINS = 64;
-constexpr int PKEY_LONG = 1 + 8 * LB_BINS;
+constexpr int GAP_BINS = 16;
+constexpr int PKEY_BAND0 = 1 + 8 * LB_BINS;
+constexpr int PKEY_LONG = PKEY_BAND0 + LB_BINS * GAP_BINS;
 constexpr int PKEYS = PKEY_LONG + 1;
 
 /* Key of a pair of lengths la, lb; for key 0, *result holds the answer
  * (the gap rule, csrc/myers.c:137, and the empty rule, csrc/myers.c:90-99). */
-BL_HD int pair_key(int la, int lb, int k, int bin_width, int &result)
+BL_HD int pair_key(int la, int lb, int k, int bin_width, int band, int &result)
 {
     const int mn = la < lb ? la : lb, mx = la < lb ? lb : la;
     if (k >= 0 && mx - mn > k) { result = k + 1; return 0; }
     if (mn == 0) { result = mx; return 0; }
     const int w = (mn + 31) >> 5;                 /* 32-row units of the pattern */
     if (w > 8) return PKEY_LONG;
-    const int bin = mx / bin_width;
-    return 1 + (w - 1) * LB_BINS + (bin < LB_BINS - 1 ? bin : LB_BINS - 1);
+    int bin = mx / bin_width;
+    bin = bin < LB_BINS - 1 ? bin : LB_BINS - 1;
+    if (band > 0 && w >= band && mx - mn <= 63) return PKEY_BAND0 + bin * GAP_BINS + (mx - mn) / 4;
+    return 1 + (w - 1) * LB_BINS + bin;
 }
 
-/* 32-row units of the tile whose largest key is `key`. */
+/* Band tiles: those whose largest key is a band key. */
+BL_HD bool key_band(int key) { return key >= PKEY_BAND0 && key < PKEY_LONG; }
+
+/* 32-row units of a whole-pattern tile whose largest key is `key`. */
 BL_HD int key_units(int key) { return (key - 1) / LB_BINS + 1; }
 
 /* Pair i of packed sets through the register-resident lane code (shorter

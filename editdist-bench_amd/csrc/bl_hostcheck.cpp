@@ -115,18 +115,18 @@ static int w32_pair(int w, const u8 *ba, const int64_t *oa, const u8 *bb, const 
 /* The batched pairs pipeline emulated step by step with the kernels' shared
  * code and the same index arithmetic: keys from the offsets -> per-chunk
  * bucket scan (descending keys) -> chunk / tile prefix sums -> scatter, each
- * 256-pair block ranking its pairs per key and reserving the key's range with
+ * 1024-pair block ranking its pairs per key and reserving the key's range with
  * one add -> tiles taken in order, the chunk of each tile found by the same
  * bisection. Every index is checked, and so is the tile invariant (no pair in
  * a tile has a shorter string of more 32-row units than its first pair). chunk:
- * pairs per chunk (a multiple of 256 on the GPU; any multiple of 256 here).
+ * pairs per chunk (a multiple of 1024, the bucketing block, as on the GPU).
  * Returns 0, or a negative code naming the violated check. */
 template <typename BandTile>
 static int blh_pairs_batched_core(const u8 *ba, const int64_t *oa, const u8 *bb,
                                   const int64_t *ob, int64_t n, int32_t *out, int trim, int k,
                                   int bin_width, int64_t chunk, int band, BandTile band_tile)
 {
-    if (chunk < 256 || chunk % 256) return -9;
+    if (chunk < 1024 || chunk % 1024) return -9;
     const int64_t nchunks = n ? (n + chunk - 1) / chunk : 1;
     std::vector<int> key((size_t)n);
     std::vector<uint64_t> hist((size_t)(nchunks * PKEYS), 0), cursor(hist.size(), 0);
@@ -138,7 +138,8 @@ static int blh_pairs_batched_core(const u8 *ba, const int64_t *oa, const u8 *bb,
     };
     for (int64_t i = 0; i < n; i++) {
         int result = 0;
-        const int kk = pair_key((int)(oa[i + 1] - oa[i]), (int)(ob[i + 1] - ob[i]), k, bin_width, result);
+        const int kk = pair_key((int)(oa[i + 1] - oa[i]), (int)(ob[i + 1] - ob[i]), k, bin_width,
+                                band, result);
         if (kk < 0 || kk >= PKEYS) return -10;
         if (kk == 0) out[i] = result;
         key[(size_t)i] = kk;
@@ -161,8 +162,8 @@ static int blh_pairs_batched_core(const u8 *ba, const int64_t *oa, const u8 *bb,
     }
     const int64_t total = (int64_t)cbase[(size_t)nchunks];
     std::vector<int32_t> sorted((size_t)total, -1);
-    for (int64_t b0 = 0; b0 < n; b0 += 256) {             /* one scatter block */
-        const int64_t b1 = b0 + 256 < n ? b0 + 256 : n;
+    for (int64_t b0 = 0; b0 < n; b0 += 1024) {            /* one scatter block */
+        const int64_t b1 = b0 + 1024 < n ? b0 + 1024 : n;
         const int64_t c = b0 / chunk;
         std::vector<uint32_t> cnt(PKEYS, 0), rank((size_t)(b1 - b0), 0);
         for (int64_t i = b0; i < b1; i++)
@@ -210,13 +211,17 @@ static int blh_pairs_batched_core(const u8 *ba, const int64_t *oa, const u8 *bb,
         const int64_t ce = (int64_t)cbase[(size_t)c + 1];
         const int64_t s1 = s0 + 64 < ce ? s0 + 64 : ce;
         if (s0 >= ce) return -14;
-        const int w = key_units(key[(size_t)sorted[(size_t)s0]]);
-        if (w < 1 || w > 8) return -15;
-        for (int64_t q = s0; q < s1; q++)
-            if (((minlen(sorted[(size_t)q]) + 31) >> 5) > w) return -16;
-        if (band > 0 && w >= band) {
+        const int key0 = key[(size_t)sorted[(size_t)s0]];
+        if (key_band(key0)) {
+            if (band <= 0) return -19;
             band_tile(std::vector<int32_t>(sorted.begin() + s0, sorted.begin() + s1));
             continue;
+        }
+        const int w = key_units(key0);
+        if (w < 1 || w > 8) return -15;
+        for (int64_t q = s0; q < s1; q++) {
+            if (key_band(key[(size_t)sorted[(size_t)q]])) return -20;
+            if (((minlen(sorted[(size_t)q]) + 31) >> 5) > w) return -16;
         }
         for (int64_t q = s0; q < s1; q++) {
             const int64_t i = sorted[(size_t)q];

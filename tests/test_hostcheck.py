@@ -83,7 +83,7 @@ def test_peq_lane_code_matches_golden(hc, golden, force_long):
                                           err_msg=f"{name} k={k}")
 
 
-@pytest.mark.parametrize("bin_width,chunk", [(1, 256), (16, 512), (16, 1024), (64, 1 << 20)])
+@pytest.mark.parametrize("bin_width,chunk", [(1, 1024), (16, 2048), (16, 3072), (64, 1 << 20)])
 def test_batched_pairs_pipeline_emulated(hc, golden, bin_width, chunk):
     """keys from the offsets -> per-chunk bucket scan -> block-ranked scatter ->
     tiles, as the persistent kernel walks them, with every index and the tile
