@@ -404,9 +404,10 @@ __device__ void cdist_tile_strided(const u32 *lo32, const u64 *__restrict__ gpeq
 }
 
 /* SWAR mode, for small thresholds (k <= 3) and queries of at least 8 bytes:
- * each lane carries four choices of a 256-choice window (w + l, w + 64 + l,
- * w + 128 + l, w + 192 + l -- four stripe groups, so every load is coalesced)
- * in the four bytes of one u32, each byte holding pattern rows 0..7 of its
+ * each lane carries four consecutive choices of a 256-choice window (w + 4l ..
+ * w + 4l + 3, whose first text dwords are adjacent in the stripes, so a lane
+ * reads them with one 16-byte load) in the four bytes of one u32, each byte
+ * holding pattern rows 0..7 of its
  * choice. Four text columns of the Myers step run on all four at once with
  * byte-segmented arithmetic (carries and shifts kept inside each byte; row 0
  * of every segment gets the +1 of a global alignment), then the final-diagonal
@@ -429,18 +430,33 @@ __device__ __forceinline__ void swar_window(WaveLds &L, int m, const u32 *__rest
 {
     const int lane = lane_id();
     const u32 kk1 = (u32)k + 1u;            /* k + 1 <= 4 fits every output type */
+    /* lane l takes choices w + 4l .. w + 4l + 3: one stripe group ((w >> 6) +
+     * l / 16, w a multiple of 256), whose first dwords sit side by side, so
+     * the text, the lengths and (in full windows) the results move as one
+     * 16-byte (results: 4-element) access per lane */
+    const int64_t cl = w + 4 * lane;
     u32 d[4];
     int n[4];
     bool valid[4];
+    if (FULL) {
+        const uint4 t = *(const uint4 *)(words + gbase[(w >> 6) + (lane >> 4)] + 4 * (lane & 15));
+        const int4 l4 = *(const int4 *)(lw + 4 * lane);
+        d[0] = t.x; d[1] = t.y; d[2] = t.z; d[3] = t.w;
+        n[0] = l4.x; n[1] = l4.y; n[2] = l4.z; n[3] = l4.w;
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-        const u32 *gp = words + gbase[(w >> 6) + s];          /* a scalar load per group */
-        valid[s] = FULL || w + 64 * s + lane < c1;
-        n[s] = valid[s] ? lw[64 * s + lane] : 0;
-        d[s] = valid[s] ? gp[lane] : 0u;
+        for (int s = 0; s < 4; s++) valid[s] = true;
+    } else {
+        const u32 *gp = cl < c1 ? words + gbase[(w >> 6) + (lane >> 4)] + 4 * (lane & 15) : words;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            valid[s] = cl + s < c1;
+            n[s] = valid[s] ? lw[4 * lane + s] : 0;
+            d[s] = valid[s] ? gp[s] : 0u;
+        }
     }
-    int bound[4];
-    swar_bounds4(L.lo8, d, m, n, bound, 1);
+    u32 Pv, Mv;
+    swar_steps4(L.lo8, d, Pv, Mv, 1);
+    const u32 over = swar_over4(Pv, Mv, m, n, k);
     bool open[4];
     bool any = false;
 #pragma unroll
@@ -448,17 +464,30 @@ __device__ __forceinline__ void swar_window(WaveLds &L, int m, const u32 *__rest
         const int gap = n[s] > m ? n[s] - m : m - n[s];
         /* out of band (an empty choice has gap m >= 8 > k), or the bound at
          * column 4 exceeds k: the answer is k + 1 */
-        const bool decided = gap > k || bound[s] > k;
-        /* (k + 1 is never a match: a match list gets nothing) */
-        if (DENSE && valid[s] && decided) ow[64 * s + lane] = (OutT)kk1;
+        const bool decided = gap > k || ((over >> (8 * s + 7)) & 1u);
         open[s] = valid[s] && !decided;
         any = any || open[s];
+        /* (k + 1 is never a match: a match list gets nothing) */
+        if (DENSE && !FULL && valid[s] && decided) ow[4 * lane + s] = (OutT)kk1;
+    }
+    if (DENSE && FULL) {
+        /* all four get k + 1; an open one is overwritten by its rescan later
+         * in this wave's program order */
+        OutT *p = ow + 4 * lane;
+        if (((uintptr_t)ow & (4 * sizeof(OutT) - 1)) == 0) {
+            if (sizeof(OutT) == 1) *(u32 *)p = kk1 * 0x01010101u;
+            else if (sizeof(OutT) == 2) *(uint2 *)p = make_uint2(kk1 * 0x00010001u, kk1 * 0x00010001u);
+            else *(uint4 *)p = make_uint4(kk1, kk1, kk1, kk1);
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; s++) p[s] = (OutT)kk1;
+        }
     }
     if (wave_ballot(any) == 0) return;
 #pragma unroll
     for (int s = 0; s < 4; s++) {
         const u64 sm = wave_ballot(open[s]);
-        if (open[s]) L.open[ns + rank_below(sm)] = orow + w + 64 * s + lane;
+        if (open[s]) L.open[ns + rank_below(sm)] = orow + cl + s;
         ns = uni(ns + __popcll(sm));
     }
     wave_lds_sync();

@@ -972,10 +972,10 @@ BL_HD u32 swar_add(u32 x, u32 y)
  * bound[s] = the final-diagonal bound D[4 + m - n[s]][4] when that row lies
  * in 1..8 (exact: rows 0..7 evolve as in the whole scan), else -1. */
 template <typename Lo8Ptr>
-BL_HD void swar_bounds4(Lo8Ptr lo8, const u32 (&d)[4], int m, const int (&n)[4], int (&bound)[4],
-                        int stride = 4)
+BL_HD void swar_steps4(Lo8Ptr lo8, const u32 (&d)[4], u32 &Pv, u32 &Mv, int stride)
 {
-    u32 Pv = ~0u, Mv = 0u;
+    Pv = ~0u;
+    Mv = 0u;
 #pragma unroll
     for (int u = 0; u < 4; u++) {
         const u32 E = (u32)lo8[stride * ((d[0] >> (8 * u)) & 0xFFu)] |
@@ -992,6 +992,14 @@ BL_HD void swar_bounds4(Lo8Ptr lo8, const u32 (&d)[4], int m, const int (&n)[4],
         Pv = bop3<tt(TA | ~(TB | TC))>(Mhs, Xv, Phs);               /* Mhs | ~(Xv | Phs) */
         Mv = Phs & Xv;
     }
+}
+
+template <typename Lo8Ptr>
+BL_HD void swar_bounds4(Lo8Ptr lo8, const u32 (&d)[4], int m, const int (&n)[4], int (&bound)[4],
+                        int stride = 4)
+{
+    u32 Pv, Mv;
+    swar_steps4(lo8, d, Pv, Mv, stride);
 #pragma unroll
     for (int s = 0; s < 4; s++) {
         const int i0 = 4 + m - n[s];
@@ -999,6 +1007,30 @@ BL_HD void swar_bounds4(Lo8Ptr lo8, const u32 (&d)[4], int m, const int (&n)[4],
         const int b = 4 + popc((u32)((Pv >> (8 * s)) & mk)) - popc((u32)((Mv >> (8 * s)) & mk));
         bound[s] = (i0 >= 1 && i0 <= 8) ? b : -1;
     }
+}
+
+/* The four bounds of swar_bounds4 tested against k <= 3 at once: bit 7 of
+ * byte s is set when bound[s] > k, for every choice whose row 4 + m - n[s]
+ * lies in 1..8 (the others have |m - n| >= 4 > k, decided by the gap). Per
+ * byte: a mask of the rows below that one, popcounts p of Pv and q of Mv
+ * under it (<= 8 each), then 128 + p - q compared with 124 + k by one add
+ * (no borrow or carry crosses a byte). */
+BL_HD u32 swar_over4(u32 Pv, u32 Mv, int m, const int (&n)[4], int k)
+{
+    u32 mk = 0;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        int r = 4 + m - n[s];
+        r = r < 0 ? 0 : r > 8 ? 8 : r;
+        mk |= (u32)((1u << r) - 1u) << (8 * s);
+    }
+    auto pc8 = [](u32 x) {                       /* popcount of every byte */
+        x = x - ((x >> 1) & 0x55555555u);
+        x = (x & 0x33333333u) + ((x >> 2) & 0x33333333u);
+        return (x + (x >> 4)) & 0x0F0F0F0Fu;
+    };
+    const u32 t = (pc8(Pv & mk) | 0x80808080u) - pc8(Mv & mk);
+    return (t + (u32)(3 - k) * 0x01010101u) & 0x80808080u;
 }
 
 /* ------------------------------------------------------------ long strings */

@@ -381,6 +381,22 @@ int blh_pairs_batched(const u8 *ba, const int64_t *oa, const u8 *bb, const int64
     return blh_pairs_batched_band(ba, oa, bb, ob, n, out, trim, k, bin_width, chunk, 0, nullptr);
 }
 
+/* The same filter's SWAR test bound > k (k <= 3): bit 7 of byte s. */
+uint32_t blh_swar_over(const u8 *q, int m, const u8 *t, const int *n, int k)
+{
+    std::vector<u8> lo8(256, 0);
+    for (int i = 0; i < 8 && i < m; i++) lo8[q[i]] |= (u8)(1u << i);
+    u32 d[4];
+    int nn[4];
+    for (int s = 0; s < 4; s++) {
+        memcpy(&d[s], t + 4 * s, 4);
+        nn[s] = n[s];
+    }
+    u32 Pv, Mv;
+    swar_steps4(lo8.data(), d, Pv, Mv, 1);
+    return swar_over4(Pv, Mv, m, nn, k);
+}
+
 /* The 4-choice byte-segment filter: bounds of choices t[0..3] (first 4 bytes
  * each, lengths n[0..3]) against a pattern of m >= 8 bytes. */
 void blh_swar_bounds(const u8 *q, int m, const u8 *t, const int *n, int *bound)

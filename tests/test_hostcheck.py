@@ -257,9 +257,13 @@ def _dp_cell(a: bytes, b: bytes, i: int, j: int) -> int:
 def test_swar_filter_bounds_equal_the_dp(hc):
     """The 4-choice filter's bound for each byte segment is exactly the DP cell
     D[4 + m - n][4] (its row within 1..8), whatever the other three choices
-    are: no carry or shift leaks between segments."""
+    are: no carry or shift leaks between segments; and the byte-parallel test
+    of all four bounds against k = 0..3 agrees with that cell."""
     hc.blh_swar_bounds.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    hc.blh_swar_over.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
+                                 ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    hc.blh_swar_over.restype = ctypes.c_uint32
     rng = np.random.default_rng(44)
     checked = 0
     for trial in range(3000):
@@ -282,10 +286,14 @@ def test_swar_filter_bounds_equal_the_dp(hc):
         nn = (ctypes.c_int * 4)(*ns)
         out = (ctypes.c_int * 4)()
         hc.blh_swar_bounds(q, m, packed, nn, out)
+        overs = [hc.blh_swar_over(q, m, packed, nn, k) for k in range(4)]
         for s in range(4):
             i0 = 4 + m - ns[s]
             if 1 <= i0 <= 8:
-                assert out[s] == _dp_cell(q, texts[s], i0, 4), (trial, s, m, ns[s])
+                cell = _dp_cell(q, texts[s], i0, 4)
+                assert out[s] == cell, (trial, s, m, ns[s])
+                for k in range(4):        # the SWAR test of the same bound against k
+                    assert bool((overs[k] >> (8 * s + 7)) & 1) == (cell > k), (trial, s, k)
                 checked += 1
             else:
                 assert out[s] == -1
