@@ -214,9 +214,16 @@ def main():
             ws = hip.Workspace()
             step = make_step(form, tuned, ws)
             if not args.no_graph:
+                # graph replay or eager launches, whichever is faster: a graph
+                # saves per-launch host work on multi-kernel pipelines but adds
+                # ~10 us, which dominates a batch that runs in ~10 us
                 g = hip.Graph(step, stream)
-                step = g.launch
-                extra["graph"] = True
+                we, _ = timed_steps(step, ev, 20, 2)
+                wg, _ = timed_steps(g.launch, ev, 20, 2)
+                extra["probe_ms_per_step"] = {"eager": round(we * 50, 4), "graph": round(wg * 50, 4)}
+                extra["graph"] = bool(wg <= we)
+                if extra["graph"]:
+                    step = g.launch
         wall, per = timed_steps(step, ev, args.steps, args.warmup)
         ms_step = wall * 1e3 / args.steps
         measured[form] = {
