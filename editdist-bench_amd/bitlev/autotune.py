@@ -7,8 +7,9 @@ tail of short tiles can rank close shapes differently), keeps the fastest and
 caches it per device and workload signature in a JSON file
 ($BITLEV_TUNE_CACHE, default ~/.cache/bitlev/autotune.json).
 
-cdist: lane mode (compacted lockstep passes, strided lanes, or for k <= 3 the
-four-choice byte filter), persistent waves per CU, choices per tile. pairs: direct (one launch) or bucketed, and
+cdist: lane mode (compacted lockstep passes, strided lanes, gap windows for
+k >= 0, or for k <= 3 the four-choice byte filter), persistent waves per CU,
+choices per tile. pairs: direct (one launch) or bucketed, and
 for bucketed persistent waves per CU, workgroup size, text-length bucket
 width, bucketing chunk; whether to trim common affixes (which never changes a
 distance); then, on the two best bucketed shapes, the smallest tile (in
@@ -32,6 +33,8 @@ CDIST_SPACE = {"mode": ["strided", "queue"], "waves_per_cu": [8, 16, 32],
                "tile": [2048, 8192, 32768]}
 # the four-choice byte filter, only where it can decide (k <= 3)
 CDIST_SWAR_SPACE = {"mode": ["swar"], "waves_per_cu": [8, 16, 32], "tile": [2048, 8192, 32768]}
+# gap windows (any threshold): the length test on 256-choice windows, the rest staged
+CDIST_GAP_SPACE = {"mode": ["gap"], "waves_per_cu": [8, 16, 32], "tile": [2048, 8192, 32768]}
 PAIRS_SPACE = {"waves_per_cu": [8, 16, 32], "block": [64, 256], "bin_width": [8, 16, 32],
                "chunk": [16384, 65536, 262144], "trim": [True, False]}
 # direct mode (one launch, no bucketing): only the launch shape matters
@@ -161,6 +164,8 @@ def tune_cdist(Q: hip.Packed, C: hip.Packed, out: hip.DeviceArray, k: int = -1, 
     cands = list(_grid(space or CDIST_SPACE))
     if space is None and 0 <= k <= 3:
         cands += list(_grid(CDIST_SWAR_SPACE))
+    if space is None and k >= 0:
+        cands += list(_grid(CDIST_GAP_SPACE))
     for p in cands:
         ms = _time(lambda: hip.cdist_batched_into(Qs, C, out, k, stream, ws, p), stream)
         res.append((ms, p))

@@ -245,7 +245,7 @@ def _cdist_params(params):
     p = CdistParams()
     p.waves_per_cu = int(params.get("waves_per_cu", 0))
     p.tile = int(params.get("tile", 0))
-    p.reserved[0] = {"strided": 1, "swar": 2}.get(params.get("mode"), 0)
+    p.reserved[0] = {"strided": 1, "swar": 2, "gap": 3}.get(params.get("mode"), 0)
     p.reserved[1] = int(params.get("grab", 0))
     return ctypes.byref(p)
 

@@ -422,7 +422,7 @@ __device__ void cdist_tile_strided(const u32 *lo32, const u64 *__restrict__ gpeq
  * open are pushed on the wave's open stack as their output index orow + c
  * (the caller keeps fewer than 64 there between windows, so it never holds
  * more than 319). */
-template <bool FULL, bool DENSE, typename OutT>
+template <bool FULL, bool FILTER, bool DENSE, typename OutT>
 __device__ __forceinline__ void swar_window(WaveLds &L, int m, const u32 *__restrict__ words,
                                             const int64_t *__restrict__ gbase,
                                             const int32_t *__restrict__ lw, OutT *ow, int64_t w,
@@ -439,32 +439,39 @@ __device__ __forceinline__ void swar_window(WaveLds &L, int m, const u32 *__rest
     int n[4];
     bool valid[4];
     if (FULL) {
-        const uint4 t = *(const uint4 *)(words + gbase[(w >> 6) + (lane >> 4)] + 4 * (lane & 15));
+        if (FILTER) {
+            const uint4 t = *(const uint4 *)(words + gbase[(w >> 6) + (lane >> 4)] + 4 * (lane & 15));
+            d[0] = t.x; d[1] = t.y; d[2] = t.z; d[3] = t.w;
+        }
         const int4 l4 = *(const int4 *)(lw + 4 * lane);
-        d[0] = t.x; d[1] = t.y; d[2] = t.z; d[3] = t.w;
         n[0] = l4.x; n[1] = l4.y; n[2] = l4.z; n[3] = l4.w;
 #pragma unroll
         for (int s = 0; s < 4; s++) valid[s] = true;
     } else {
-        const u32 *gp = cl < c1 ? words + gbase[(w >> 6) + (lane >> 4)] + 4 * (lane & 15) : words;
+        const u32 *gp = FILTER && cl < c1 ? words + gbase[(w >> 6) + (lane >> 4)] + 4 * (lane & 15)
+                                          : words;
 #pragma unroll
         for (int s = 0; s < 4; s++) {
             valid[s] = cl + s < c1;
             n[s] = valid[s] ? lw[4 * lane + s] : 0;
-            d[s] = valid[s] ? gp[s] : 0u;
+            d[s] = FILTER && valid[s] ? gp[s] : 0u;
         }
     }
-    u32 Pv, Mv;
-    swar_steps4(L.lo8, d, Pv, Mv, 1);
-    const u32 over = swar_over4(Pv, Mv, m, n, k);
+    u32 over = 0;
+    if (FILTER) {
+        u32 Pv, Mv;
+        swar_steps4(L.lo8, d, Pv, Mv, 1);
+        over = swar_over4(Pv, Mv, m, n, k);
+    }
     bool open[4];
     bool any = false;
 #pragma unroll
     for (int s = 0; s < 4; s++) {
         const int gap = n[s] > m ? n[s] - m : m - n[s];
-        /* out of band (an empty choice has gap m >= 8 > k), or the bound at
-         * column 4 exceeds k: the answer is k + 1 */
-        const bool decided = gap > k || ((over >> (8 * s + 7)) & 1u);
+        /* out of band, or (FILTER) the bound at column 4 exceeds k: the answer
+         * is k + 1 (FILTER: an empty choice has gap m >= 8 > k; without the
+         * filter one with m <= k stays open and its scan returns m) */
+        const bool decided = gap > k || (FILTER && ((over >> (8 * s + 7)) & 1u));
         open[s] = valid[s] && !decided;
         any = any || open[s];
         /* (k + 1 is never a match: a match list gets nothing) */
@@ -558,29 +565,39 @@ __global__ __launch_bounds__(256) void cdist_batched_kernel(
         }
         /* (the filter's rescan of 8-word queries would cost the whole launch its
          * occupancy: queries above 256 bytes use the other modes) */
-        if (NWMAX <= 4 && mode == 2 && k >= 0 && k <= 3 && m >= 8 && (c0 & 255) == 0) {
-            /* 256-choice windows through the four-choice filter; the open
-             * stack is emptied to below 64 between windows */
+        const bool filt = NWMAX <= 4 && mode == 2 && k >= 0 && k <= 3 && m >= 8;
+        const bool gapw = mode == 3 && k >= 0;
+        if ((filt || gapw) && (c0 & 255) == 0) {
+            /* 256-choice windows through the four-choice filter (or the gap
+             * test alone); the open stack is emptied to below 64 between windows */
             int64_t w = c0;
-            auto window = [&](auto full) {
+            auto window = [&](auto full, auto filter) {
                 constexpr bool F = decltype(full)::value;
+                constexpr bool FI = decltype(filter)::value;
                 int64_t w0 = w;
                 if (!Sink::kDense)
-                    swar_window<F, false>(L, m, words, gbase, clen + w0, (u8 *)nullptr, w0, c1, k,
-                                          orow, ns);
+                    swar_window<F, FI, false>(L, m, words, gbase, clen + w0, (u8 *)nullptr, w0, c1,
+                                              k, orow, ns);
                 else if (dense_dtype(sink) == DT_UINT8)
-                    swar_window<F, true>(L, m, words, gbase, clen + w0,
-                                         (u8 *)dense_out(sink) + orow + w0, w0, c1, k, orow, ns);
+                    swar_window<F, FI, true>(L, m, words, gbase, clen + w0,
+                                             (u8 *)dense_out(sink) + orow + w0, w0, c1, k, orow, ns);
                 else if (dense_dtype(sink) == DT_UINT16)
-                    swar_window<F, true>(L, m, words, gbase, clen + w0,
-                                         (uint16_t *)dense_out(sink) + orow + w0, w0, c1, k, orow, ns);
+                    swar_window<F, FI, true>(L, m, words, gbase, clen + w0,
+                                             (uint16_t *)dense_out(sink) + orow + w0, w0, c1, k, orow,
+                                             ns);
                 else
-                    swar_window<F, true>(L, m, words, gbase, clen + w0,
-                                         (int32_t *)dense_out(sink) + orow + w0, w0, c1, k, orow, ns);
+                    swar_window<F, FI, true>(L, m, words, gbase, clen + w0,
+                                             (int32_t *)dense_out(sink) + orow + w0, w0, c1, k, orow,
+                                             ns);
                 while (ns >= 64) drain(64);
             };
-            for (; w + 256 <= c1; w += 256) window(std::true_type());
-            if (w < c1) window(std::false_type());
+            if (filt) {
+                for (; w + 256 <= c1; w += 256) window(std::true_type(), std::true_type());
+                if (w < c1) window(std::false_type(), std::true_type());
+            } else {
+                for (; w + 256 <= c1; w += 256) window(std::true_type(), std::false_type());
+                if (w < c1) window(std::false_type(), std::false_type());
+            }
             while (ns > 0) drain(ns < 64 ? ns : 64);
         }
         else {
@@ -1034,7 +1051,7 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
     /* tiles start on stripe groups (multiples of 64 choices; of 256 for the
      * four-choice filter's windows) */
     const int64_t tile = (prm && prm->tile > 0 ? (prm->tile + 255) / 256 * 256 : 8192);
-    const int mode = prm && prm->reserved[0] >= 1 && prm->reserved[0] <= 2 ? prm->reserved[0] : 0;
+    const int mode = prm && prm->reserved[0] >= 1 && prm->reserved[0] <= 3 ? prm->reserved[0] : 0;
     const int grab = prm && prm->reserved[1] > 0 ? prm->reserved[1] : 1;
     const int nwmax = cdist_nwmax(max_qlen);
     const size_t lds = sizeof(WaveLds) * 4;

@@ -142,7 +142,9 @@ typedef struct {
     int64_t tile;          /* choices per (query, block) tile (default 8192) */
     int reserved[4];       /* [0]: lane mode -- 0 compacted passes (default), 1 strided
                               lanes, 2 four-choice byte filter for k <= 3 (queries of
-                              8 to 256 bytes; other queries run compacted);
+                              8 to 256 bytes; other queries run compacted), 3 gap
+                              windows for k >= 0 (the length test on 256 choices at
+                              a time, the in-band ones scanned 64 at a time);
                               [1]: tiles per work-queue ticket (default 1) */
 } bl_cdist_params;
 

@@ -85,7 +85,8 @@ def test_every_cdist_launch_shape_gives_the_same_results(gen, k):
     pq, pc = W.cdist(gen, 12, 5000, seed=3)
     Q, C = hip.Packed(*pq), hip.Packed(*pc)
     want = O.cross(pq, pc, k=k, threads=8)
-    space = {"mode": ["strided", "queue", "swar"], "waves_per_cu": [4, 32], "tile": [64, 1000, 8192]}
+    space = {"mode": ["strided", "queue", "swar", "gap"], "waves_per_cu": [4, 32],
+             "tile": [64, 1000, 8192]}
     for p in autotune._grid(space):
         out = hip.DeviceArray.empty((Q.n, C.n), np.uint8)
         hip.cdist_batched_into(Q, C, out, k, None, None, p)
@@ -124,6 +125,44 @@ def test_cdist_stage_escalation_paths():
             np.testing.assert_array_equal(out.get(), want, err_msg=f"k={k} {p}")
 
 
+def test_gap_windows_edge_cases():
+    """Gap windows: queries no longer than their threshold (an empty choice is
+    then in band and its scan returns m), empty choices, lengths around the
+    gap, windows cut by the tile end, exact queries (which run another mode),
+    every output dtype."""
+    rng = np.random.default_rng(31)
+    Qs = [b"a", b"ab", b"abc", bytes(rng.integers(97, 101, 40).astype(np.uint8)),
+          bytes(rng.integers(97, 101, 200).astype(np.uint8))]
+    Cs = []
+    for i in range(3001):
+        q = Qs[i % len(Qs)]
+        r = i % 7
+        if r == 0:
+            Cs.append(b"")
+        elif r < 4:
+            b = bytearray(q[: max(0, len(q) - int(rng.integers(0, 4)))])
+            for _ in range(int(rng.integers(0, 3))):
+                if b:
+                    b[int(rng.integers(0, len(b)))] = int(rng.integers(97, 101))
+            Cs.append(bytes(b))
+        else:
+            Cs.append(bytes(rng.integers(97, 101, int(rng.integers(0, 60))).astype(np.uint8)))
+    pq, pc = O.pack(Qs), O.pack(Cs)
+    Q, C = hip.Packed(*pq), hip.Packed(*pc)
+    exact = O.cross(pq, pc, threads=8)
+    for k in (0, 1, 3, 5, 8, 40):
+        for tile in (256, 1024):
+            for dt in (np.uint8, np.uint16, np.int32):
+                out = hip.DeviceArray.empty((Q.n, C.n), dt)
+                hip.cdist_batched_into(Q, C, out, k, None, None, {"mode": "gap", "tile": tile})
+                np.testing.assert_array_equal(out.get(), O.saturate(np.minimum(exact, k + 1), dt),
+                                              err_msg=f"k={k} tile={tile} {dt}")
+    kv = np.array([-1, 0, 4, -1, 8], np.int32)
+    out = hip.DeviceArray.empty((Q.n, C.n), np.int32)
+    hip.cdist_batched_into(Q, C, out, -1, None, None, {"mode": "gap"}, kvec=hip.DeviceBuffer.from_host(kv))
+    np.testing.assert_array_equal(out.get(), np.where(kv[:, None] < 0, exact, np.minimum(exact, kv[:, None] + 1)))
+
+
 def test_filter_mode_with_every_choice_open():
     """The four-choice filter leaves every choice open when all are within k
     of the query (here: copies with at most one substitution, k = 3): the
@@ -151,7 +190,7 @@ def test_filter_mode_with_every_choice_open():
             np.testing.assert_array_equal(out.get(), want, err_msg=f"{kq} {p}")
 
 
-@pytest.mark.parametrize("mode", ["swar", "queue", "strided"])
+@pytest.mark.parametrize("mode", ["swar", "queue", "strided", "gap"])
 def test_match_list_equals_the_dense_result(mode):
     """bl_cdist_matches returns exactly the cells the dense cdist puts at or
     below k (here with per-query thresholds, one of them negative: that query
