@@ -500,6 +500,125 @@ __device__ __forceinline__ void swar_window(WaveLds &L, int m, const u32 *__rest
     wave_lds_sync();
 }
 
+/* Gap mode (k >= 0): 256-choice windows settle the choices outside the band
+ * (one length load and one result store per four choices) and stack the rest
+ * (as output indices on the open stack); lanes scan the stacked choices'
+ * 32-row stage in lockstep 4-column groups, and after every group the lanes
+ * whose choice is decided take the next one from the stack, so lanes never
+ * wait for a slow neighbour and no scan state is ever parked. Choices the
+ * 32-row stage cannot decide go to the escalation stack and through the
+ * 64-row and whole stages 64 at a time. */
+template <int NW, typename Sink, typename Window>
+__device__ void cdist_tile_refill(WaveLds &L, const u64 *__restrict__ gpeq, int m,
+                                  const u32 *__restrict__ words, const int64_t *__restrict__ gbase,
+                                  const int32_t *__restrict__ clen, int64_t c0, int64_t c1,
+                                  const Sink &sink, int64_t orow, int k, Window window, int &ns)
+{
+    const int lane = lane_id();
+    const bool whole = m <= 32;
+    int64_t w = c0;
+    int ne = 0;
+    bool act = false;
+    int64_t c = 0;
+    int n = 0, j = 0;
+    u32 Pv = ~0u, Mv = 0u, c4 = 0u, c4n = 0u;
+    const u32 *tp = words;
+    for (;;) {
+        /* ---- feed the stack with windows while it is short */
+        while (ns < 64 && w < c1) {
+            window(w);
+            w += 256;
+        }
+        /* ---- free lanes take stacked choices */
+        const u64 fr = wave_ballot(!act);
+        const int take = __popcll(fr) < ns ? __popcll(fr) : ns;
+        if (take > 0) {
+            const int r = rank_below(fr);
+            if (!act && r < take) {
+                c = L.open[ns - take + r] - orow;
+                n = clen[c];
+                j = 0;
+                Pv = ~0u;
+                Mv = 0u;
+                tp = words + gbase[c >> 6] + (c & 63);
+                act = true;
+                if (n == 0) {                    /* (in band only when m <= k) */
+                    sink.put(orow + c, m, k);
+                    act = false;
+                } else {
+                    c4 = tp[0];
+                    c4n = n > 4 ? tp[64] : 0u;
+                }
+            }
+            ns = uni(ns - take);
+            wave_lds_sync();                     /* entries read before their slots are reused */
+        }
+        /* ---- escalations: a full wave of them, or the rest at the very end */
+        const u64 am0 = wave_ballot(act);
+        const bool done = am0 == 0 && ns == 0 && w >= c1;
+        if (ne >= 64 || (done && ne > 0)) {
+            const int cnt = ne < 64 ? ne : 64;
+            const int base = ne - cnt;
+            if (lane < cnt) {
+                const int64_t ce = c0 + L.esc[base + lane];
+                const StripeText text{words + gbase[ce >> 6] + (ce & 63)};
+                sink.put(orow + ce, peq_lane<NW, 1>((const u32 *)nullptr, gpeq, m, text, clen[ce], k), k);
+            }
+            ne = uni(base);
+            wave_lds_sync();
+            continue;
+        }
+        if (done) break;
+        if (am0 == 0) continue;
+        /* ---- one 4-column group of the 32-row stage */
+        bool escalate = false;
+        if (act) {
+            const int cnt = n - j < 4 ? n - j : 4;
+            const u32 cur = c4;
+            c4 = c4n;
+            const int nd = (j >> 2) + 2, last = (n - 1) >> 2;
+            c4n = nd <= last ? tp[64 * nd] : 0u;
+            u32 E[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) E[u] = L.lo32[(cur >> (8 * u)) & 0xFFu];
+            u32 P1[1] = {Pv}, M1[1] = {Mv};
+            if (cnt == 4) {
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const u32 e1[1] = {E[u]};
+                    column<u32, 1>(P1, M1, 1, e1);
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < 3; u++) {
+                    if (u < cnt) {
+                        const u32 e1[1] = {E[u]};
+                        column<u32, 1>(P1, M1, 1, e1);
+                    }
+                }
+            }
+            Pv = P1[0];
+            Mv = M1[0];
+            j += cnt;
+            /* the bound row j + m - n is within the 32 rows while the stage runs */
+            const int bound = diag_bound<u32, 1>(P1, M1, 1, j, m, n);
+            if (bound > k || (whole && j >= n)) {
+                sink.put(orow + c, bound > k ? k + 1 : bound, k);
+                act = false;
+            } else if (!whole && j + 4 + m - n > 32) {
+                escalate = true;
+                act = false;
+            }
+        }
+        const u64 em = wave_ballot(escalate);
+        if (em) {
+            if (escalate) L.esc[ne + rank_below(em)] = (int32_t)(c - c0);
+            ne = uni(ne + __popcll(em));
+            wave_lds_sync();
+        }
+    }
+}
+
 /* Persistent cdist over queries of at most 64*NWMAX bytes whose Peq tables
  * peq_build_kernel wrote to gpeq. Waves pull (choice block, query) tiles,
  * block-major, from the work-queue counter. */
@@ -571,7 +690,7 @@ __global__ __launch_bounds__(256) void cdist_batched_kernel(
             /* 256-choice windows through the four-choice filter (or the gap
              * test alone); the open stack is emptied to below 64 between windows */
             int64_t w = c0;
-            auto window = [&](auto full, auto filter) {
+            auto window = [&](auto full, auto filter, bool drain_open = true) {
                 constexpr bool F = decltype(full)::value;
                 constexpr bool FI = decltype(filter)::value;
                 int64_t w0 = w;
@@ -589,16 +708,23 @@ __global__ __launch_bounds__(256) void cdist_batched_kernel(
                     swar_window<F, FI, true>(L, m, words, gbase, clen + w0,
                                              (int32_t *)dense_out(sink) + orow + w0, w0, c1, k, orow,
                                              ns);
-                while (ns >= 64) drain(64);
+                if (drain_open)
+                    while (ns >= 64) drain(64);
             };
             if (filt) {
                 for (; w + 256 <= c1; w += 256) window(std::true_type(), std::true_type());
                 if (w < c1) window(std::false_type(), std::true_type());
+                while (ns > 0) drain(ns < 64 ? ns : 64);
             } else {
-                for (; w + 256 <= c1; w += 256) window(std::true_type(), std::false_type());
-                if (w < c1) window(std::false_type(), std::false_type());
+                /* gap windows feed lanes that refill after every group */
+                auto gap_window = [&](int64_t w0) {
+                    w = w0;
+                    if (w0 + 256 <= c1) window(std::true_type(), std::false_type(), false);
+                    else window(std::false_type(), std::false_type(), false);
+                };
+                cdist_tile_refill<NWMAX>(L, qpeq, m, words, gbase, clen, c0, c1, sink, orow, k,
+                                         gap_window, ns);
             }
-            while (ns > 0) drain(ns < 64 ? ns : 64);
         }
         else {
             if (mode == 1 || (m > 32 && k < 0))
