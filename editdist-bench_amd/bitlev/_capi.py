@@ -50,6 +50,7 @@ SIGNATURES = {
     "bl_pairs_class": [_int, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _int, _int, _int, _vp],
     "bl_stripes_layout": [_vp, _i64, _vp],
     "bl_stripes_build": [_vp, _vp, _i64, _vp, _vp, _vp, _vp],
+    "bl_gather_cols": [_vp, _vp, _vp, _i64, _i64, _int, _vp],
     "bl_cdist_class": [_int, _vp, _vp, _vp, _i64, _vp, _vp, _int, _int, _vp],
     "bl_long": [_int, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _int, _int, _int, _i64, _vp],
     "bl_cdist_workspace_size": [_i64, _int],

@@ -227,7 +227,9 @@ class Server:
         nq = sum(len(r.payload[1]) - 1 for r in reqs)
         inp = sum(_span(r.payload) + 8 * len(r.payload[1]) for r in reqs)
         mq = max(_longest(r.payload) for r in reqs)
-        return out + inp + 4 * nq + self._exec.cdist_workspace(nq, mq)
+        # a length-sorted set computes into a second matrix before the gather
+        sorted_set = getattr(self.sets.get(reqs[0].set_name), "col_order", None) is not None
+        return (2 * out if sorted_set else out) + inp + 4 * nq + self._exec.cdist_workspace(nq, mq)
 
     def _take_batch(self):
         """FIFO head and every later pending request it can fuse with."""
@@ -321,7 +323,21 @@ class _GpuExecutor:
         return _Slot(i)
 
     def resident(self, packed):
-        P = hip.Packed(*packed)
+        """A set kept on the device, sorted by length (a wave's 64 consecutive
+        choices then have similar lengths, so an exact search's lanes finish
+        together); searches gather their results back into the caller's
+        order through col_order (position of each original choice)."""
+        buf, off = packed
+        lens = np.diff(off)
+        order = np.argsort(lens, kind="stable")
+        if len(order) > 1 and bool(np.any(order[1:] < order[:-1])):
+            P = hip.Packed(*_gather_strings(buf, off, order))
+            inv = np.empty(len(order), np.int64)
+            inv[order] = np.arange(len(order), dtype=np.int64)
+            P.col_order = DeviceBuffer.from_host(inv)
+        else:
+            P = hip.Packed(*packed)
+            P.col_order = None
         P.stripes()                        # the search layout, built once per set
         call("bl_device_sync")
         return P
@@ -363,8 +379,13 @@ class _GpuExecutor:
             Q = hip.Packed(*pq, stream=slot.stream)
             kvec = DeviceBuffer.from_host(kv, slot.stream)
             out = DeviceArray.empty((Q.n, C.n), dt)
-            hip.cdist_batched_into(Q, C, out, -1, slot.stream, slot.workspace, self.cdist_params,
+            inv = getattr(C, "col_order", None)
+            tmp = DeviceArray.empty((Q.n, C.n), dt) if inv is not None else out
+            hip.cdist_batched_into(Q, C, tmp, -1, slot.stream, slot.workspace, self.cdist_params,
                                    kvec=kvec)
+            if inv is not None:
+                call("bl_gather_cols", tmp.ptr, inv.ptr, out.ptr, Q.n, C.n, DTYPE_CODE[dt],
+                     slot.stream)
             offs = np.cumsum([0] + nqs)
             views = [DeviceArray((q, C.n), dt, out._buf, out._off + int(o) * C.n * dt.itemsize)
                      for q, o in zip(nqs, offs)]
@@ -374,9 +395,12 @@ class _GpuExecutor:
 
 
 def _resident_bytes(P) -> int:
-    """Device bytes a resident set holds (packed buffer, offsets, stripes)."""
+    """Device bytes a resident set holds (packed buffer, offsets, stripes, and
+    the column order of a length-sorted set)."""
     b = getattr(P, "device_bytes", None)
-    return int(b) if b is not None else int(P.nbytes) + 8 * (P.n + 1)
+    b = int(b) if b is not None else int(P.nbytes) + 8 * (P.n + 1)
+    order = getattr(P, "col_order", None)
+    return b + (int(order.nbytes) if order is not None else 0)
 
 
 def _span(packed) -> int:
@@ -387,6 +411,25 @@ def _span(packed) -> int:
 def _longest(packed) -> int:
     off = packed[1]
     return int(np.diff(off).max()) if len(off) > 1 else 0
+
+
+def _gather_strings(buf, off, order, chunk=1 << 16):
+    """Packed host set of the strings in `order` (buf, off as packed)."""
+    lens = np.diff(off)[order]
+    noff = np.zeros(len(order) + 1, np.int64)
+    np.cumsum(lens, out=noff[1:])
+    nbuf = np.empty(max(int(noff[-1]), 1), np.uint8)
+    src = np.asarray(buf, np.uint8)
+    for a in range(0, len(order), chunk):
+        o = order[a:a + chunk]
+        ln = lens[a:a + chunk]
+        tot = int(ln.sum())
+        if tot == 0:
+            continue
+        starts = np.repeat(off[o] - noff[a:a + len(o)], ln)
+        pos = np.arange(noff[a], noff[a] + tot, dtype=np.int64)
+        nbuf[pos] = src[starts + pos]
+    return nbuf, noff
 
 
 def _concat(parts):

@@ -47,9 +47,53 @@ __global__ __launch_bounds__(256) void stripes_build_kernel(const u8 *__restrict
     for (int64_t j = 0; j < width; j++) words[gb + 64 * j + lane] = stripe_word(s, ln, j);
 }
 
+/* dst[r][i] = src[r][inv[i]]: one thread per column, rows in turn, so every
+ * row being gathered from stays in L2 and the writes are coalesced. */
+template <typename T>
+__global__ __launch_bounds__(256) void gather_cols_kernel(const T *__restrict__ src,
+                                                          const int64_t *__restrict__ inv,
+                                                          T *__restrict__ dst, int64_t rows,
+                                                          int64_t cols)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cols) return;
+    const int64_t s = inv[i];
+    for (int64_t r = 0; r < rows; r++) dst[r * cols + i] = src[r * cols + s];
+}
+
 }  // namespace
 
 extern "C" {
+
+int bl_gather_cols(const void *d_src, const int64_t *d_inv, void *d_dst, int64_t rows, int64_t cols,
+                   int dtype, void *stream)
+{
+    if (rows < 0 || cols < 0) return fail(BL_EINVAL, "bl_gather_cols: negative size");
+    if (rows == 0 || cols == 0) return BL_OK;
+    if (!d_src || !d_inv || !d_dst) return fail(BL_EINVAL, "bl_gather_cols: null pointer");
+    if (d_src == d_dst) return fail(BL_EINVAL, "bl_gather_cols: source and destination overlap");
+    const int64_t blocks = (cols + 255) / 256;
+    if (blocks > 0x7fffffff) return fail(BL_EINVAL, "bl_gather_cols: too many columns");
+    hipStream_t s = (hipStream_t)stream;
+    switch (dtype) {
+    case BL_INT32:
+        gather_cols_kernel<int32_t><<<(unsigned)blocks, 256, 0, s>>>(
+            (const int32_t *)d_src, d_inv, (int32_t *)d_dst, rows, cols);
+        break;
+    case BL_UINT16:
+        gather_cols_kernel<uint16_t><<<(unsigned)blocks, 256, 0, s>>>(
+            (const uint16_t *)d_src, d_inv, (uint16_t *)d_dst, rows, cols);
+        break;
+    case BL_UINT8:
+        gather_cols_kernel<uint8_t><<<(unsigned)blocks, 256, 0, s>>>(
+            (const uint8_t *)d_src, d_inv, (uint8_t *)d_dst, rows, cols);
+        break;
+    default:
+        return fail(BL_EINVAL, "bl_gather_cols: bad dtype %d", dtype);
+    }
+    BL_LAUNCHED("gather_cols_kernel");
+    return BL_OK;
+}
 
 int64_t bl_stripes_layout(const int64_t *h_off, int64_t n, int64_t *h_gbase)
 {

@@ -112,6 +112,14 @@ int64_t bl_stripes_layout(const int64_t *h_off, int64_t n, int64_t *h_gbase);
 int bl_stripes_build(const uint8_t *d_buf, const int64_t *d_off, int64_t n,
                      const int64_t *d_gbase, uint32_t *d_words, int32_t *d_len, void *stream);
 
+/* dst[r][i] = src[r][inv[i]] for r < rows, i < cols (dtype BL_INT32 /
+ * BL_UINT16 / BL_UINT8; src and dst distinct). No counterpart in the
+ * reference: a resident choice set kept sorted by length (so a wave's
+ * choices have similar lengths) gets its search results back in the
+ * caller's choice order with it (inv[i] = sorted position of choice i). */
+int bl_gather_cols(const void *d_src, const int64_t *d_inv, void *d_dst, int64_t rows, int64_t cols,
+                   int dtype, void *stream);
+
 /* One cdist class: queries d_qidx[0..nqc) (each at most 64*nw bytes, nw in
  * {1, 2, 4, 8}) against all choices; out is the row-major nq x nc matrix of
  * the call (out[q * nc + c], nc = choices->n). One workgroup per (query,

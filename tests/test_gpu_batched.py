@@ -284,3 +284,19 @@ def test_autotuner_picks_and_caches(tmp_path, monkeypatch):
     np.testing.assert_array_equal(outc.get(), O.saturate(O.cross(pq, pc, k=2, threads=8), np.uint8))
     assert autotune.tune_cdist(Q, C, outc, 2, None) == pc_
     assert (tmp_path / "tune.json").exists()
+
+
+def test_gather_cols_every_dtype():
+    """bl_gather_cols: dst[r][i] = src[r][inv[i]] for every output dtype, a
+    column count that is not a multiple of the block, and one row."""
+    rng = np.random.default_rng(14)
+    for rows, cols in ((5, 1000), (1, 77), (3, 70000)):
+        inv = rng.permutation(cols).astype(np.int64)
+        dinv = hip.DeviceBuffer.from_host(inv)
+        for dt in (np.uint8, np.uint16, np.int32):
+            src = rng.integers(0, 200, (rows, cols)).astype(dt)
+            ds = hip.DeviceBuffer.from_host(src)
+            dd = hip.DeviceArray.empty((rows, cols), dt)
+            call("bl_gather_cols", ds.ptr, dinv.ptr, dd.ptr, rows, cols, hip.DTYPE_CODE[np.dtype(dt)], None)
+            call("bl_device_sync")
+            np.testing.assert_array_equal(dd.get(), src[:, inv])

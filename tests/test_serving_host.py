@@ -139,3 +139,26 @@ def test_argument_errors():
     with pytest.raises(ValueError):
         srv.submit_cdist(["q" * 600], "s")
     srv.close()
+
+
+def test_length_sorted_resident_layout():
+    """A resident set is reordered by length on the host: every string lands
+    intact at its sorted position (empty strings included), and the column
+    order maps each original choice to that position."""
+    from bitlev.serving import _gather_strings
+    rng = np.random.default_rng(3)
+    S = [bytes(rng.integers(97, 123, int(rng.integers(0, 40))).astype(np.uint8))
+         for _ in range(70000)] + [b""] * 5
+    off = np.zeros(len(S) + 1, np.int64)
+    np.cumsum([len(x) for x in S], out=off[1:])
+    buf = np.frombuffer(b"".join(S), np.uint8)
+    order = np.argsort(np.diff(off), kind="stable")
+    nbuf, noff = _gather_strings(buf, off, order, chunk=4096)
+    assert np.all(np.diff(np.diff(noff)) >= 0)
+    for j in range(len(S)):
+        assert bytes(nbuf[noff[j]:noff[j + 1]]) == S[order[j]], j
+    inv = np.empty(len(order), np.int64)
+    inv[order] = np.arange(len(order))
+    for i in range(0, len(S), 101):
+        j = inv[i]
+        assert bytes(nbuf[noff[j]:noff[j + 1]]) == S[i]
