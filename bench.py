@@ -62,6 +62,10 @@ def parse():
     ap.add_argument("--serving-load", type=float, default=0.75,
                     help="offered load as a fraction of the measured batched-offline throughput")
     ap.add_argument("--serving-slots", type=int, default=4)
+    ap.add_argument("--ingest-pairs", type=int, default=1 << 21,
+                    help="pairs workloads: end-to-end leg from Python lists of this many pairs "
+                         "(packing, PCIe copies, kernels, results back), staged pipeline vs one "
+                         "whole-batch pack; 0 skips it")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 3 times (a kernel of known traffic "
                          "for calibrating the HBM byte counters) and print the byte count")
@@ -270,6 +274,8 @@ def main():
     }
     plain = main_m
 
+    if wl["kind"] == "pairs" and args.ingest_pairs > 0:
+        res["ingest"] = ingest_leg(hip, ha, hb, k, dt, out, args.ingest_pairs, stream)
     if not args.no_cpu:
         cpu_leg(res, args, wl, k, dt, hq if wl["kind"] == "cdist" else ha,
                 hc if wl["kind"] == "cdist" else hb, out, plain)
@@ -278,6 +284,45 @@ def main():
         del out, X, Y
         res["serving"] = serving_leg(args, seed)
     print(json.dumps(res), flush=True)
+
+
+def ingest_leg(hip, ha, hb, k, dt, out, n, stream):
+    """End-to-end pairwise from Python lists of bytes (the first n pairs of the
+    workload): the staged pipeline (chunks packed on a thread pool into pinned
+    buffers, copies and kernels overlapped) against one whole-batch pack and
+    pageable copy. Host-clock wall time; results checked against the
+    resident run's."""
+    from bitlev import ingest
+    n = min(n, len(ha[1]) - 1)
+    (ba, oa), (bb, ob) = ha, hb
+    A = [ba[oa[i]:oa[i + 1]].tobytes() for i in range(n)]
+    B = [bb[ob[i]:ob[i + 1]].tobytes() for i in range(n)]
+    want = out.get(stream.ptr)[:n]
+    md = None if k < 0 else k
+
+    def timed(fn):
+        fn()                                   # warm: pinned slots, workspace, tuning cache
+        best, r = 1e9, None
+        for _ in range(2):
+            t = time.perf_counter()
+            r = fn()
+            best = min(best, time.perf_counter() - t)
+        return best, r
+
+    t_staged, r1 = timed(lambda: hip.pairwise(A, B, max_dist=md, dtype=dt))
+    saved = hip.STAGED_PAIRS
+    hip.STAGED_PAIRS = 1 << 62
+    try:
+        t_whole, r2 = timed(lambda: hip.pairwise(A, B, max_dist=md, dtype=dt))
+    finally:
+        hip.STAGED_PAIRS = saved
+    if not (np.array_equal(r1, want) and np.array_equal(r2, want)):
+        raise RuntimeError("ingest leg: end-to-end results differ from the resident run")
+    _log(f"ingest: {n} pairs staged {t_staged:.3f}s, whole batch {t_whole:.3f}s")
+    return {"pairs": n, "threads": ingest.effective_threads(),
+            "staged_mpairs_s": round(n / t_staged / 1e6, 2),
+            "whole_batch_mpairs_s": round(n / t_whole / 1e6, 2),
+            "staged_over_whole": round(t_whole / t_staged, 2), "results_match": True}
 
 
 def serving_config(args):

@@ -40,6 +40,7 @@ SIGNATURES = {
     "bl_event_create": [_c.POINTER(_vp)],
     "bl_event_destroy": [_vp],
     "bl_event_record": [_vp, _vp],
+    "bl_stream_wait_event": [_vp, _vp],
     "bl_event_sync": [_vp],
     "bl_event_query": [_vp],
     "bl_event_elapsed_ms": [_vp, _vp, _c.POINTER(_c.c_float)],

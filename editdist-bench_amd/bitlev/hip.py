@@ -389,6 +389,10 @@ def _finish(out: DeviceArray, return_device: bool, stream=None):
 
 
 # ------------------------------------------------------------------ public API --
+# pairwise calls of at least this many pairs (results to the host) go through
+# the staged ingest pipeline (bitlev.ingest); smaller ones pack in one piece
+STAGED_PAIRS = 1 << 16
+
 def pairwise(a, b, max_dist=None, dtype=np.int32, return_device: bool = False):
     """out[i] = d(a[i], b[i]) on the GPU (bitlev/gpu.py:71-103).
     return_device=True returns a DeviceArray (`.get()` gives numpy)."""
@@ -397,6 +401,10 @@ def pairwise(a, b, max_dist=None, dtype=np.int32, return_device: bool = False):
         raise ValueError(f"length mismatch: {len(a)} vs {len(b)}")
     k, dt = check_k(max_dist), check_dtype(dtype)
     n = len(a)
+    if n >= STAGED_PAIRS and not return_device:
+        _require_device()
+        from . import ingest                   # packing, copies and kernels overlapped
+        return ingest.pairwise(a, b, k, dt, True)
     pa, pb = pack(a), pack(b)
     _require_device()
     out = DeviceArray.empty((n,), dt)

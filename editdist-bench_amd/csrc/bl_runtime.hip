@@ -167,6 +167,13 @@ int bl_event_record(void *event, void *stream)
     return BL_OK;
 }
 
+int bl_stream_wait_event(void *stream, void *event)
+{
+    if (!event) return bl::fail(BL_EINVAL, "bl_stream_wait_event: null event");
+    BL_HIP(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0));
+    return BL_OK;
+}
+
 int bl_event_sync(void *event)
 {
     BL_HIP(hipEventSynchronize((hipEvent_t)event));

@@ -11,6 +11,9 @@
  *   pack(seq) -> (buf: bytes, offsets: bytes of int64[n + 1])
  *   pack_into(seq, buf, offsets) -> total bytes   (writable buffers, no copy out)
  *   sizes(seq) -> (n, total_bytes, max_len)
+ *   pack_range(seq, lo, hi, buf, offsets) -> (total_bytes, max_len)
+ *       items lo..hi-1 of a list or tuple, entirely without the GIL, so that
+ *       several threads pack chunks of one batch at once
  */
 #define PY_SSIZE_T_CLEAN
 #include <Python.h>
@@ -126,6 +129,114 @@ static PyObject *py_pack_into(PyObject *self, PyObject *args)
     return ret;
 }
 
+/* One item read without the GIL (type flags, kind and length are fixed for
+ * the object's life): 0 ok, 1 neither str nor bytes, 2 str wider than one
+ * byte per character, 3 str still in the legacy non-canonical form (made
+ * canonical under the GIL by the caller, then read again). */
+static int item_bytes_nogil(PyObject *o, const char **p, Py_ssize_t *len)
+{
+    if (PyUnicode_Check(o)) {
+#if PY_VERSION_HEX < 0x030C0000
+        if (!PyUnicode_IS_READY(o)) return 3;
+#endif
+        if (PyUnicode_KIND(o) != PyUnicode_1BYTE_KIND) return 2;
+        *p = (const char *)PyUnicode_1BYTE_DATA(o);
+        *len = PyUnicode_GET_LENGTH(o);
+        return 0;
+    }
+    if (PyBytes_Check(o)) {
+        *p = PyBytes_AS_STRING(o);
+        *len = PyBytes_GET_SIZE(o);
+        return 0;
+    }
+    return 1;
+}
+
+/* Offsets (relative to the first item) and bytes of n items in one pass. The
+ * objects are scattered over the heap, so their headers are prefetched a few
+ * items ahead; bytes past `cap` are counted but not written (the caller
+ * grows its buffer to the returned total and packs again). */
+static int fill_range(PyObject *const *items, Py_ssize_t n, uint8_t *buf, int64_t cap,
+                      int64_t *off, int64_t *total, int64_t *maxlen, Py_ssize_t *bad)
+{
+    enum { AHEAD = 12 };
+    int64_t t = 0, mx = 0;
+    off[0] = 0;
+    for (Py_ssize_t i = 0; i < n; i++) {
+        if (i + AHEAD < n) {
+            const char *q = (const char *)items[i + AHEAD];
+            __builtin_prefetch(q);
+            __builtin_prefetch(q + 64);
+        }
+        const char *p;
+        Py_ssize_t len;
+        int e = item_bytes_nogil(items[i], &p, &len);
+        if (e) {
+            *bad = i;
+            return e;
+        }
+        if (t + len <= cap) memcpy(buf + t, p, (size_t)len);
+        t += len;
+        off[i + 1] = t;
+        if (len > mx) mx = len;
+    }
+    *total = t;
+    *maxlen = mx;
+    return 0;
+}
+
+static PyObject *py_pack_range(PyObject *self, PyObject *args)
+{
+    (void)self;
+    PyObject *seq;
+    Py_ssize_t lo, hi;
+    Py_buffer bv, ov;
+    if (!PyArg_ParseTuple(args, "Onnw*w*", &seq, &lo, &hi, &bv, &ov)) return NULL;
+    PyObject *ret = NULL;
+    if (!PyList_Check(seq) && !PyTuple_Check(seq)) {
+        PyErr_SetString(PyExc_TypeError, "pack_range needs a list or tuple");
+    } else if (lo < 0 || hi < lo || hi > PySequence_Fast_GET_SIZE(seq)) {
+        PyErr_SetString(PyExc_IndexError, "pack_range: range outside the sequence");
+    } else if (ov.len < (Py_ssize_t)((hi - lo + 1) * sizeof(int64_t))) {
+        PyErr_SetString(PyExc_ValueError, "offsets buffer too small");
+    } else {
+        PyObject *const *items = PySequence_Fast_ITEMS(seq) + lo;
+        for (int attempt = 0;; attempt++) {
+            int64_t total = 0, mx = 0;
+            Py_ssize_t bad = 0;
+            int err;
+            Py_BEGIN_ALLOW_THREADS
+            err = fill_range(items, hi - lo, (uint8_t *)bv.buf, (int64_t)bv.len,
+                             (int64_t *)ov.buf, &total, &mx, &bad);
+            Py_END_ALLOW_THREADS
+            if (err == 3 && attempt == 0) {
+                Py_ssize_t i;
+                for (i = 0; i < hi - lo; i++) {
+                    const char *p;
+                    Py_ssize_t len;
+                    if (PyUnicode_Check(items[i]) && item_bytes(items[i], &p, &len) < 0) break;
+                }
+                if (i < hi - lo) break;            /* item_bytes set the error */
+                continue;
+            }
+            if (err == 1) {
+                PyErr_SetString(PyExc_TypeError, "expected str or bytes");
+            } else if (err == 2) {
+                PyErr_SetString(PyExc_ValueError,
+                                "string has characters above U+00FF; pass bytes (e.g. UTF-8 encoded) instead");
+            } else if (err == 3) {
+                PyErr_SetString(PyExc_RuntimeError, "pack_range: str could not be made canonical");
+            } else {
+                ret = Py_BuildValue("(LL)", (long long)total, (long long)mx);
+            }
+            break;
+        }
+    }
+    PyBuffer_Release(&bv);
+    PyBuffer_Release(&ov);
+    return ret;
+}
+
 static PyObject *py_sizes(PyObject *self, PyObject *args)
 {
     (void)self;
@@ -142,6 +253,8 @@ static PyObject *py_sizes(PyObject *self, PyObject *args)
 static PyMethodDef methods[] = {
     {"pack", py_pack, METH_VARARGS, "pack(seq) -> (buf bytes, int64 offsets bytes)"},
     {"pack_into", py_pack_into, METH_VARARGS, "pack_into(seq, buf, offsets) -> total bytes"},
+    {"pack_range", py_pack_range, METH_VARARGS,
+     "pack_range(seq, lo, hi, buf, offsets) -> (total bytes, longest item)"},
     {"sizes", py_sizes, METH_VARARGS, "sizes(seq) -> (n, total_bytes, max_len)"},
     {NULL, NULL, 0, NULL},
 };

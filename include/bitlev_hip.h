@@ -64,6 +64,8 @@ int bl_device_sync(void);
 int bl_event_create(void **event);
 int bl_event_destroy(void *event);
 int bl_event_record(void *event, void *stream);
+/* Work issued to `stream` after this call waits for `event` (no host wait). */
+int bl_stream_wait_event(void *stream, void *event);
 int bl_event_sync(void *event);
 int bl_event_query(void *event);                       /* 1 done, 0 pending, <0 error */
 int bl_event_elapsed_ms(void *start, void *stop, float *ms);

@@ -80,6 +80,38 @@ def test_pack_errors():
         _edpy.pack_into(["abc"], bytearray(2), np.zeros(2, np.int64))
 
 
+def test_pack_range_matches_pack_on_every_span():
+    """The GIL-free range packer: same bytes and offsets as pack() of the
+    slice, a short buffer reports the size it needs, errors as pack()'s."""
+    rng = np.random.default_rng(5)
+    S = []
+    for i in range(300):
+        s = bytes(rng.integers(0, 256, int(rng.integers(0, 90))).astype(np.uint8))
+        S.append(s.decode("latin-1") if i % 2 else s)
+    for seq in (S, tuple(S)):
+        for lo, hi in ((0, 0), (0, 300), (17, 18), (40, 261), (299, 300)):
+            buf, off = hip.pack(S[lo:hi])
+            b = np.zeros(int(off[-1]) + 3, np.uint8)
+            o = np.full(hi - lo + 1, -1, np.int64)
+            total, mx = _edpy.pack_range(seq, lo, hi, b, o)
+            assert total == off[-1] and o.tolist() == off.tolist()
+            assert mx == (int(np.diff(off).max()) if hi > lo else 0)
+            assert bytes(b[:total]) == bytes(buf[:total])
+            if total:
+                short = np.zeros(total - 1, np.uint8)
+                assert _edpy.pack_range(seq, lo, hi, short, o)[0] == total
+    with pytest.raises(TypeError):
+        _edpy.pack_range(iter(S), 0, 1, bytearray(8), np.zeros(2, np.int64))
+    with pytest.raises(IndexError):
+        _edpy.pack_range(S, 299, 301, bytearray(8), np.zeros(3, np.int64))
+    with pytest.raises(ValueError):
+        _edpy.pack_range(S, 0, 4, bytearray(8), np.zeros(4, np.int64))
+    with pytest.raises(ValueError):
+        _edpy.pack_range(["ok", "ā"], 0, 2, bytearray(8), np.zeros(3, np.int64))
+    with pytest.raises(TypeError):
+        _edpy.pack_range(["ok", 3], 0, 2, bytearray(8), np.zeros(3, np.int64))
+
+
 def test_argument_checks():
     assert check_k(None) == -1 and check_k(3) == 3 and check_k(2**40) == 2**30
     with pytest.raises(ValueError):
