@@ -60,9 +60,16 @@ def _require_device():
 
 # ------------------------------------------------------------------ host side --
 def pack(strings) -> tuple[np.ndarray, np.ndarray]:
-    """str (chars <= U+00FF) / bytes -> (uint8 buffer, int64 offsets[n + 1])."""
+    """str (chars <= U+00FF) / bytes -> (uint8 buffer, int64 offsets[n + 1]).
+    Long lists are packed by a thread pool (bitlev.ingest.pack)."""
+    if isinstance(strings, (list, tuple)) and len(strings) >= _PARALLEL_PACK:
+        from . import ingest
+        return ingest.pack(strings)
     buf, off = _edpy.pack(strings)
     return np.frombuffer(buf, dtype=np.uint8), np.frombuffer(off, dtype=np.int64)
+
+
+_PARALLEL_PACK = 1 << 15
 
 
 def pair_classes(off_a: np.ndarray, off_b: np.ndarray):

@@ -40,6 +40,59 @@ def effective_threads() -> int:
     return max(1, min(16, n))
 
 
+_pool_lock = threading.Lock()
+_pool_obj = None
+
+
+def _pool() -> concurrent.futures.ThreadPoolExecutor:
+    """The packing threads shared by every call of this module."""
+    global _pool_obj
+    with _pool_lock:
+        if _pool_obj is None:
+            _pool_obj = concurrent.futures.ThreadPoolExecutor(
+                max_workers=effective_threads(), thread_name_prefix="bitlev-pack")
+        return _pool_obj
+
+
+def pack(seq, threads: int | None = None):
+    """(uint8 buffer, int64 offsets[n + 1]) of a list/tuple of str/bytes, the
+    layout of _edpy.pack, packed by the thread pool in two passes: every
+    thread measures a range of items, then copies it to its place in the
+    buffer. Items are read in place without the GIL (the sequence must not be
+    changed during the call)."""
+    if not isinstance(seq, (list, tuple)):
+        seq = list(seq)
+    n = len(seq)
+    T = threads or effective_threads()
+    step = max(4096, -(-n // (4 * T)))
+    ranges = [(lo, min(lo + step, n)) for lo in range(0, n, step)]
+    empty = np.empty(0, np.uint8)
+
+    def measure(r):
+        lo, hi = r
+        return _edpy.pack_range(seq, lo, hi, empty, np.empty(hi - lo + 1, np.int64))[0]
+
+    pool = _pool()
+    totals = list(pool.map(measure, ranges))
+    base = np.zeros(len(ranges) + 1, np.int64)
+    np.cumsum(totals, out=base[1:])
+    buf = np.empty(max(int(base[-1]), 1), np.uint8)
+    off = np.empty(n + 1, np.int64)
+    off[0] = 0
+    if not n:
+        buf[0] = 0
+        return buf, off
+
+    def fill(j):
+        lo, hi = ranges[j]
+        rel = np.empty(hi - lo + 1, np.int64)
+        _edpy.pack_range(seq, lo, hi, buf[base[j]:base[j + 1]], rel)
+        np.add(rel[1:], base[j], out=off[lo + 1:hi + 1])
+
+    list(pool.map(fill, range(len(ranges))))
+    return buf, off
+
+
 class _Pinned:
     """Pinned (page-locked) host bytes, viewed as numpy."""
 
@@ -138,13 +191,13 @@ def _pack_chunk(slot: _Slot, a, b, lo: int, hi: int):
 
 
 class Stager:
-    """The pipeline's pinned slots, streams, events, workspace and packing
-    threads, kept from one call to the next (pinning host memory costs about
+    """The pipeline's pinned slots, streams, events and workspace, kept from
+    one call to the next (pinning host memory costs about
     as much as the copies it speeds up). One call at a time per stager;
     pairwise() hands each concurrent caller a stager of its own."""
 
     def __init__(self, threads: int | None = None):
-        self.threads = threads or effective_threads()
+        self.threads = threads or effective_threads()   # chunks being packed at once
         self.slots = [_Slot(4) for _ in range(self.threads + 2)]
         self.streams = []
         for _ in range(2):
@@ -152,10 +205,8 @@ class Stager:
             call("bl_stream_create", ctypes.byref(s))
             self.streams.append(s.value)
         self.ws = _RetainingWorkspace()
-        self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=self.threads)
 
     def close(self):
-        self.pool.shutdown(wait=True)
         for s in self.streams:
             call("bl_stream_sync", s)
         for slot in self.slots:
@@ -210,7 +261,7 @@ class Stager:
                     slot = slots[j % len(slots)]
                     finalize(slot)                 # its previous chunk's results, then reuse
                     lo, hi = bounds[j]
-                    futures[j] = self.pool.submit(_pack_chunk, slot, a, b, lo, hi)
+                    futures[j] = _pool().submit(_pack_chunk, slot, a, b, lo, hi)
                     submitted += 1
                 enqueue(i, futures.pop(i).result())
             for slot in slots:

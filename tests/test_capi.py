@@ -199,3 +199,21 @@ def test_stripes_layout_on_host():
 def test_match_list_needs_a_threshold():
     with pytest.raises(ValueError):
         hip.cdist_matches(["abc"], ["abd"], None)
+
+
+@pytest.mark.parametrize("n", [0, 1, 4095, 4096, 40000, (1 << 15) + 3])
+def test_parallel_pack_matches_serial(n):
+    from bitlev import ingest
+    rng = np.random.default_rng(n)
+    S = [bytes(rng.integers(0, 256, int(l)).astype(np.uint8)) for l in rng.integers(0, 40, n)]
+    S = [s.decode("latin-1") if i % 3 == 0 else s for i, s in enumerate(S)]
+    b0, o0 = _edpy.pack(S)
+    for threads in (1, 3, None):
+        buf, off = ingest.pack(tuple(S) if threads == 3 else S, threads=threads)
+        assert off.tolist() == np.frombuffer(o0, np.int64).tolist()
+        assert bytes(buf[:off[-1]]) == bytes(b0)[:off[-1]] and len(buf) >= 1
+    buf, off = hip.pack(S)                       # the public packer routes long lists here
+    assert off.tolist() == np.frombuffer(o0, np.int64).tolist()
+    if n > 10:
+        with pytest.raises(TypeError):
+            ingest.pack(S[:-1] + [None])
