@@ -35,6 +35,10 @@ CDIST_SPACE = {"mode": ["strided", "queue"], "waves_per_cu": [8, 16, 32],
 CDIST_SWAR_SPACE = {"mode": ["swar"], "waves_per_cu": [8, 16, 32], "tile": [2048, 8192, 32768]}
 # gap windows (any threshold): the length test on 256-choice windows, the rest staged
 CDIST_GAP_SPACE = {"mode": ["gap"], "waves_per_cu": [8, 16, 32], "tile": [2048, 8192, 32768]}
+# count windows (any threshold; choices with class profiles): a choice block
+# against a group of queries, the length and class-count bounds first
+CDIST_COUNT_SPACE = {"mode": ["count"], "waves_per_cu": [16, 32], "tile": [2048, 4096, 8192],
+                     "qgroup": [8, 16, 32]}
 PAIRS_SPACE = {"waves_per_cu": [8, 16, 32], "block": [64, 256], "bin_width": [8, 16, 32],
                "chunk": [16384, 65536, 262144], "trim": [True, False]}
 # direct mode (one launch, no bucketing): only the launch shape matters
@@ -166,6 +170,8 @@ def tune_cdist(Q: hip.Packed, C: hip.Packed, out: hip.DeviceArray, k: int = -1, 
         cands += list(_grid(CDIST_SWAR_SPACE))
     if space is None and k >= 0:
         cands += list(_grid(CDIST_GAP_SPACE))
+        if C.stripes(stream).desc.counts:
+            cands += list(_grid(CDIST_COUNT_SPACE))
     for p in cands:
         ms = _time(lambda: hip.cdist_batched_into(Qs, C, out, k, stream, ws, p), stream)
         res.append((ms, p))

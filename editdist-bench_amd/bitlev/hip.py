@@ -131,21 +131,26 @@ class Packed:
 class ChoiceStripes:
     """A packed set interleaved by dword in groups of 64 strings (bl_stripes):
     the layout the cdist kernels read, so that a wave's lanes load 256
-    contiguous bytes per step."""
+    contiguous bytes per step, plus each string's 16-byte byte-class profile
+    (the batched searches' class-count bound; profiles=False leaves it out)."""
 
-    def __init__(self, P: Packed, stream=None):
+    def __init__(self, P: Packed, stream=None, profiles: bool = True):
         n = P.n
         gbase = np.empty((n + 63) // 64 + 1, np.int64)
         total = call("bl_stripes_layout", P.h_off.ctypes.data, n, gbase.ctypes.data)
         self.d_gbase = DeviceBuffer.from_host(gbase, stream)
         self.d_words = DeviceBuffer(max(int(total), 1) * 4)
         self.d_len = DeviceBuffer(max(n, 1) * 4)
+        self.d_counts = DeviceBuffer(max(n, 1) * 16) if profiles else None
+        counts = self.d_counts.ptr if profiles else None
         call("bl_stripes_build", P.d_buf.ptr, P.d_off.ptr, n, self.d_gbase.ptr, self.d_words.ptr,
-             self.d_len.ptr, stream)
+             self.d_len.ptr, counts, stream)
         # built once per set; finish here so that any stream may read them next
         call("bl_stream_sync", stream)
         self.nbytes = self.d_gbase.nbytes + self.d_words.nbytes + self.d_len.nbytes
-        self.desc = Stripes(self.d_words.ptr, self.d_gbase.ptr, self.d_len.ptr, n)
+        if profiles:
+            self.nbytes += self.d_counts.nbytes
+        self.desc = Stripes(self.d_words.ptr, self.d_gbase.ptr, self.d_len.ptr, n, counts)
 
     @property
     def ref(self):
@@ -252,7 +257,8 @@ def _cdist_params(params):
     p = CdistParams()
     p.waves_per_cu = int(params.get("waves_per_cu", 0))
     p.tile = int(params.get("tile", 0))
-    p.reserved[0] = {"strided": 1, "swar": 2, "gap": 3}.get(params.get("mode"), 0)
+    p.reserved[0] = {"strided": 1, "swar": 2, "gap": 3, "count": 4}.get(params.get("mode"), 0)
+    p.reserved[2] = int(params.get("qgroup", 0))
     p.reserved[1] = int(params.get("grab", 0))
     return ctypes.byref(p)
 

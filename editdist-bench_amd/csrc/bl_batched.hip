@@ -180,22 +180,39 @@ struct WaveLds {
  * 64 * nwmax (a caller error, answered -1 by the search kernel) gets zeros. */
 __global__ __launch_bounds__(256) void peq_build_kernel(const u8 *__restrict__ bq,
                                                         const int64_t *__restrict__ oq, int nwmax,
-                                                        u64 *__restrict__ peq)
+                                                        u64 *__restrict__ peq, Profile *__restrict__ qcnt)
 {
     __shared__ u8 qs[512];
+    __shared__ u32 cls[16];
     const int64_t q = blockIdx.x;
     const int64_t q0 = oq[q];
     int m = (int)(oq[q + 1] - q0);
     if (m > 64 * nwmax) m = 0;
     for (int i = threadIdx.x; i < m; i += 256) qs[i] = bq[q0 + i];
+    if (threadIdx.x < 16) cls[threadIdx.x] = 0;
     __syncthreads();
     const u32 c = threadIdx.x;
     u64 *row = peq + ((size_t)q * 256 + c) * nwmax;
+    u32 cnt = 0;
     for (int w = 0; w < nwmax; w++) {
         u64 word = 0;
         const int e = m < 64 * w + 64 ? m : 64 * w + 64;
         for (int i = 64 * w; i < e; i++) word |= (u64)(qs[i] == c) << (i - 64 * w);
         row[w] = word;
+        cnt += (u32)popc64(word);
+    }
+    if (!qcnt) return;
+    /* the query's byte-class profile (the class-count bound's query side):
+     * the occurrences of byte c are the set bits of its Peq row */
+    atomicAdd(&cls[c & 15u], cnt);
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        u32 v = 0;
+        for (int i = 0; i < 4; i++) {
+            const u32 x = cls[4 * threadIdx.x + i];
+            v |= (x > 255u ? 255u : x) << (8 * i);
+        }
+        ((u32 *)(qcnt + q))[threadIdx.x] = v;
     }
 }
 
@@ -403,6 +420,17 @@ __device__ void cdist_tile_strided(const u32 *lo32, const u64 *__restrict__ gpeq
     }
 }
 
+/* k + 1 as stored in an output element of type OutT: saturated like every
+ * other result (bitlev/gpu.py:64-68), so k = 300 stores 255 in uint8. */
+template <typename OutT>
+__device__ __forceinline__ u32 out_k1(int k)
+{
+    const u32 v = (u32)k + 1u;
+    if (sizeof(OutT) == 1) return v > 255u ? 255u : v;
+    if (sizeof(OutT) == 2) return v > 65535u ? 65535u : v;
+    return v;
+}
+
 /* SWAR mode, for small thresholds (k <= 3) and queries of at least 8 bytes:
  * each lane carries four consecutive choices of a 256-choice window (w + 4l ..
  * w + 4l + 3, whose first text dwords are adjacent in the stripes, so a lane
@@ -425,57 +453,68 @@ __device__ void cdist_tile_strided(const u32 *lo32, const u64 *__restrict__ gpeq
 template <bool FULL, bool FILTER, bool DENSE, typename OutT>
 __device__ __forceinline__ void swar_window(WaveLds &L, int m, const u32 *__restrict__ words,
                                             const int64_t *__restrict__ gbase,
-                                            const int32_t *__restrict__ lw, OutT *ow, int64_t w,
-                                            int64_t c1, int k, int64_t orow, int &ns)
+                                            const int32_t *__restrict__ lw,
+                                            const Profile *__restrict__ cw, const Profile &qc, OutT *ow,
+                                            int64_t w, int64_t c1, int k, int64_t orow, int &ns)
 {
     const int lane = lane_id();
-    const u32 kk1 = (u32)k + 1u;            /* k + 1 <= 4 fits every output type */
+    const u32 kk1 = out_k1<OutT>(k);
     /* lane l takes choices w + 4l .. w + 4l + 3: one stripe group ((w >> 6) +
      * l / 16, w a multiple of 256), whose first dwords sit side by side, so
      * the text, the lengths and (in full windows) the results move as one
      * 16-byte (results: 4-element) access per lane */
     const int64_t cl = w + 4 * lane;
-    u32 d[4];
+    u32 d[4] = {0u, 0u, 0u, 0u};
     int n[4];
     bool valid[4];
     if (FULL) {
-        if (FILTER) {
-            const uint4 t = *(const uint4 *)(words + gbase[(w >> 6) + (lane >> 4)] + 4 * (lane & 15));
-            d[0] = t.x; d[1] = t.y; d[2] = t.z; d[3] = t.w;
-        }
         const int4 l4 = *(const int4 *)(lw + 4 * lane);
         n[0] = l4.x; n[1] = l4.y; n[2] = l4.z; n[3] = l4.w;
 #pragma unroll
         for (int s = 0; s < 4; s++) valid[s] = true;
     } else {
-        const u32 *gp = FILTER && cl < c1 ? words + gbase[(w >> 6) + (lane >> 4)] + 4 * (lane & 15)
-                                          : words;
 #pragma unroll
         for (int s = 0; s < 4; s++) {
             valid[s] = cl + s < c1;
             n[s] = valid[s] ? lw[4 * lane + s] : 0;
-            d[s] = FILTER && valid[s] ? gp[s] : 0u;
         }
     }
-    u32 over = 0;
-    if (FILTER) {
+    /* out of band by the length gap or by the class-count bound (cw: the
+     * window's choice profiles, qc the query's): the answer is k + 1 */
+    bool decided[4];
+    bool need = false;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const int gap = n[s] > m ? n[s] - m : m - n[s];
+        decided[s] = gap > k;
+        if (cw && !decided[s] && valid[s]) decided[s] = count_over(qc, cw[4 * lane + s], m, n[s], k);
+        need = need || (valid[s] && !decided[s]);
+    }
+    if (FILTER && wave_ballot(need) != 0) {
+        if (FULL) {
+            const uint4 t = *(const uint4 *)(words + gbase[(w >> 6) + (lane >> 4)] + 4 * (lane & 15));
+            d[0] = t.x; d[1] = t.y; d[2] = t.z; d[3] = t.w;
+        } else {
+            const u32 *gp = cl < c1 ? words + gbase[(w >> 6) + (lane >> 4)] + 4 * (lane & 15) : words;
+#pragma unroll
+            for (int s = 0; s < 4; s++) d[s] = valid[s] ? gp[s] : 0u;
+        }
         u32 Pv, Mv;
         swar_steps4(L.lo8, d, Pv, Mv, 1);
-        over = swar_over4(Pv, Mv, m, n, k);
+        const u32 over = swar_over4(Pv, Mv, m, n, k);
+        /* the bound at column 4 exceeds k (an empty choice has gap m >= 8 > k;
+         * without the filter one with m <= k stays open and its scan returns m) */
+#pragma unroll
+        for (int s = 0; s < 4; s++) decided[s] = decided[s] || ((over >> (8 * s + 7)) & 1u);
     }
     bool open[4];
     bool any = false;
 #pragma unroll
     for (int s = 0; s < 4; s++) {
-        const int gap = n[s] > m ? n[s] - m : m - n[s];
-        /* out of band, or (FILTER) the bound at column 4 exceeds k: the answer
-         * is k + 1 (FILTER: an empty choice has gap m >= 8 > k; without the
-         * filter one with m <= k stays open and its scan returns m) */
-        const bool decided = gap > k || (FILTER && ((over >> (8 * s + 7)) & 1u));
-        open[s] = valid[s] && !decided;
+        open[s] = valid[s] && !decided[s];
         any = any || open[s];
         /* (k + 1 is never a match: a match list gets nothing) */
-        if (DENSE && !FULL && valid[s] && decided) ow[4 * lane + s] = (OutT)kk1;
+        if (DENSE && !FULL && valid[s] && decided[s]) ow[4 * lane + s] = (OutT)kk1;
     }
     if (DENSE && FULL) {
         /* all four get k + 1; an open one is overwritten by its rescan later
@@ -619,6 +658,14 @@ __device__ void cdist_tile_refill(WaveLds &L, const u64 *__restrict__ gpeq, int 
     }
 }
 
+/* Count-mode rescans read a query's 32-row table from its global Peq table
+ * (lane by lane, each lane's own query) instead of the wave's LDS copy. */
+template <int NWMAX>
+struct GlobalLo32 {
+    const u64 *peq;
+    __device__ __forceinline__ u32 operator[](u32 c) const { return (u32)peq[c * NWMAX]; }
+};
+
 /* Persistent cdist over queries of at most 64*NWMAX bytes whose Peq tables
  * peq_build_kernel wrote to gpeq. Waves pull (choice block, query) tiles,
  * block-major, from the work-queue counter. */
@@ -626,7 +673,8 @@ template <int NWMAX, typename Sink>
 __global__ __launch_bounds__(256) void cdist_batched_kernel(
     const int64_t *__restrict__ oq, int64_t nq, const u64 *__restrict__ gpeq,
     const u32 *__restrict__ words, const int64_t *__restrict__ gbase,
-    const int32_t *__restrict__ clen, int64_t nc, Sink sink,
+    const int32_t *__restrict__ clen, const Profile *__restrict__ ccnt,
+    const Profile *__restrict__ qcnt, int64_t nc, Sink sink,
     int k_all, const int32_t *__restrict__ kvec, int64_t tile, int mode, int grab,
     unsigned long long *counter)
 {
@@ -642,6 +690,7 @@ __global__ __launch_bounds__(256) void cdist_batched_kernel(
     int ns = 0;
     const u64 *qpeq = nullptr;
     int m = 0, k = 0;
+    Profile qc{0u, 0u, 0u, 0u};   /* the current query's class profile */
     auto drain = [&](int cnt) {
         const int base = ns - cnt;
         if (lane < cnt) {
@@ -679,6 +728,7 @@ __global__ __launch_bounds__(256) void cdist_batched_kernel(
                 L.lo32[x * 64 + lane] = v;
                 L.lo8[x * 64 + lane] = (u8)v;
             }
+            if (ccnt) qc = qcnt[q];
             wave_lds_sync();
             cur_q = q;
         }
@@ -694,18 +744,19 @@ __global__ __launch_bounds__(256) void cdist_batched_kernel(
                 constexpr bool F = decltype(full)::value;
                 constexpr bool FI = decltype(filter)::value;
                 int64_t w0 = w;
+                const Profile *cw = ccnt ? ccnt + w0 : nullptr;
                 if (!Sink::kDense)
-                    swar_window<F, FI, false>(L, m, words, gbase, clen + w0, (u8 *)nullptr, w0, c1,
-                                              k, orow, ns);
+                    swar_window<F, FI, false>(L, m, words, gbase, clen + w0, cw, qc, (u8 *)nullptr,
+                                              w0, c1, k, orow, ns);
                 else if (dense_dtype(sink) == DT_UINT8)
-                    swar_window<F, FI, true>(L, m, words, gbase, clen + w0,
+                    swar_window<F, FI, true>(L, m, words, gbase, clen + w0, cw, qc,
                                              (u8 *)dense_out(sink) + orow + w0, w0, c1, k, orow, ns);
                 else if (dense_dtype(sink) == DT_UINT16)
-                    swar_window<F, FI, true>(L, m, words, gbase, clen + w0,
+                    swar_window<F, FI, true>(L, m, words, gbase, clen + w0, cw, qc,
                                              (uint16_t *)dense_out(sink) + orow + w0, w0, c1, k, orow,
                                              ns);
                 else
-                    swar_window<F, FI, true>(L, m, words, gbase, clen + w0,
+                    swar_window<F, FI, true>(L, m, words, gbase, clen + w0, cw, qc,
                                              (int32_t *)dense_out(sink) + orow + w0, w0, c1, k, orow,
                                              ns);
                 if (drain_open)
@@ -732,6 +783,218 @@ __global__ __launch_bounds__(256) void cdist_batched_kernel(
             else
                 cdist_tile<NWMAX>(L, qpeq, m, words, gbase, clen, c0, c1, sink, orow, k);
         }
+    }
+}
+
+/* Count mode, the windows of one tile: choices c0..c1 (c0 a multiple of 256)
+ * against the queries q0 + j whose bit j is set in okmask, query j's length,
+ * threshold and profile in lane j of lm, lk, lp. A lane holds four
+ * consecutive choices of each 256-choice window (one 16-byte length load,
+ * four profile loads, one packed result store), kept in registers for the
+ * whole group. A choice is settled when its length gap exceeds k or its
+ * class-count bound does (d >= (L1 + gap) / 2; L1 / 2 if a length exceeds
+ * 255); when every length of the window and the query's fit a byte, the gap
+ * joins the v_sad_u8 chain (|n - m| of the low bytes) and one compare decides
+ * (L1 >= gap, so L1 + gap > 2k already covers gap > k). Settled choices get
+ * k + 1 (DENSE: the row at out + q * nc); the rest are pushed on the open
+ * stack as output indices for the per-lane rescans (drain). */
+template <bool DENSE, typename OutT, typename Drain>
+__device__ __forceinline__ void count_windows(WaveLds &L, const int32_t *__restrict__ clen,
+                                              const Profile *__restrict__ ccnt, int64_t c0,
+                                              int64_t c1, int64_t nc, int64_t q0, int gq, u64 okmask,
+                                              int lm, int lk, const Profile &lp, OutT *out,
+                                              bool rows_aligned, int &ns, Drain &drain)
+{
+    const int lane = lane_id();
+    for (int64_t w = c0; w < c1; w += 256) {
+        const int64_t cl = w + 4 * lane;
+        const bool full = w + 256 <= c1;
+        int n[4];
+        bool valid[4];
+        Profile cp[4];
+        if (full) {
+            const int4 l4 = *(const int4 *)(clen + cl);
+            n[0] = l4.x; n[1] = l4.y; n[2] = l4.z; n[3] = l4.w;
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                valid[s] = true;
+                cp[s] = ccnt[cl + s];
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                valid[s] = cl + s < c1;
+                n[s] = valid[s] ? clen[cl + s] : 0;
+                cp[s] = valid[s] ? ccnt[cl + s] : Profile{0u, 0u, 0u, 0u};
+            }
+        }
+        bool big = false;
+#pragma unroll
+        for (int s = 0; s < 4; s++) big = big || n[s] > 255;
+        const bool small = wave_ballot(big) == 0;
+        int64_t orow = q0 * nc;
+        for (int j = 0; j < gq; j++, orow += nc) {
+            if (!((okmask >> j) & 1u)) continue;
+            const int mq = __builtin_amdgcn_readlane(lm, j);
+            const int kq = __builtin_amdgcn_readlane(lk, j);
+            const Profile pq{(u32)__builtin_amdgcn_readlane((int)lp.x, j),
+                             (u32)__builtin_amdgcn_readlane((int)lp.y, j),
+                             (u32)__builtin_amdgcn_readlane((int)lp.z, j),
+                             (u32)__builtin_amdgcn_readlane((int)lp.w, j)};
+            const u32 k2 = 2u * (u32)kq;                 /* kq <= 2^30 */
+            bool open[4];
+            if (small && mq <= 255) {
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    u32 t = sad_u8(pq.x, cp[s].x, 0u);
+                    t = sad_u8(pq.y, cp[s].y, t);
+                    t = sad_u8(pq.z, cp[s].z, t);
+                    t = sad_u8(pq.w, cp[s].w, t);
+                    t = sad_u8((u32)n[s], (u32)mq, t);
+                    open[s] = valid[s] & (t <= k2);
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    const u32 gap = (u32)(n[s] > mq ? n[s] - mq : mq - n[s]);
+                    u32 t = sad_u8(pq.x, cp[s].x, 0u);
+                    t = sad_u8(pq.y, cp[s].y, t);
+                    t = sad_u8(pq.z, cp[s].z, t);
+                    t = sad_u8(pq.w, cp[s].w, t);
+                    t += (mq <= 255) & (n[s] <= 255) ? gap : 0u;
+                    open[s] = valid[s] & (gap <= (u32)kq) & (t <= k2);
+                }
+            }
+            if (DENSE) {
+                const u32 kk1 = out_k1<OutT>(kq);
+                OutT *ow = out + orow + w;
+                if (full && (rows_aligned || ((uintptr_t)ow & (4 * sizeof(OutT) - 1)) == 0)) {
+                    /* k + 1 for all four; an open one is overwritten by its
+                     * rescan later in this wave's program order */
+                    OutT *p = ow + 4 * lane;
+                    if (sizeof(OutT) == 1) *(u32 *)p = kk1 * 0x01010101u;
+                    else if (sizeof(OutT) == 2) *(uint2 *)p = make_uint2(kk1 * 0x00010001u, kk1 * 0x00010001u);
+                    else *(uint4 *)p = make_uint4(kk1, kk1, kk1, kk1);
+                } else {
+#pragma unroll
+                    for (int s = 0; s < 4; s++)
+                        if (valid[s]) ow[4 * lane + s] = (OutT)kk1;
+                }
+            }
+            const bool any = open[0] | open[1] | open[2] | open[3];
+            if (wave_ballot(any) == 0) continue;
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const u64 sm = wave_ballot(open[s]);
+                if (open[s]) L.open[ns + rank_below(sm)] = orow + cl + s;
+                ns = uni(ns + __popcll(sm));
+            }
+            wave_lds_sync();
+            while (ns >= 64) drain(64);
+        }
+    }
+}
+
+/* Count mode: persistent waves pull (choice block, query group) tiles. Each
+ * 256-choice window of the block is tested against every query of the group
+ * (qgroup queries) with the length and class-count bounds, so a window's
+ * lengths and profiles are loaded once per group and stay in registers; the
+ * choices neither bound settles are stacked as output indices
+ * and rescanned 64 at a time, each lane with its own query's global Peq
+ * table. Empty, too-long and exact queries run lane-strided. Needs the
+ * choices' and the queries' class profiles. */
+template <int NWMAX, typename Sink>
+__global__ __launch_bounds__(256) void cdist_count_kernel(
+    const int64_t *__restrict__ oq, int64_t nq, const u64 *__restrict__ gpeq,
+    const u32 *__restrict__ words, const int64_t *__restrict__ gbase,
+    const int32_t *__restrict__ clen, const Profile *__restrict__ ccnt,
+    const Profile *__restrict__ qcnt, int64_t nc, Sink sink,
+    int k_all, const int32_t *__restrict__ kvec, int64_t tile, int grab, int qgroup,
+    unsigned long long *counter)
+{
+    extern __shared__ u64 lds[];
+    WaveLds &L = ((WaveLds *)lds)[threadIdx.x >> 6];
+    const int lane = lane_id();
+    const int64_t nblk = (nc + tile - 1) / tile;
+    const int64_t ng = (nq + qgroup - 1) / qgroup;
+    const int64_t ntiles = nblk * ng;
+    int ns = 0;
+    auto drain = [&](int cnt) {
+        const int base = ns - cnt;
+        if (lane < cnt) {
+            const int64_t idx = L.open[base + lane];
+            const int64_t q = idx / nc;
+            const int64_t c = idx - q * nc;
+            const int mq = (int)(oq[q + 1] - oq[q]);
+            const int kq = kvec ? clamp_k_dev(kvec[q]) : k_all;
+            const u64 *qp = gpeq + (size_t)q * 256 * NWMAX;
+            const GlobalLo32<NWMAX> lo{qp};
+            const StripeText text{words + gbase[c >> 6] + (c & 63)};
+            sink.put(idx, peq_lane<NWMAX, 0>(lo, qp, mq, text, clen[c], kq), kq);
+        }
+        ns = uni(base);
+        wave_lds_sync();
+    };
+    Tickets tk;
+    for (;;) {
+        const long long t = tk.get(counter, grab);
+        if (t >= ntiles) break;
+        const int64_t blk = t / ng;
+        const int64_t q0 = (t % ng) * qgroup;
+        const int64_t q1 = q0 + qgroup < nq ? q0 + qgroup : nq;
+        const int64_t c0 = blk * tile;
+        const int64_t c1 = c0 + tile < nc ? c0 + tile : nc;
+        for (int64_t q = q0; q < q1; q++) {
+            const int mq = (int)(oq[q + 1] - oq[q]);
+            const int kq = kvec ? clamp_k_dev(kvec[q]) : k_all;
+            const int64_t orow = q * nc;
+            if (mq == 0 || mq > 64 * NWMAX) {
+                /* empty query: n, or k + 1 (csrc/gpu_kernels.cu:53-54); too long
+                 * for this launch: caller error, marked -1 */
+                for (int64_t c = c0 + lane; c < c1; c += 64)
+                    sink.put(orow + c, mq == 0 ? empty_pattern(clen[c], kq) : -1, kq);
+            } else if (kq < 0) {
+                const u64 *qp = gpeq + (size_t)q * 256 * NWMAX;
+                const GlobalLo32<NWMAX> lo{qp};
+                for (int64_t c = c0 + lane; c < c1; c += 64) {
+                    const StripeText text{words + gbase[c >> 6] + (c & 63)};
+                    sink.put(orow + c, peq_lane<NWMAX, 0>(lo, qp, mq, text, clen[c], kq), kq);
+                }
+            }
+        }
+        /* the group's lengths, thresholds and profiles, query q0 + j in lane j
+         * (qgroup <= 64), read back with readlane: no memory access per window */
+        const int gq = (int)(q1 - q0);
+        int lm = 0, lk = -1;
+        Profile lp{0u, 0u, 0u, 0u};
+        if (lane < gq) {
+            const int64_t q = q0 + lane;
+            lm = (int)(oq[q + 1] - oq[q]);
+            lk = kvec ? clamp_k_dev(kvec[q]) : k_all;
+            lp = qcnt[q];
+        }
+        const u64 okmask = wave_ballot(lane < gq && lm > 0 && lm <= 64 * NWMAX && lk >= 0);
+        if (okmask) {
+            if (!Sink::kDense) {
+                count_windows<false>(L, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, lp,
+                                     (u8 *)nullptr, false, ns, drain);
+            } else {
+                void *base = dense_out(sink);
+                const int dt = dense_dtype(sink);
+                const size_t esz = dt == DT_UINT8 ? 1 : dt == DT_UINT16 ? 2 : 4;
+                const bool aligned = (nc & 3) == 0 && ((uintptr_t)base & (4 * esz - 1)) == 0;
+                if (dt == DT_UINT8)
+                    count_windows<true>(L, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, lp,
+                                        (u8 *)base, aligned, ns, drain);
+                else if (dt == DT_UINT16)
+                    count_windows<true>(L, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, lp,
+                                        (uint16_t *)base, aligned, ns, drain);
+                else
+                    count_windows<true>(L, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, lp,
+                                        (int32_t *)base, aligned, ns, drain);
+            }
+        }
+        while (ns > 0) drain(ns < 64 ? ns : 64);
     }
 }
 
@@ -1151,11 +1414,17 @@ static int cdist_nwmax(int max_qlen)
     return words <= 1 ? 1 : words <= 2 ? 2 : words <= 4 ? 4 : 8;
 }
 
-/* Workspace: the work-queue counter (256 bytes), then the queries' Peq tables. */
+/* Workspace: the work-queue counter (256 bytes), the queries' Peq tables,
+ * then one 16-byte class profile per query. */
+static size_t cdist_peq_end(int64_t nq, int max_qlen)
+{
+    return 256 + (size_t)nq * 256 * cdist_nwmax(max_qlen) * sizeof(u64);
+}
+
 size_t bl_cdist_workspace_size(int64_t nq, int max_qlen)
 {
     if (nq < 0 || max_qlen < 0) return 0;
-    return 256 + (size_t)nq * 256 * cdist_nwmax(max_qlen) * sizeof(u64);
+    return cdist_peq_end(nq, max_qlen) + (size_t)nq * sizeof(Profile);
 }
 
 }  /* extern "C" */
@@ -1177,12 +1446,15 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
     /* tiles start on stripe groups (multiples of 64 choices; of 256 for the
      * four-choice filter's windows) */
     const int64_t tile = (prm && prm->tile > 0 ? (prm->tile + 255) / 256 * 256 : 8192);
-    const int mode = prm && prm->reserved[0] >= 1 && prm->reserved[0] <= 3 ? prm->reserved[0] : 0;
+    int mode = prm && prm->reserved[0] >= 1 && prm->reserved[0] <= 4 ? prm->reserved[0] : 0;
+    if (mode == 4 && !choices->counts) mode = 0;   /* count mode needs the choices' profiles */
     const int grab = prm && prm->reserved[1] > 0 ? prm->reserved[1] : 1;
+    const int qgroup = mode == 4 ? (prm->reserved[2] > 0 ? (prm->reserved[2] < 64 ? prm->reserved[2] : 64) : 16)
+                                 : 1;
     const int nwmax = cdist_nwmax(max_qlen);
     const size_t lds = sizeof(WaveLds) * 4;
     int64_t blocks = (int64_t)device_cus() * wpc / 4;
-    const int64_t ntiles = ((nc + tile - 1) / tile) * nq;
+    const int64_t ntiles = ((nc + tile - 1) / tile) * ((nq + qgroup - 1) / qgroup);
     const int64_t need = (ntiles + 3) / 4;
     if (blocks > need) blocks = need;
     if (blocks < 1) blocks = 1;
@@ -1190,31 +1462,29 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
     if (!ws) BL_HIP(hipMallocAsync((void **)&ws, need_ws, s));
     unsigned long long *ctr = (unsigned long long *)ws;
     u64 *gpeq = (u64 *)(ws + 256);
+    /* query class profiles after the Peq tables, when the choices carry theirs */
+    const Profile *ccnt = (const Profile *)choices->counts;
+    Profile *qcnt = ccnt ? (Profile *)(ws + cdist_peq_end(nq, max_qlen)) : nullptr;
     int rc = launch_clear(ctr, 1, s);
     if (rc == BL_OK && extra_clear) rc = launch_clear(extra_clear, 1, s);
     if (rc == BL_OK) {
-        peq_build_kernel<<<(unsigned)nq, 256, 0, s>>>(d_buf_q, d_off_q, nwmax, gpeq);
+        peq_build_kernel<<<(unsigned)nq, 256, 0, s>>>(d_buf_q, d_off_q, nwmax, gpeq, qcnt);
+        auto go = [&](auto nwc) {
+            constexpr int NW = decltype(nwc)::value;
+            if (mode == 4)
+                cdist_count_kernel<NW><<<(unsigned)blocks, 256, lds, s>>>(
+                    d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, nc,
+                    sink, k, d_kvec, tile, grab, qgroup, ctr);
+            else
+                cdist_batched_kernel<NW><<<(unsigned)blocks, 256, lds, s>>>(
+                    d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, nc,
+                    sink, k, d_kvec, tile, mode, grab, ctr);
+        };
         switch (nwmax) {
-        case 1:
-            cdist_batched_kernel<1><<<(unsigned)blocks, 256, lds, s>>>(
-                d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, nc, sink, k, d_kvec,
-                tile, mode, grab, ctr);
-            break;
-        case 2:
-            cdist_batched_kernel<2><<<(unsigned)blocks, 256, lds, s>>>(
-                d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, nc, sink, k, d_kvec,
-                tile, mode, grab, ctr);
-            break;
-        case 4:
-            cdist_batched_kernel<4><<<(unsigned)blocks, 256, lds, s>>>(
-                d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, nc, sink, k, d_kvec,
-                tile, mode, grab, ctr);
-            break;
-        default:
-            cdist_batched_kernel<8><<<(unsigned)blocks, 256, lds, s>>>(
-                d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, nc, sink, k, d_kvec,
-                tile, mode, grab, ctr);
-            break;
+        case 1: go(std::integral_constant<int, 1>()); break;
+        case 2: go(std::integral_constant<int, 2>()); break;
+        case 4: go(std::integral_constant<int, 4>()); break;
+        default: go(std::integral_constant<int, 8>()); break;
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) rc = fail(BL_EHIP, "%s launch: %s", who, hipGetErrorString(e));

@@ -1093,6 +1093,78 @@ BL_HD u32 swar_over4(u32 Pv, u32 Mv, int m, const int (&n)[4], int k)
     return (t + (u32)(3 - k) * 0x01010101u) & 0x80808080u;
 }
 
+/* ------------------------------------------------------------ class counts */
+/* A lower bound on the distance from byte-class counts, used by the search
+ * kernels before any scan. Bytes fall into 16 classes, class(b) = b & 15; a
+ * string's profile holds, in byte i of dword j, the number of its bytes of
+ * class 4j + i, saturated at 255.
+ *
+ * Let a_c, b_c be the true class counts of strings a, b and P = sum of
+ * max(a_c - b_c, 0), N = sum of max(b_c - a_c, 0). An insertion or deletion
+ * changes one count by one, a substitution lowers one count and raises
+ * another, so every edit lowers P by at most one and N by at most one; hence
+ * d(a, b) >= max(P, N) = (L1 + |len a - len b|) / 2 with L1 = P + N. With
+ * saturated counts, L1' <= L1 (clamping moves no two counts apart), so
+ * d >= L1' / 2 always, and when both strings are at most 255 bytes no count
+ * saturates and d >= (L1' + |len a - len b|) / 2. The bound is exact in the
+ * sense the results need: a pair it puts above k has d > k, and its stored
+ * result min(d, k + 1) = k + 1 is what the scan would give. */
+struct alignas(16) Profile {
+    u32 x, y, z, w;
+};
+
+BL_HD u32 sad_u8(u32 a, u32 b, u32 acc)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sad_u8(a, b, acc);
+#else
+    for (int i = 0; i < 32; i += 8) {
+        const int x = (int)((a >> i) & 255u), y = (int)((b >> i) & 255u);
+        acc += (u32)(x > y ? x - y : y - x);
+    }
+    return acc;
+#endif
+}
+
+/* Does the class-count bound put d(query, choice) above k? qc, cc: the
+ * profiles; m, n: the lengths. */
+BL_HD bool count_over(const Profile &qc, const Profile &cc, int m, int n, int k)
+{
+    u32 l1 = sad_u8(qc.x, cc.x, 0u);
+    l1 = sad_u8(qc.y, cc.y, l1);
+    l1 = sad_u8(qc.z, cc.z, l1);
+    l1 = sad_u8(qc.w, cc.w, l1);
+    const u32 gap = (u32)(n > m ? n - m : m - n);
+    const u32 twice = l1 + (m <= 255 && n <= 255 ? gap : 0u);
+    return twice > 2u * (u32)k;              /* k <= 2^30: no overflow */
+}
+
+/* The profile of s[0..n): 16 class counters of 16 bits in 8 dwords while
+ * counting, then saturated to bytes. */
+BL_HD Profile class_profile(const u8 *s, int n)
+{
+    u32 c16[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < n; i++) {
+        const u32 cls = s[i] & 15u;
+#pragma unroll
+        for (int t = 0; t < 8; t++)
+            c16[t] += (cls >> 1) == (u32)t ? 1u << (16 * (cls & 1u)) : 0u;
+    }
+    u32 w[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        u32 v = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int cls = 4 * j + i;
+            const u32 cnt = (c16[cls >> 1] >> (16 * (cls & 1))) & 0xFFFFu;
+            v |= (cnt > 255u ? 255u : cnt) << (8 * i);
+        }
+        w[j] = v;
+    }
+    return Profile{w[0], w[1], w[2], w[3]};
+}
+
 /* ------------------------------------------------------------ long strings */
 /* Pairs whose shorter string exceeds 256 bytes, and cross queries longer than
  * 512 bytes, run here instead of on the CPU (the reference falls back to its C

@@ -382,6 +382,19 @@ int blh_pairs_batched(const u8 *ba, const int64_t *oa, const u8 *bb, const int64
 }
 
 /* The same filter's SWAR test bound > k (k <= 3): bit 7 of byte s. */
+/* Byte-class profile of s[0..n) into out[0..3], and the class-count test of
+ * the search kernels for two strings at threshold k (1: d > k by the bound). */
+void blh_class_profile(const u8 *s, int n, uint32_t *out)
+{
+    const Profile p = class_profile(s, n);
+    out[0] = p.x; out[1] = p.y; out[2] = p.z; out[3] = p.w;
+}
+
+int blh_count_over(const u8 *a, int m, const u8 *b, int n, int k)
+{
+    return count_over(class_profile(a, m), class_profile(b, n), m, n, k) ? 1 : 0;
+}
+
 uint32_t blh_swar_over(const u8 *q, int m, const u8 *t, const int *n, int k)
 {
     std::vector<u8> lo8(256, 0);

@@ -94,14 +94,21 @@ int bl_pairs_class(int cap, const uint8_t *d_buf_a, const int64_t *d_off_a,
  * interleaved by dword, dword j of choice c at words[gbase[c / 64] + 64 * j +
  * c % 64] (zero-padded to the group's widest string, at least one dword), so
  * the lanes of a wave, holding consecutive choices, read 256 contiguous bytes
- * per step. len[c] is the length of choice c. All pointers are device memory.
- * The reference searches the packed buffer directly (csrc/gpu_kernels.cu:62-68,
- * strided byte loads); the stripes are this library's own resident format. */
+ * per step. len[c] is the length of choice c. counts (optional, 16-byte
+ * aligned) holds each choice's byte-class profile in dwords 4c .. 4c + 3: byte
+ * i of dword 4c + j counts the bytes b of choice c with b & 15 == 4j + i,
+ * saturated at 255. The batched search kernels use it for an exact lower
+ * bound on the distance, d >= (L1 + |m - n|) / 2 over the two profiles (d >=
+ * L1 / 2 if either string exceeds 255 bytes), before any scan; NULL turns
+ * that test off. All pointers are device memory. The reference searches the
+ * packed buffer directly (csrc/gpu_kernels.cu:62-68, strided byte loads); the
+ * stripes are this library's own resident format. */
 typedef struct {
     const uint32_t *words;   /* gbase[ceil(n / 64)] dwords */
     const int64_t *gbase;    /* ceil(n / 64) + 1 group bases, in dwords */
     const int32_t *len;      /* n lengths */
     int64_t n;               /* number of choices */
+    const uint32_t *counts;  /* 4 * n dwords of class profiles, or NULL */
 } bl_stripes;
 
 /* Host: group bases of the stripes of n strings with host offsets
@@ -109,10 +116,12 @@ typedef struct {
  * words array needs (h_gbase[ceil(n/64)]), or a negative status. */
 int64_t bl_stripes_layout(const int64_t *h_off, int64_t n, int64_t *h_gbase);
 
-/* Device: fill words and len from packed strings (d_buf, d_off) using the
- * group bases d_gbase that bl_stripes_layout computed for the same offsets. */
+/* Device: fill words, len and (unless d_counts is NULL) the class profiles
+ * from packed strings (d_buf, d_off) using the group bases d_gbase that
+ * bl_stripes_layout computed for the same offsets. */
 int bl_stripes_build(const uint8_t *d_buf, const int64_t *d_off, int64_t n,
-                     const int64_t *d_gbase, uint32_t *d_words, int32_t *d_len, void *stream);
+                     const int64_t *d_gbase, uint32_t *d_words, int32_t *d_len,
+                     uint32_t *d_counts, void *stream);
 
 /* dst[r][i] = src[r][inv[i]] for r < rows, i < cols (dtype BL_INT32 /
  * BL_UINT16 / BL_UINT8; src and dst distinct). No counterpart in the
@@ -154,8 +163,14 @@ typedef struct {
                               lanes, 2 four-choice byte filter for k <= 3 (queries of
                               8 to 256 bytes; other queries run compacted), 3 gap
                               windows for k >= 0 (the length test on 256 choices at
-                              a time, the in-band ones scanned 64 at a time);
-                              [1]: tiles per work-queue ticket (default 1) */
+                              a time, the in-band ones scanned 64 at a time), 4 count
+                              windows for k >= 0 (needs the choices' class profiles,
+                              else mode 0): a tile is a choice block against a group
+                              of queries, the length and class-count bounds settle
+                              most choices, the rest are rescanned with per-lane
+                              queries;
+                              [1]: tiles per work-queue ticket (default 1);
+                              [2]: count mode: queries per tile (default 16, at most 64) */
 } bl_cdist_params;
 
 typedef struct {

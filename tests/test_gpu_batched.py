@@ -85,7 +85,7 @@ def test_every_cdist_launch_shape_gives_the_same_results(gen, k):
     pq, pc = W.cdist(gen, 12, 5000, seed=3)
     Q, C = hip.Packed(*pq), hip.Packed(*pc)
     want = O.cross(pq, pc, k=k, threads=8)
-    space = {"mode": ["strided", "queue", "swar", "gap"], "waves_per_cu": [4, 32],
+    space = {"mode": ["strided", "queue", "swar", "gap", "count"], "waves_per_cu": [4, 32],
              "tile": [64, 1000, 8192]}
     for p in autotune._grid(space):
         out = hip.DeviceArray.empty((Q.n, C.n), np.uint8)
@@ -163,6 +163,81 @@ def test_gap_windows_edge_cases():
     np.testing.assert_array_equal(out.get(), np.where(kv[:, None] < 0, exact, np.minimum(exact, kv[:, None] + 1)))
 
 
+def test_class_count_bound_in_searches():
+    """The class-count bound the batched searches apply before any scan:
+    choices built to sit exactly at the bound (edits between distinct byte
+    classes, d = k and k + 1), anagrams of the query (same profile, large
+    distance), choices differing from the query only inside one class
+    (profile equal, d > 0), queries of 255 and 256 bytes of one byte (the
+    count saturates at 256), empty choices; the filter, gap and compacted
+    modes, dense and match-list output, with the profiles and without them."""
+    rng = np.random.default_rng(53)
+    base = bytes(rng.integers(97, 123, 64).astype(np.uint8))
+    Qs = [base, b"\x41" * 255, b"\x41" * 256, base[:20], bytes(rng.integers(0, 256, 200).astype(np.uint8)),
+          b""]
+    Cs = []
+    for i in range(4096):
+        q = Qs[i % (len(Qs) - 1)]
+        r = i % 9
+        b = bytearray(q)
+        if r == 0:
+            Cs.append(b"")
+            continue
+        if r == 1:                              # anagram
+            a = np.frombuffer(q, np.uint8).copy()
+            rng.shuffle(a)
+            Cs.append(a.tobytes())
+            continue
+        if r == 2:                              # same class: b ^ 0x10 keeps b & 15
+            for _ in range(int(rng.integers(1, 5))):
+                p_ = int(rng.integers(0, len(b)))
+                b[p_] ^= 0x10
+        elif r in (3, 4, 5):                    # t edits across classes
+            for _ in range(int(rng.integers(0, 6))):
+                op = int(rng.integers(0, 3))
+                p_ = int(rng.integers(0, len(b) + 1))
+                ch = int(rng.integers(0, 256))
+                if op == 0 and p_ < len(b):
+                    b[p_] = ch
+                elif op == 1:
+                    b.insert(p_, ch)
+                elif p_ < len(b):
+                    del b[p_]
+        elif r == 6:                            # one byte value, around 255/256 long
+            b = bytearray(b"\x41" * int(rng.integers(250, 258)))
+        elif r == 7 and i % 2:                  # far longer than any query: gap > k up to 300
+            b = bytearray(rng.integers(0, 256, int(rng.integers(300, 500))).astype(np.uint8))
+        else:
+            b = bytearray(rng.integers(0, 256, int(rng.integers(0, 260))).astype(np.uint8))
+        Cs.append(bytes(b))
+    pq, pc = O.pack(Qs), O.pack(Cs)
+    Q, C = hip.Packed(*pq), hip.Packed(*pc)
+    exact = O.cross(pq, pc, threads=8)
+    S = C.stripes()
+    assert S.desc.counts
+    saved = S.desc.counts
+    try:
+        for with_profiles in (True, False):
+            S.desc.counts = saved if with_profiles else None
+            for k in (0, 1, 2, 3, 5, 8, 300):
+                want = np.minimum(exact, k + 1)
+                for mode, extra in (("swar", {}), ("gap", {}), ("compacted", {}), ("count", {}),
+                                    ("count", {"qgroup": 2}), ("count", {"qgroup": 64})):
+                    for dt in (np.uint8, np.uint16, np.int32):
+                        out = hip.DeviceArray.empty((Q.n, C.n), dt)
+                        hip.cdist_batched_into(Q, C, out, k, None, None, {"mode": mode, "tile": 1024, **extra})
+                        np.testing.assert_array_equal(out.get(), O.saturate(want, dt),
+                                                      err_msg=f"k={k} {mode} {extra} {dt} profiles={with_profiles}")
+                for mode in ("swar", "count"):
+                    qi, ci, di = hip.cdist_matches_into(Q, C, k, params={"mode": mode})
+                    got = sorted(zip(qi.get().tolist(), ci.get().tolist(), di.get().tolist()))
+                    wq, wc = np.nonzero(exact <= k)
+                    assert got == sorted(zip(wq.tolist(), wc.tolist(), exact[wq, wc].tolist())), \
+                        (k, mode, with_profiles)
+    finally:
+        S.desc.counts = saved
+
+
 def test_filter_mode_with_every_choice_open():
     """The four-choice filter leaves every choice open when all are within k
     of the query (here: copies with at most one substitution, k = 3): the
@@ -184,13 +259,14 @@ def test_filter_mode_with_every_choice_open():
         kv = np.array(kq, np.int32)
         want = np.where(kv[:, None] < 0, exact, np.minimum(exact, kv[:, None] + 1))
         for p in ({"mode": "swar", "tile": 256}, {"mode": "swar", "tile": 8192},
-                  {"mode": "swar", "tile": 1024, "waves_per_cu": 1}):
+                  {"mode": "swar", "tile": 1024, "waves_per_cu": 1},
+                  {"mode": "count", "tile": 256, "qgroup": 3}, {"mode": "count", "waves_per_cu": 1}):
             out = hip.DeviceArray.empty((Q.n, C.n), np.int32)
             hip.cdist_batched_into(Q, C, out, -1, None, None, p, kvec=hip.DeviceBuffer.from_host(kv))
             np.testing.assert_array_equal(out.get(), want, err_msg=f"{kq} {p}")
 
 
-@pytest.mark.parametrize("mode", ["swar", "queue", "strided", "gap"])
+@pytest.mark.parametrize("mode", ["swar", "queue", "strided", "gap", "count"])
 def test_match_list_equals_the_dense_result(mode):
     """bl_cdist_matches returns exactly the cells the dense cdist puts at or
     below k (here with per-query thresholds, one of them negative: that query

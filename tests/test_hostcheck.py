@@ -427,3 +427,75 @@ def test_trim_matches_definition(hc, a, b):
     while s < n - p and a[len(a) - 1 - s] == b[len(b) - 1 - s]:
         s += 1
     assert list(res) == [p, len(a) - p - s, p, len(b) - p - s]
+
+
+def test_class_profiles_and_count_bound(hc):
+    """The search kernels' class-count bound: profiles are the saturated counts
+    of b & 15; whenever the bound puts a pair above k its exact distance (the
+    oracle) is above k; the bound is reached on pairs that make it tight (so
+    an off-by-one in the test or the gap term fails here); strings above 255
+    bytes fall back to L1 / 2."""
+    hc.blh_class_profile.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)]
+    hc.blh_count_over.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                  ctypes.c_int]
+    rng = np.random.default_rng(77)
+    for n in (0, 1, 15, 255, 256, 600):
+        s = bytes(rng.integers(0, 256, n).astype(np.uint8)) if n < 600 else b"\x13" * 600
+        out = (ctypes.c_uint32 * 4)()
+        hc.blh_class_profile(s, n, out)
+        want = np.minimum(np.bincount(np.frombuffer(s, np.uint8) & 15, minlength=16), 255)
+        got = np.frombuffer(bytes(out), np.uint8)
+        assert got.tolist() == want.tolist(), n
+
+    def over(a, b, k):
+        return bool(hc.blh_count_over(a, len(a), b, len(b), k))
+
+    # soundness against the exact distances
+    A, B = [], []
+    for _ in range(4000):
+        alpha = int(rng.choice([2, 4, 16, 26, 256]))
+        m = int(rng.integers(0, 300))
+        a = bytes(rng.integers(0, alpha, m).astype(np.uint8))
+        if rng.random() < 0.5:
+            b = bytearray(a)
+            for _ in range(int(rng.integers(0, 12))):
+                op = rng.integers(0, 3)
+                pos = int(rng.integers(0, len(b) + 1))
+                ch = int(rng.integers(0, alpha))
+                if op == 0 and pos < len(b):
+                    b[pos] = ch
+                elif op == 1:
+                    b.insert(pos, ch)
+                elif pos < len(b):
+                    del b[pos]
+            b = bytes(b)
+        else:
+            if rng.random() < 0.3:                # an anagram: same profile, any distance
+                b = np.frombuffer(a, np.uint8).copy()
+                rng.shuffle(b)
+                b = b.tobytes()
+            else:
+                b = bytes(rng.integers(0, alpha, int(rng.integers(0, 300))).astype(np.uint8))
+        A.append(a)
+        B.append(b)
+    d = O.pairs(O.pack(A), O.pack(B), threads=8)
+    pruned = 0
+    for a, b, dist in zip(A, B, d):
+        for k in (0, 1, 2, 3, 5, 8, 13, 30):
+            if over(a, b, k):
+                assert dist > k, (a, b, k, dist)
+                pruned += 1
+    assert pruned > 4000
+    # tight cases: d equals the bound, so k = d - 1 is pruned and k = d is not
+    tight = [(b"a" * 10, b"b" * 10), (b"acgt" * 8, b"acgt" * 8 + b"xyz"), (b"", b"q" * 7),
+             (b"abc", b"abcabc"), (b"\x01\x11\x21", b"\x02\x12\x22"), (b"a" * 255, b"b" * 255)]
+    for a, b in tight:
+        dist = int(O.pairs(O.pack([a]), O.pack([b]))[0])
+        assert over(a, b, dist - 1) and not over(a, b, dist), (a, b, dist)
+    # above 255 bytes: saturated counts, L1 / 2 only
+    a, b = b"a" * 300, b"b" * 300            # L1' = 510: bound 255 <= d = 300
+    assert over(a, b, 254) and not over(a, b, 255)
+    a, b = b"a" * 300, b"a" * 290            # counts saturate alike: no pruning (the gap test decides)
+    assert not over(a, b, 0)
+    # same counts in every class, far apart: the bound cannot prune (it is a bound, not a filter)
+    assert not over(b"ab" * 20, b"ba" * 20, 0)
