@@ -310,14 +310,27 @@ class _GpuExecutor:
     """Runs a fused batch on a slot's stream with the batched kernels. Each
     launch takes 1/slots of the GPU's wave slots, so batches of different
     slots run side by side instead of a long one holding every CU while the
-    others wait; searches use the four-choice filter where a query's
-    max_dist allows it (the launch falls back per query otherwise)."""
+    others wait. A search batch whose requests all have a max_dist runs the
+    count kernel (the batch's queries as one group against blocks of the
+    set, most choices settled by the length and class-count bounds), sized
+    so each slot's waves get about two blocks each; a batch with an exact
+    request uses the four-choice filter (the launch falls back per query)."""
 
     def __init__(self, slots: int = 4):
         wpc = max(4, 32 // max(1, int(slots)))
+        self.wpc = wpc
         self.cdist_params = {"mode": "swar", "waves_per_cu": wpc, "tile": 8192}
         self.pairs_params = {"waves_per_cu": wpc, "block": 256, "bin_width": 16, "chunk": 65536,
                              "trim": True}
+
+    def search_params(self, nq: int, nc: int, kv: np.ndarray) -> dict:
+        if len(kv) == 0 or int(kv.min()) < 0:
+            return self.cdist_params
+        g = min(64, nq)
+        waves = self.wpc * 256
+        blocks_wanted = max(1, 2 * waves // ((nq + g - 1) // g))
+        tile = min(8192, max(256, (nc // blocks_wanted) // 256 * 256))
+        return {"mode": "count", "waves_per_cu": self.wpc, "tile": tile, "qgroup": g}
 
     def new_slot(self, i):
         return _Slot(i)
@@ -381,8 +394,8 @@ class _GpuExecutor:
             out = DeviceArray.empty((Q.n, C.n), dt)
             inv = getattr(C, "col_order", None)
             tmp = DeviceArray.empty((Q.n, C.n), dt) if inv is not None else out
-            hip.cdist_batched_into(Q, C, tmp, -1, slot.stream, slot.workspace, self.cdist_params,
-                                   kvec=kvec)
+            hip.cdist_batched_into(Q, C, tmp, -1, slot.stream, slot.workspace,
+                                   self.search_params(Q.n, C.n, kv), kvec=kvec)
             if inv is not None:
                 call("bl_gather_cols", tmp.ptr, inv.ptr, out.ptr, Q.n, C.n, DTYPE_CODE[dt],
                      slot.stream)

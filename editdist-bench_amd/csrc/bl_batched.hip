@@ -802,17 +802,18 @@ template <bool DENSE, typename OutT, typename Drain>
 __device__ __forceinline__ void count_windows(WaveLds &L, const int32_t *__restrict__ clen,
                                               const Profile *__restrict__ ccnt, int64_t c0,
                                               int64_t c1, int64_t nc, int64_t q0, int gq, u64 okmask,
-                                              int lm, int lk, const Profile &lp, OutT *out,
-                                              bool rows_aligned, int &ns, Drain &drain)
+                                              int lm, int lk, bool uniform_k, const Profile &lp,
+                                              OutT *out, bool rows_aligned, int &ns, Drain &drain)
 {
     const int lane = lane_id();
-    for (int64_t w = c0; w < c1; w += 256) {
+    const int k_uni = __builtin_amdgcn_readfirstlane(lk);
+    auto window = [&](int64_t w, auto fullc) {
+        constexpr bool FULL = decltype(fullc)::value;
         const int64_t cl = w + 4 * lane;
-        const bool full = w + 256 <= c1;
         int n[4];
         bool valid[4];
         Profile cp[4];
-        if (full) {
+        if (FULL) {
             const int4 l4 = *(const int4 *)(clen + cl);
             n[0] = l4.x; n[1] = l4.y; n[2] = l4.z; n[3] = l4.w;
 #pragma unroll
@@ -836,13 +837,14 @@ __device__ __forceinline__ void count_windows(WaveLds &L, const int32_t *__restr
         for (int j = 0; j < gq; j++, orow += nc) {
             if (!((okmask >> j) & 1u)) continue;
             const int mq = __builtin_amdgcn_readlane(lm, j);
-            const int kq = __builtin_amdgcn_readlane(lk, j);
+            const int kq = uniform_k ? k_uni : __builtin_amdgcn_readlane(lk, j);
             const Profile pq{(u32)__builtin_amdgcn_readlane((int)lp.x, j),
                              (u32)__builtin_amdgcn_readlane((int)lp.y, j),
                              (u32)__builtin_amdgcn_readlane((int)lp.z, j),
                              (u32)__builtin_amdgcn_readlane((int)lp.w, j)};
             const u32 k2 = 2u * (u32)kq;                 /* kq <= 2^30 */
-            bool open[4];
+            /* the open choices as wave masks, straight from the compares */
+            u64 om[4];
             if (small && mq <= 255) {
 #pragma unroll
                 for (int s = 0; s < 4; s++) {
@@ -851,7 +853,7 @@ __device__ __forceinline__ void count_windows(WaveLds &L, const int32_t *__restr
                     t = sad_u8(pq.z, cp[s].z, t);
                     t = sad_u8(pq.w, cp[s].w, t);
                     t = sad_u8((u32)n[s], (u32)mq, t);
-                    open[s] = valid[s] & (t <= k2);
+                    om[s] = wave_ballot(valid[s] && t <= k2);
                 }
             } else {
 #pragma unroll
@@ -862,16 +864,16 @@ __device__ __forceinline__ void count_windows(WaveLds &L, const int32_t *__restr
                     t = sad_u8(pq.z, cp[s].z, t);
                     t = sad_u8(pq.w, cp[s].w, t);
                     t += (mq <= 255) & (n[s] <= 255) ? gap : 0u;
-                    open[s] = valid[s] & (gap <= (u32)kq) & (t <= k2);
+                    om[s] = wave_ballot(valid[s] && gap <= (u32)kq && t <= k2);
                 }
             }
             if (DENSE) {
                 const u32 kk1 = out_k1<OutT>(kq);
                 OutT *ow = out + orow + w;
-                if (full && (rows_aligned || ((uintptr_t)ow & (4 * sizeof(OutT) - 1)) == 0)) {
+                if (FULL && (rows_aligned || ((uintptr_t)ow & (4 * sizeof(OutT) - 1)) == 0)) {
                     /* k + 1 for all four; an open one is overwritten by its
                      * rescan later in this wave's program order */
-                    OutT *p = ow + 4 * lane;
+                    OutT *p = (OutT *)__builtin_assume_aligned(ow + 4 * lane, 4 * sizeof(OutT));
                     if (sizeof(OutT) == 1) *(u32 *)p = kk1 * 0x01010101u;
                     else if (sizeof(OutT) == 2) *(uint2 *)p = make_uint2(kk1 * 0x00010001u, kk1 * 0x00010001u);
                     else *(uint4 *)p = make_uint4(kk1, kk1, kk1, kk1);
@@ -881,18 +883,19 @@ __device__ __forceinline__ void count_windows(WaveLds &L, const int32_t *__restr
                         if (valid[s]) ow[4 * lane + s] = (OutT)kk1;
                 }
             }
-            const bool any = open[0] | open[1] | open[2] | open[3];
-            if (wave_ballot(any) == 0) continue;
+            if ((om[0] | om[1] | om[2] | om[3]) == 0) continue;
 #pragma unroll
             for (int s = 0; s < 4; s++) {
-                const u64 sm = wave_ballot(open[s]);
-                if (open[s]) L.open[ns + rank_below(sm)] = orow + cl + s;
-                ns = uni(ns + __popcll(sm));
+                if ((om[s] >> lane) & 1u) L.open[ns + rank_below(om[s])] = orow + cl + s;
+                ns = uni(ns + __popcll(om[s]));
             }
             wave_lds_sync();
             while (ns >= 64) drain(64);
         }
-    }
+    };
+    int64_t w = c0;
+    for (; w + 256 <= c1; w += 256) window(w, std::true_type());
+    if (w < c1) window(w, std::false_type());
 }
 
 /* Count mode: persistent waves pull (choice block, query group) tiles. Each
@@ -976,7 +979,7 @@ __global__ __launch_bounds__(256) void cdist_count_kernel(
         const u64 okmask = wave_ballot(lane < gq && lm > 0 && lm <= 64 * NWMAX && lk >= 0);
         if (okmask) {
             if (!Sink::kDense) {
-                count_windows<false>(L, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, lp,
+                count_windows<false>(L, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
                                      (u8 *)nullptr, false, ns, drain);
             } else {
                 void *base = dense_out(sink);
@@ -984,13 +987,13 @@ __global__ __launch_bounds__(256) void cdist_count_kernel(
                 const size_t esz = dt == DT_UINT8 ? 1 : dt == DT_UINT16 ? 2 : 4;
                 const bool aligned = (nc & 3) == 0 && ((uintptr_t)base & (4 * esz - 1)) == 0;
                 if (dt == DT_UINT8)
-                    count_windows<true>(L, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, lp,
+                    count_windows<true>(L, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
                                         (u8 *)base, aligned, ns, drain);
                 else if (dt == DT_UINT16)
-                    count_windows<true>(L, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, lp,
+                    count_windows<true>(L, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
                                         (uint16_t *)base, aligned, ns, drain);
                 else
-                    count_windows<true>(L, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, lp,
+                    count_windows<true>(L, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
                                         (int32_t *)base, aligned, ns, drain);
             }
         }
