@@ -435,12 +435,20 @@ def cdist(queries, choices, max_dist=None, dtype=np.int32, return_device: bool =
     q, c = as_list(queries), as_list(choices)
     k, dt = check_k(max_dist), check_dtype(dtype)
     nq, nc = len(q), len(c)
-    pq, pc = pack(q), pack(c)
+    # a long choice list is packed straight into pinned memory and uploaded
+    # from there (bitlev.ingest); a short one the ordinary way
+    staged = nc >= _PARALLEL_PACK
+    pq, pc = pack(q), (None if staged else pack(c))
     _require_device()
     out = DeviceArray.empty((nq, nc), dt)
     if nq == 0 or nc == 0:
         return out if return_device else np.empty((nq, nc), dt)
-    Q, C = Packed(*pq), Packed(*pc)
+    Q = Packed(*pq)
+    if staged:
+        from . import ingest
+        C = ingest.upload_packed(c)
+    else:
+        C = Packed(*pc)
     keep = run_cdist(Q, C, out, k, None)
     res = _finish(out, return_device)
     del keep

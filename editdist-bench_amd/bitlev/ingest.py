@@ -54,12 +54,13 @@ def _pool() -> concurrent.futures.ThreadPoolExecutor:
         return _pool_obj
 
 
-def pack(seq, threads: int | None = None):
+def pack(seq, threads: int | None = None, alloc=None):
     """(uint8 buffer, int64 offsets[n + 1]) of a list/tuple of str/bytes, the
     layout of _edpy.pack, packed by the thread pool in two passes: every
     thread measures a range of items, then copies it to its place in the
     buffer. Items are read in place without the GIL (the sequence must not be
-    changed during the call)."""
+    changed during the call). alloc(nbytes) -> uint8 array of at least nbytes
+    supplies the buffer (e.g. pinned memory); None allocates one."""
     if not isinstance(seq, (list, tuple)):
         seq = list(seq)
     n = len(seq)
@@ -76,7 +77,8 @@ def pack(seq, threads: int | None = None):
     totals = list(pool.map(measure, ranges))
     base = np.zeros(len(ranges) + 1, np.int64)
     np.cumsum(totals, out=base[1:])
-    buf = np.empty(max(int(base[-1]), 1), np.uint8)
+    need = max(int(base[-1]), 1)
+    buf = np.empty(need, np.uint8) if alloc is None else alloc(need)[:need]
     off = np.empty(n + 1, np.int64)
     off[0] = 0
     if not n:
@@ -154,6 +156,42 @@ class _Slot:
             p.free()
         call("bl_event_destroy", self.copied)
         call("bl_event_destroy", self.done)
+
+
+class _PinnedStage:
+    """One pinned host buffer, grown on demand and reused, for packing a set
+    right before its upload: a copy from pinned memory goes straight to the
+    device, a pageable one through the driver's bounce buffers."""
+
+    def __init__(self):
+        self.lock = threading.Lock()
+        self.mem = None
+
+    def take(self, nbytes: int) -> np.ndarray:
+        if self.mem is None or self.mem.nbytes < nbytes:
+            if self.mem is not None:
+                self.mem.free()
+            self.mem = _Pinned(max(nbytes, 1 << 20))
+        return self.mem.arr
+
+
+_stage = _PinnedStage()
+
+
+def upload_packed(strings, stream=None):
+    """hip.Packed of a list/tuple of str/bytes, packed by the thread pool into
+    a reused pinned buffer and uploaded from it (the stream is synchronised
+    before the buffer is reused). Falls back to ordinary packing when another
+    thread holds the buffer."""
+    if not _stage.lock.acquire(blocking=False):
+        return hip.Packed(*pack(strings), stream=stream)
+    try:
+        buf, off = pack(strings, alloc=_stage.take)
+        P = hip.Packed(buf, off, stream=stream)
+        call("bl_stream_sync", stream)
+        return P
+    finally:
+        _stage.lock.release()
 
 
 class _RetainingWorkspace(hip.Workspace):
@@ -284,11 +322,16 @@ _idle_lock = threading.Lock()
 
 
 def release():
-    """Free the pinned and device memory of every idle stager."""
+    """Free the pinned and device memory of every idle stager and the pinned
+    packing buffer."""
     with _idle_lock:
         stagers, _idle[:] = list(_idle), []
     for st in stagers:
         st.close()
+    with _stage.lock:
+        if _stage.mem is not None:
+            _stage.mem.free()
+            _stage.mem = None
 
 
 def pairwise(a, b, k: int = -1, dtype=np.int32, trim: bool = True, chunk: int = 1 << 17,

@@ -81,3 +81,24 @@ def test_staged_errors():
         ingest.pairwise([b"a"] * 3000, [b"b"] * 2999 + [7], chunk=1024, threads=2)
     with pytest.raises(ValueError):
         ingest.pairwise([b"a"], [b"b"], dtype=np.float32)
+
+
+def test_cdist_of_long_lists_through_pinned_staging():
+    """Choice lists of 32,768 strings or more are packed by the thread pool
+    into the reused pinned buffer and uploaded from it; two calls in a row
+    (the second with a shorter list) give the oracle's results."""
+    rng = np.random.default_rng(8)
+    Qs = [bytes(rng.integers(97, 101, m).astype(np.uint8)) for m in (0, 5, 40, 130)]
+    for nc in (40000, hip._PARALLEL_PACK):
+        C = []
+        for i in range(nc):
+            s = bytes(rng.integers(97, 101, int(rng.integers(0, 60))).astype(np.uint8))
+            C.append(s.decode("latin-1") if i % 2 else s)
+        pq, pc = O.pack(Qs), O.pack(_as_bytes(C))
+        exact = O.cross(pq, pc, threads=8)
+        for k, dt in ((None, np.int32), (3, np.uint8)):
+            got = hip.cdist(Qs, C, max_dist=k, dtype=dt)
+            want = exact if k is None else np.minimum(exact, k + 1)
+            np.testing.assert_array_equal(got, O.saturate(want, dt), err_msg=f"nc={nc} k={k}")
+    with pytest.raises(TypeError):
+        hip.cdist(Qs, C[:-1] + [None], max_dist=2)
