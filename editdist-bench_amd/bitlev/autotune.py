@@ -37,8 +37,8 @@ CDIST_SWAR_SPACE = {"mode": ["swar"], "waves_per_cu": [8, 16, 32], "tile": [2048
 CDIST_GAP_SPACE = {"mode": ["gap"], "waves_per_cu": [8, 16, 32], "tile": [2048, 8192, 32768]}
 # count windows (any threshold; choices with class profiles): a choice block
 # against a group of queries, the length and class-count bounds first
-CDIST_COUNT_SPACE = {"mode": ["count"], "waves_per_cu": [16, 32], "tile": [2048, 4096, 8192],
-                     "qgroup": [16, 32, 64]}
+CDIST_COUNT_SPACE = {"mode": ["count"], "waves_per_cu": [16, 32], "tile": [2048, 8192, 32768],
+                     "qgroup": [4, 16, 64]}
 PAIRS_SPACE = {"waves_per_cu": [8, 16, 32], "block": [64, 256], "bin_width": [8, 16, 32],
                "chunk": [16384, 65536, 262144], "trim": [True, False]}
 # direct mode (one launch, no bucketing): only the launch shape matters

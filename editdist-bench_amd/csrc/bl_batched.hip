@@ -666,6 +666,16 @@ struct GlobalLo32 {
     __device__ __forceinline__ u32 operator[](u32 c) const { return (u32)peq[c * NWMAX]; }
 };
 
+/* One wave's LDS in count mode: the open stack, the escalation stack, and
+ * the 32-row tables of up to GL queries of the current group (GL = 0: none). */
+template <int GL>
+struct CountLds {
+    int64_t open[CD_OCAP];
+    int64_t esc[CD_ECAP];       /* entries the 32-row stage left undecided */
+    u32 lo32[GL > 0 ? GL : 1][256];
+};
+constexpr int CD_GL = 4;        /* query groups up to this size keep their tables in LDS */
+
 /* Persistent cdist over queries of at most 64*NWMAX bytes whose Peq tables
  * peq_build_kernel wrote to gpeq. Waves pull (choice block, query) tiles,
  * block-major, from the work-queue counter. */
@@ -799,7 +809,7 @@ __global__ __launch_bounds__(256) void cdist_batched_kernel(
  * k + 1 (DENSE: the row at out + q * nc); the rest are pushed on the open
  * stack as output indices for the per-lane rescans (drain). */
 template <bool DENSE, typename OutT, typename Drain>
-__device__ __forceinline__ void count_windows(WaveLds &L, const int32_t *__restrict__ clen,
+__device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t *__restrict__ clen,
                                               const Profile *__restrict__ ccnt, int64_t c0,
                                               int64_t c1, int64_t nc, int64_t q0, int gq, u64 okmask,
                                               int lm, int lk, bool uniform_k, const Profile &lp,
@@ -886,7 +896,7 @@ __device__ __forceinline__ void count_windows(WaveLds &L, const int32_t *__restr
             if ((om[0] | om[1] | om[2] | om[3]) == 0) continue;
 #pragma unroll
             for (int s = 0; s < 4; s++) {
-                if ((om[s] >> lane) & 1u) L.open[ns + rank_below(om[s])] = orow + cl + s;
+                if ((om[s] >> lane) & 1u) open_stack[ns + rank_below(om[s])] = orow + cl + s;
                 ns = uni(ns + __popcll(om[s]));
             }
             wave_lds_sync();
@@ -902,11 +912,14 @@ __device__ __forceinline__ void count_windows(WaveLds &L, const int32_t *__restr
  * 256-choice window of the block is tested against every query of the group
  * (qgroup queries) with the length and class-count bounds, so a window's
  * lengths and profiles are loaded once per group and stay in registers; the
- * choices neither bound settles are stacked as output indices
- * and rescanned 64 at a time, each lane with its own query's global Peq
- * table. Empty, too-long and exact queries run lane-strided. Needs the
- * choices' and the queries' class profiles. */
-template <int NWMAX, typename Sink>
+ * choices neither bound settles are stacked as output indices and rescanned
+ * 64 at a time through the 32-row stage, each lane with its own query (the
+ * group's tables in LDS for groups of up to CD_GL queries, else the global
+ * Peq tables); the few pairs that stage leaves open (near matches) go on an
+ * escalation stack and through the wider stages 64 at a time. Empty,
+ * too-long and exact queries run lane-strided. Needs the choices' and the
+ * queries' class profiles. */
+template <int NWMAX, int GL, typename Sink>
 __global__ __launch_bounds__(256) void cdist_count_kernel(
     const int64_t *__restrict__ oq, int64_t nq, const u64 *__restrict__ gpeq,
     const u32 *__restrict__ words, const int64_t *__restrict__ gbase,
@@ -916,35 +929,89 @@ __global__ __launch_bounds__(256) void cdist_count_kernel(
     unsigned long long *counter)
 {
     extern __shared__ u64 lds[];
-    WaveLds &L = ((WaveLds *)lds)[threadIdx.x >> 6];
+    CountLds<GL> &L = ((CountLds<GL> *)lds)[threadIdx.x >> 6];
     const int lane = lane_id();
     const int64_t nblk = (nc + tile - 1) / tile;
     const int64_t ng = (nq + qgroup - 1) / qgroup;
     const int64_t ntiles = nblk * ng;
-    int ns = 0;
-    auto drain = [&](int cnt) {
-        const int base = ns - cnt;
+    int ns = 0, ne = 0;
+    int64_t q0 = 0;
+    /* pairs the 32-row stage leaves undecided (near matches, which need the
+     * whole pattern) go through the wider stages 64 at a time, so one hard
+     * pair does not hold 63 easy lanes for its whole scan */
+    auto escalate = [&](int cnt) {
+        const int base = ne - cnt;
         if (lane < cnt) {
-            const int64_t idx = L.open[base + lane];
+            const int64_t idx = L.esc[base + lane];
             const int64_t q = idx / nc;
             const int64_t c = idx - q * nc;
             const int mq = (int)(oq[q + 1] - oq[q]);
             const int kq = kvec ? clamp_k_dev(kvec[q]) : k_all;
             const u64 *qp = gpeq + (size_t)q * 256 * NWMAX;
-            const GlobalLo32<NWMAX> lo{qp};
             const StripeText text{words + gbase[c >> 6] + (c & 63)};
-            sink.put(idx, peq_lane<NWMAX, 0>(lo, qp, mq, text, clen[c], kq), kq);
+            sink.put(idx, peq_lane<NWMAX, 1>((const u32 *)nullptr, qp, mq, text, clen[c], kq), kq);
         }
+        ne = uni(base);
+        wave_lds_sync();
+    };
+    /* rescans: each lane takes an entry of its own query and runs the 32-row
+     * stage (the whole scan when the query fits it); with GL > 0 the group's
+     * 32-row tables sit in this wave's LDS, else they come from the global
+     * Peq tables */
+    auto drain = [&](int cnt) {
+        const int base = ns - cnt;
+        bool up = false;
+        int64_t idx = 0;
+        if (lane < cnt) {
+            idx = L.open[base + lane];
+            const int64_t q = idx / nc;
+            const int64_t c = idx - q * nc;
+            const int mq = (int)(oq[q + 1] - oq[q]);
+            const int kq = kvec ? clamp_k_dev(kvec[q]) : k_all;
+            const int n = clen[c];
+            const u64 *qp = gpeq + (size_t)q * 256 * NWMAX;
+            const StripeText text{words + gbase[c >> 6] + (c & 63)};
+            int d;
+            if ((n > mq ? n - mq : mq - n) > kq) {
+                d = kq + 1;                          /* (count windows settle these) */
+            } else if (n == 0) {
+                d = mq;
+            } else if (GL > 0) {
+                const u32 *lo = &L.lo32[(int)(q - q0)][0];
+                d = scan_text<u32, 1, true>(mq, 1, 32, text, n, kq,
+                                            [&](u32 ch, int) { return lo[ch]; });
+            } else {
+                const GlobalLo32<NWMAX> lo{qp};
+                d = scan_text<u32, 1, true>(mq, 1, 32, text, n, kq,
+                                            [&](u32 ch, int) { return lo[ch]; });
+            }
+            if (d >= 0) sink.put(idx, d, kq);
+            else up = true;                          /* rows past 32 needed */
+        }
+        const u64 um = wave_ballot(up);
+        if (up) L.esc[ne + rank_below(um)] = idx;
+        ne = uni(ne + __popcll(um));
         ns = uni(base);
         wave_lds_sync();
+        while (ne >= 64) escalate(64);
     };
     Tickets tk;
     for (;;) {
         const long long t = tk.get(counter, grab);
         if (t >= ntiles) break;
         const int64_t blk = t / ng;
-        const int64_t q0 = (t % ng) * qgroup;
+        q0 = (t % ng) * qgroup;
         const int64_t q1 = q0 + qgroup < nq ? q0 + qgroup : nq;
+        if (GL > 0) {
+            /* the group's 32-row tables (qgroup <= GL), for the rescans */
+            wave_lds_sync();
+            for (int j = 0; j < (int)(q1 - q0); j++) {
+                const u64 *qp = gpeq + (size_t)(q0 + j) * 256 * NWMAX;
+#pragma unroll
+                for (int x = 0; x < 4; x++) L.lo32[j][x * 64 + lane] = (u32)qp[(x * 64 + lane) * NWMAX];
+            }
+            wave_lds_sync();
+        }
         const int64_t c0 = blk * tile;
         const int64_t c1 = c0 + tile < nc ? c0 + tile : nc;
         for (int64_t q = q0; q < q1; q++) {
@@ -979,7 +1046,7 @@ __global__ __launch_bounds__(256) void cdist_count_kernel(
         const u64 okmask = wave_ballot(lane < gq && lm > 0 && lm <= 64 * NWMAX && lk >= 0);
         if (okmask) {
             if (!Sink::kDense) {
-                count_windows<false>(L, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
+                count_windows<false>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
                                      (u8 *)nullptr, false, ns, drain);
             } else {
                 void *base = dense_out(sink);
@@ -987,17 +1054,18 @@ __global__ __launch_bounds__(256) void cdist_count_kernel(
                 const size_t esz = dt == DT_UINT8 ? 1 : dt == DT_UINT16 ? 2 : 4;
                 const bool aligned = (nc & 3) == 0 && ((uintptr_t)base & (4 * esz - 1)) == 0;
                 if (dt == DT_UINT8)
-                    count_windows<true>(L, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
+                    count_windows<true>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
                                         (u8 *)base, aligned, ns, drain);
                 else if (dt == DT_UINT16)
-                    count_windows<true>(L, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
+                    count_windows<true>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
                                         (uint16_t *)base, aligned, ns, drain);
                 else
-                    count_windows<true>(L, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
+                    count_windows<true>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
                                         (int32_t *)base, aligned, ns, drain);
             }
         }
         while (ns > 0) drain(ns < 64 ? ns : 64);
+        while (ne > 0) escalate(ne < 64 ? ne : 64);
     }
 }
 
@@ -1474,8 +1542,12 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
         peq_build_kernel<<<(unsigned)nq, 256, 0, s>>>(d_buf_q, d_off_q, nwmax, gpeq, qcnt);
         auto go = [&](auto nwc) {
             constexpr int NW = decltype(nwc)::value;
-            if (mode == 4)
-                cdist_count_kernel<NW><<<(unsigned)blocks, 256, lds, s>>>(
+            if (mode == 4 && qgroup <= CD_GL)
+                cdist_count_kernel<NW, CD_GL><<<(unsigned)blocks, 256, sizeof(CountLds<CD_GL>) * 4, s>>>(
+                    d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, nc,
+                    sink, k, d_kvec, tile, grab, qgroup, ctr);
+            else if (mode == 4)
+                cdist_count_kernel<NW, 0><<<(unsigned)blocks, 256, sizeof(CountLds<0>) * 4, s>>>(
                     d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, nc,
                     sink, k, d_kvec, tile, grab, qgroup, ctr);
             else

@@ -287,6 +287,40 @@ struct StripeText {
     }
 };
 
+/* The same string read with four dwords (16 columns) in flight: for lanes
+ * that each scan a different, scattered string (rescans of single choices),
+ * where one dword ahead leaves every fourth column waiting on memory. The
+ * window shifts through registers (no indexed array). */
+struct DeepStripeStream {
+    const u32 *p, *last;           /* next dword to load; the string's last dword */
+    u32 b0, b1, b2, b3;
+    BL_HDM u32 load()
+    {
+        const u32 v = *(p <= last ? p : last);
+        p += 64;
+        return v;
+    }
+    BL_HDM u32 next4()
+    {
+        const u32 v = b0;
+        b0 = b1; b1 = b2; b2 = b3;
+        b3 = load();
+        return v;
+    }
+};
+
+struct DeepStripeText {
+    const u32 *first;
+    BL_HDM DeepStripeStream open(int n) const
+    {
+        DeepStripeStream s;
+        s.p = first;
+        s.last = first + 64 * (n > 0 ? (n - 1) >> 2 : 0);
+        s.b0 = s.load(); s.b1 = s.load(); s.b2 = s.load(); s.b3 = s.load();
+        return s;
+    }
+};
+
 /* ------------------------------------------------------------ output */
 BL_HD void store_out(void *out, int64_t i, int d, int dtype)
 {
