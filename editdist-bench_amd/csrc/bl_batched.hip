@@ -131,10 +131,16 @@ struct MatchSink {
     int32_t *md;
     int64_t cap, nc;
     static constexpr bool kDense = false;
+    /* The active lanes' matches take one run of the list with one atomic
+     * (the first active lane adds their number), each lane its rank in it. */
     __device__ __forceinline__ void put(int64_t idx, int d, int k) const
     {
-        if (d >= 0 && d <= k) {
-            const unsigned long long pos = atomicAdd(count, 1ull);
+        const bool hit = d >= 0 && d <= k;
+        const u64 m = wave_ballot(hit);
+        if (m == 0) return;
+        const unsigned long long base = (unsigned long long)wave_take(count, (long long)__popcll(m));
+        if (hit) {
+            const unsigned long long pos = base + (unsigned long long)rank_below(m);
             if (pos < (unsigned long long)cap) {
                 const int64_t q = idx / nc;
                 mq[pos] = (int32_t)q;
