@@ -888,11 +888,21 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
                 OutT *ow = out + orow + w;
                 if (FULL && (rows_aligned || ((uintptr_t)ow & (4 * sizeof(OutT) - 1)) == 0)) {
                     /* k + 1 for all four; an open one is overwritten by its
-                     * rescan later in this wave's program order */
+                     * rescan later in this wave's program order (same
+                     * address, same L2 channel: the later store lands last).
+                     * Non-temporal: the matrix is written once and streams
+                     * past L2 instead of evicting the choice data (C2: 0.70
+                     * -> 0.56 ms) */
                     OutT *p = (OutT *)__builtin_assume_aligned(ow + 4 * lane, 4 * sizeof(OutT));
-                    if (sizeof(OutT) == 1) *(u32 *)p = kk1 * 0x01010101u;
-                    else if (sizeof(OutT) == 2) *(uint2 *)p = make_uint2(kk1 * 0x00010001u, kk1 * 0x00010001u);
-                    else *(uint4 *)p = make_uint4(kk1, kk1, kk1, kk1);
+                    if (sizeof(OutT) == 1) {
+                        __builtin_nontemporal_store(kk1 * 0x01010101u, (u32 *)p);
+                    } else if (sizeof(OutT) == 2) {
+                        __builtin_nontemporal_store(kk1 * 0x00010001u, (u32 *)p);
+                        __builtin_nontemporal_store(kk1 * 0x00010001u, (u32 *)p + 1);
+                    } else {
+#pragma unroll
+                        for (int s = 0; s < 4; s++) __builtin_nontemporal_store(kk1, (u32 *)p + s);
+                    }
                 } else {
 #pragma unroll
                     for (int s = 0; s < 4; s++)
