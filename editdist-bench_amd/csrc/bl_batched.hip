@@ -862,14 +862,28 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
             /* the open choices as wave masks, straight from the compares */
             u64 om[4];
             if (small && mq <= 255) {
+                /* classes 0-7 and the gap first: a partial L1 is still a lower
+                 * bound, and it settles most choices; classes 8-15 only when a
+                 * lane of the wave is left open */
+                u32 t[4];
+                bool half = false;
 #pragma unroll
                 for (int s = 0; s < 4; s++) {
-                    u32 t = sad_u8(pq.x, cp[s].x, 0u);
-                    t = sad_u8(pq.y, cp[s].y, t);
-                    t = sad_u8(pq.z, cp[s].z, t);
-                    t = sad_u8(pq.w, cp[s].w, t);
-                    t = sad_u8((u32)n[s], (u32)mq, t);
-                    om[s] = wave_ballot(valid[s] && t <= k2);
+                    t[s] = sad_u8(pq.x, cp[s].x, 0u);
+                    t[s] = sad_u8(pq.y, cp[s].y, t[s]);
+                    t[s] = sad_u8((u32)n[s], (u32)mq, t[s]);
+                    half = half || (valid[s] && t[s] <= k2);
+                }
+                if (wave_ballot(half) == 0) {
+#pragma unroll
+                    for (int s = 0; s < 4; s++) om[s] = 0;
+                } else {
+#pragma unroll
+                    for (int s = 0; s < 4; s++) {
+                        u32 u = sad_u8(pq.z, cp[s].z, t[s]);
+                        u = sad_u8(pq.w, cp[s].w, u);
+                        om[s] = wave_ballot(valid[s] && u <= k2);
+                    }
                 }
             } else {
 #pragma unroll
