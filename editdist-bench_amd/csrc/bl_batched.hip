@@ -823,6 +823,15 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
 {
     const int lane = lane_id();
     const int k_uni = __builtin_amdgcn_readfirstlane(lk);
+    /* the common tile: every query of the group counted, one threshold, all
+     * query lengths within a byte, aligned rows -- then full windows whose
+     * choices fit a byte run a loop with nothing to decide per query */
+    const u64 group_mask = gq >= 64 ? ~0ull : ((1ull << gq) - 1ull);
+    const bool fast_tile = okmask == group_mask && uniform_k &&
+                           wave_ballot(lane < gq && lm > 255) == 0 && (!DENSE || rows_aligned);
+    const u32 k2_uni = 2u * (u32)k_uni;
+    const u32 fill_uni = out_k1<OutT>(k_uni) * (sizeof(OutT) == 1 ? 0x01010101u
+                                                : sizeof(OutT) == 2 ? 0x00010001u : 1u);
     auto window = [&](int64_t w, auto fullc) {
         constexpr bool FULL = decltype(fullc)::value;
         const int64_t cl = w + 4 * lane;
@@ -850,6 +859,46 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
         for (int s = 0; s < 4; s++) big = big || n[s] > 255;
         const bool small = wave_ballot(big) == 0;
         int64_t orow = q0 * nc;
+        if (FULL && fast_tile && small) {
+            for (int j = 0; j < gq; j++, orow += nc) {
+                const int mq = __builtin_amdgcn_readlane(lm, j);
+                const Profile pq{(u32)__builtin_amdgcn_readlane((int)lp.x, j),
+                                 (u32)__builtin_amdgcn_readlane((int)lp.y, j),
+                                 (u32)__builtin_amdgcn_readlane((int)lp.z, j),
+                                 (u32)__builtin_amdgcn_readlane((int)lp.w, j)};
+                u32 t[4];
+                bool half = false;
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    t[s] = sad_u8(pq.x, cp[s].x, 0u);
+                    t[s] = sad_u8(pq.y, cp[s].y, t[s]);
+                    t[s] = sad_u8((u32)n[s], (u32)mq, t[s]);
+                    half = half || t[s] <= k2_uni;
+                }
+                if (DENSE) {
+                    u32 *p = (u32 *)__builtin_assume_aligned(out + orow + w + 4 * lane, 4 * sizeof(OutT));
+#pragma unroll
+                    for (int x = 0; x < (int)sizeof(OutT); x++) __builtin_nontemporal_store(fill_uni, p + x);
+                }
+                if (wave_ballot(half) == 0) continue;
+                u64 om[4];
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    u32 u = sad_u8(pq.z, cp[s].z, t[s]);
+                    u = sad_u8(pq.w, cp[s].w, u);
+                    om[s] = wave_ballot(u <= k2_uni);
+                }
+                if ((om[0] | om[1] | om[2] | om[3]) == 0) continue;
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    if ((om[s] >> lane) & 1u) open_stack[ns + rank_below(om[s])] = orow + cl + s;
+                    ns = uni(ns + __popcll(om[s]));
+                }
+                wave_lds_sync();
+                while (ns >= 64) drain(64);
+            }
+            return;
+        }
         for (int j = 0; j < gq; j++, orow += nc) {
             if (!((okmask >> j) & 1u)) continue;
             const int mq = __builtin_amdgcn_readlane(lm, j);
