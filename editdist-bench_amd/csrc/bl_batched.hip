@@ -954,8 +954,8 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
                      * rescan later in this wave's program order (same
                      * address, same L2 channel: the later store lands last).
                      * Non-temporal: the matrix is written once and streams
-                     * past L2 instead of evicting the choice data (C2: 0.70
-                     * -> 0.56 ms) */
+                     * past L2 instead of evicting the choice data (1,024 x
+                     * 1e6 length-64 search at k = 2: 0.70 -> 0.56 ms) */
                     OutT *p = (OutT *)__builtin_assume_aligned(ow + 4 * lane, 4 * sizeof(OutT));
                     if (sizeof(OutT) == 1) {
                         __builtin_nontemporal_store(kk1 * 0x01010101u, (u32 *)p);
