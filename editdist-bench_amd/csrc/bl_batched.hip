@@ -814,7 +814,7 @@ __global__ __launch_bounds__(256) void cdist_batched_kernel(
  * (L1 >= gap, so L1 + gap > 2k already covers gap > k). Settled choices get
  * k + 1 (DENSE: the row at out + q * nc); the rest are pushed on the open
  * stack as output indices for the per-lane rescans (drain). */
-template <bool DENSE, typename OutT, typename Drain>
+template <bool DENSE, bool PAIRS8, typename OutT, typename Drain>
 __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t *__restrict__ clen,
                                               const Profile *__restrict__ ccnt, int64_t c0,
                                               int64_t c1, int64_t nc, int64_t q0, int gq, u64 okmask,
@@ -983,6 +983,79 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
         }
     };
     int64_t w = c0;
+    /* pairs of full windows through the fast loop: each lane holds eight
+     * choices (four in each window), so a query's per-pass work (readlanes,
+     * loop, store address, ballots) covers twice the pairs (PAIRS8: only
+     * where the rescans' registers leave room -- queries of one word) */
+    auto window2 = [&](int64_t w) {
+        const int64_t cl = w + 4 * lane;
+        int n[8];
+        Profile cp[8];
+        const int4 la = *(const int4 *)(clen + cl);
+        const int4 lb = *(const int4 *)(clen + cl + 256);
+        n[0] = la.x; n[1] = la.y; n[2] = la.z; n[3] = la.w;
+        n[4] = lb.x; n[5] = lb.y; n[6] = lb.z; n[7] = lb.w;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            cp[s] = ccnt[cl + s];
+            cp[4 + s] = ccnt[cl + 256 + s];
+        }
+        bool big = false;
+#pragma unroll
+        for (int s = 0; s < 8; s++) big = big || n[s] > 255;
+        if (wave_ballot(big) != 0) {
+            window(w, std::true_type());
+            window(w + 256, std::true_type());
+            return;
+        }
+        int64_t orow = q0 * nc;
+        for (int j = 0; j < gq; j++, orow += nc) {
+            const int mq = __builtin_amdgcn_readlane(lm, j);
+            const Profile pq{(u32)__builtin_amdgcn_readlane((int)lp.x, j),
+                             (u32)__builtin_amdgcn_readlane((int)lp.y, j),
+                             (u32)__builtin_amdgcn_readlane((int)lp.z, j),
+                             (u32)__builtin_amdgcn_readlane((int)lp.w, j)};
+            u32 t[8];
+            bool half = false;
+#pragma unroll
+            for (int s = 0; s < 8; s++) {
+                t[s] = sad_u8(pq.x, cp[s].x, 0u);
+                t[s] = sad_u8(pq.y, cp[s].y, t[s]);
+                t[s] = sad_u8((u32)n[s], (u32)mq, t[s]);
+                half = half || t[s] <= k2_uni;
+            }
+            if (DENSE) {
+                u32 *p0 = (u32 *)__builtin_assume_aligned(out + orow + w + 4 * lane, 4 * sizeof(OutT));
+                u32 *p1 = (u32 *)__builtin_assume_aligned(out + orow + w + 256 + 4 * lane, 4 * sizeof(OutT));
+#pragma unroll
+                for (int x = 0; x < (int)sizeof(OutT); x++) {
+                    __builtin_nontemporal_store(fill_uni, p0 + x);
+                    __builtin_nontemporal_store(fill_uni, p1 + x);
+                }
+            }
+            if (wave_ballot(half) == 0) continue;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                u64 om[4];
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    u32 u = sad_u8(pq.z, cp[4 * h + s].z, t[4 * h + s]);
+                    u = sad_u8(pq.w, cp[4 * h + s].w, u);
+                    om[s] = wave_ballot(u <= k2_uni);
+                }
+                if ((om[0] | om[1] | om[2] | om[3]) == 0) continue;
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    if ((om[s] >> lane) & 1u) open_stack[ns + rank_below(om[s])] = orow + cl + 256 * h + s;
+                    ns = uni(ns + __popcll(om[s]));
+                }
+                wave_lds_sync();
+                while (ns >= 64) drain(64);
+            }
+        }
+    };
+    if (PAIRS8 && fast_tile)
+        for (; w + 512 <= c1; w += 512) window2(w);
     for (; w + 256 <= c1; w += 256) window(w, std::true_type());
     if (w < c1) window(w, std::false_type());
 }
@@ -1125,7 +1198,7 @@ __global__ __launch_bounds__(256) void cdist_count_kernel(
         const u64 okmask = wave_ballot(lane < gq && lm > 0 && lm <= 64 * NWMAX && lk >= 0);
         if (okmask) {
             if (!Sink::kDense) {
-                count_windows<false>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
+                count_windows<false, NWMAX == 1>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
                                      (u8 *)nullptr, false, ns, drain);
             } else {
                 void *base = dense_out(sink);
@@ -1133,13 +1206,13 @@ __global__ __launch_bounds__(256) void cdist_count_kernel(
                 const size_t esz = dt == DT_UINT8 ? 1 : dt == DT_UINT16 ? 2 : 4;
                 const bool aligned = (nc & 3) == 0 && ((uintptr_t)base & (4 * esz - 1)) == 0;
                 if (dt == DT_UINT8)
-                    count_windows<true>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
+                    count_windows<true, NWMAX == 1>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
                                         (u8 *)base, aligned, ns, drain);
                 else if (dt == DT_UINT16)
-                    count_windows<true>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
+                    count_windows<true, NWMAX == 1>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
                                         (uint16_t *)base, aligned, ns, drain);
                 else
-                    count_windows<true>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
+                    count_windows<true, NWMAX == 1>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
                                         (int32_t *)base, aligned, ns, drain);
             }
         }

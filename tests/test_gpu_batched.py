@@ -376,3 +376,28 @@ def test_gather_cols_every_dtype():
             call("bl_gather_cols", ds.ptr, dinv.ptr, dd.ptr, rows, cols, hip.DTYPE_CODE[np.dtype(dt)], None)
             call("bl_device_sync")
             np.testing.assert_array_equal(dd.get(), src[:, inv])
+
+
+def test_count_kernel_fast_paths_every_output():
+    """The count kernel's fast loops (one threshold for all queries, byte
+    lengths, aligned rows; pairs of windows for one-word queries) for every
+    output dtype and the match list, on a search workload with near
+    duplicates, against the oracle; an odd choice count leaves a tail window
+    and unaligned rows."""
+    for nc in (6000, 6003):
+        pq, pc = W.cdist(2, 20, nc, seed=21)
+        Q, C = hip.Packed(*pq), hip.Packed(*pc)
+        exact = O.cross(pq, pc, threads=8)
+        for k in (0, 2, 3):
+            want = np.minimum(exact, k + 1)
+            for tile, g in ((2048, 64), (1024, 8), (512, 3)):
+                p = {"mode": "count", "tile": tile, "qgroup": g}
+                for dt in (np.uint8, np.uint16, np.int32):
+                    out = hip.DeviceArray.empty((Q.n, C.n), dt)
+                    hip.cdist_batched_into(Q, C, out, k, None, None, p)
+                    np.testing.assert_array_equal(out.get(), O.saturate(want, dt),
+                                                  err_msg=f"nc={nc} k={k} {p} {dt}")
+                qi, ci, di = hip.cdist_matches_into(Q, C, k, params=p)
+                got = sorted(zip(qi.get().tolist(), ci.get().tolist(), di.get().tolist()))
+                wq, wc = np.nonzero(exact <= k)
+                assert got == sorted(zip(wq.tolist(), wc.tolist(), exact[wq, wc].tolist())), (nc, k, p)
