@@ -50,7 +50,7 @@ SIGNATURES = {
     "bl_graph_destroy": [_vp],
     "bl_pairs_class": [_int, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _int, _int, _int, _vp],
     "bl_stripes_layout": [_vp, _i64, _vp],
-    "bl_stripes_build": [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp],
+    "bl_stripes_build": [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
     "bl_gather_cols": [_vp, _vp, _vp, _i64, _i64, _int, _vp],
     "bl_cdist_class": [_int, _vp, _vp, _vp, _i64, _vp, _vp, _int, _int, _vp],
     "bl_long": [_int, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _int, _int, _int, _i64, _vp],
@@ -70,7 +70,8 @@ _RESTYPE = {"bl_last_error": _c.c_char_p, "bl_cdist_workspace_size": _sz,
 
 class Stripes(ctypes.Structure):
     """bl_stripes (include/bitlev_hip.h)."""
-    _fields_ = [("words", _vp), ("gbase", _vp), ("len", _vp), ("n", _i64), ("counts", _vp)]
+    _fields_ = [("words", _vp), ("gbase", _vp), ("len", _vp), ("n", _i64), ("counts", _vp),
+                ("counts2", _vp)]
 
 
 class CdistParams(ctypes.Structure):

@@ -131,8 +131,9 @@ class Packed:
 class ChoiceStripes:
     """A packed set interleaved by dword in groups of 64 strings (bl_stripes):
     the layout the cdist kernels read, so that a wave's lanes load 256
-    contiguous bytes per step, plus each string's 16-byte byte-class profile
-    (the batched searches' class-count bound; profiles=False leaves it out)."""
+    contiguous bytes per step, plus each string's 16-byte byte-class and
+    2-gram profiles (the batched searches' lower bounds; profiles=False leaves
+    them out)."""
 
     def __init__(self, P: Packed, stream=None, profiles: bool = True):
         n = P.n
@@ -142,15 +143,17 @@ class ChoiceStripes:
         self.d_words = DeviceBuffer(max(int(total), 1) * 4)
         self.d_len = DeviceBuffer(max(n, 1) * 4)
         self.d_counts = DeviceBuffer(max(n, 1) * 16) if profiles else None
+        self.d_counts2 = DeviceBuffer(max(n, 1) * 16) if profiles else None
         counts = self.d_counts.ptr if profiles else None
+        counts2 = self.d_counts2.ptr if profiles else None
         call("bl_stripes_build", P.d_buf.ptr, P.d_off.ptr, n, self.d_gbase.ptr, self.d_words.ptr,
-             self.d_len.ptr, counts, stream)
+             self.d_len.ptr, counts, counts2, stream)
         # built once per set; finish here so that any stream may read them next
         call("bl_stream_sync", stream)
         self.nbytes = self.d_gbase.nbytes + self.d_words.nbytes + self.d_len.nbytes
         if profiles:
-            self.nbytes += self.d_counts.nbytes
-        self.desc = Stripes(self.d_words.ptr, self.d_gbase.ptr, self.d_len.ptr, n, counts)
+            self.nbytes += self.d_counts.nbytes + self.d_counts2.nbytes
+        self.desc = Stripes(self.d_words.ptr, self.d_gbase.ptr, self.d_len.ptr, n, counts, counts2)
 
     @property
     def ref(self):

@@ -186,16 +186,21 @@ struct WaveLds {
  * 64 * nwmax (a caller error, answered -1 by the search kernel) gets zeros. */
 __global__ __launch_bounds__(256) void peq_build_kernel(const u8 *__restrict__ bq,
                                                         const int64_t *__restrict__ oq, int nwmax,
-                                                        u64 *__restrict__ peq, Profile *__restrict__ qcnt)
+                                                        u64 *__restrict__ peq, Profile *__restrict__ qcnt,
+                                                        Profile *__restrict__ qcnt2)
 {
     __shared__ u8 qs[512];
     __shared__ u32 cls[16];
+    __shared__ u32 cls2[16];
     const int64_t q = blockIdx.x;
     const int64_t q0 = oq[q];
     int m = (int)(oq[q + 1] - q0);
     if (m > 64 * nwmax) m = 0;
     for (int i = threadIdx.x; i < m; i += 256) qs[i] = bq[q0 + i];
-    if (threadIdx.x < 16) cls[threadIdx.x] = 0;
+    if (threadIdx.x < 16) {
+        cls[threadIdx.x] = 0;
+        cls2[threadIdx.x] = 0;
+    }
     __syncthreads();
     const u32 c = threadIdx.x;
     u64 *row = peq + ((size_t)q * 256 + c) * nwmax;
@@ -211,14 +216,19 @@ __global__ __launch_bounds__(256) void peq_build_kernel(const u8 *__restrict__ b
     /* the query's byte-class profile (the class-count bound's query side):
      * the occurrences of byte c are the set bits of its Peq row */
     atomicAdd(&cls[c & 15u], cnt);
+    if (qcnt2)                               /* the 2-gram profile, from the bytes */
+        for (int i = threadIdx.x; i + 1 < m; i += 256)
+            atomicAdd(&cls2[4u * class2(qs[i]) + class2(qs[i + 1])], 1u);
     __syncthreads();
     if (threadIdx.x < 4) {
-        u32 v = 0;
+        u32 v = 0, v2 = 0;
         for (int i = 0; i < 4; i++) {
-            const u32 x = cls[4 * threadIdx.x + i];
+            const u32 x = cls[4 * threadIdx.x + i], y = cls2[4 * threadIdx.x + i];
             v |= (x > 255u ? 255u : x) << (8 * i);
+            v2 |= (y > 255u ? 255u : y) << (8 * i);
         }
         ((u32 *)(qcnt + q))[threadIdx.x] = v;
+        if (qcnt2) ((u32 *)(qcnt2 + q))[threadIdx.x] = v2;
     }
 }
 
@@ -677,6 +687,7 @@ struct GlobalLo32 {
 template <int GL>
 struct CountLds {
     int64_t open[CD_OCAP];
+    int64_t scan[CD_ECAP];      /* open entries the 2-gram bound leaves for a scan */
     int64_t esc[CD_ECAP];       /* entries the 32-row stage left undecided */
     u32 lo32[GL > 0 ? GL : 1][256];
 };
@@ -819,6 +830,7 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
                                               const Profile *__restrict__ ccnt, int64_t c0,
                                               int64_t c1, int64_t nc, int64_t q0, int gq, u64 okmask,
                                               int lm, int lk, bool uniform_k, const Profile &lp,
+
                                               OutT *out, bool rows_aligned, int &ns, Drain &drain)
 {
     const int lane = lane_id();
@@ -1076,7 +1088,8 @@ __global__ __launch_bounds__(256) void cdist_count_kernel(
     const int64_t *__restrict__ oq, int64_t nq, const u64 *__restrict__ gpeq,
     const u32 *__restrict__ words, const int64_t *__restrict__ gbase,
     const int32_t *__restrict__ clen, const Profile *__restrict__ ccnt,
-    const Profile *__restrict__ qcnt, int64_t nc, Sink sink,
+    const Profile *__restrict__ qcnt, const Profile *__restrict__ ccnt2,
+    const Profile *__restrict__ qcnt2, int64_t nc, Sink sink,
     int k_all, const int32_t *__restrict__ kvec, int64_t tile, int grab, int qgroup,
     unsigned long long *counter)
 {
@@ -1086,7 +1099,7 @@ __global__ __launch_bounds__(256) void cdist_count_kernel(
     const int64_t nblk = (nc + tile - 1) / tile;
     const int64_t ng = (nq + qgroup - 1) / qgroup;
     const int64_t ntiles = nblk * ng;
-    int ns = 0, ne = 0;
+    int ns = 0, nsc = 0, ne = 0;
     int64_t q0 = 0;
     /* pairs the 32-row stage leaves undecided (near matches, which need the
      * whole pattern) go through the wider stages 64 at a time, so one hard
@@ -1110,12 +1123,12 @@ __global__ __launch_bounds__(256) void cdist_count_kernel(
      * stage (the whole scan when the query fits it); with GL > 0 the group's
      * 32-row tables sit in this wave's LDS, else they come from the global
      * Peq tables */
-    auto drain = [&](int cnt) {
-        const int base = ns - cnt;
+    auto scan_drain = [&](int cnt) {
+        const int base = nsc - cnt;
         bool up = false;
         int64_t idx = 0;
         if (lane < cnt) {
-            idx = L.open[base + lane];
+            idx = L.scan[base + lane];
             const int64_t q = idx / nc;
             const int64_t c = idx - q * nc;
             const int mq = (int)(oq[q + 1] - oq[q]);
@@ -1143,9 +1156,34 @@ __global__ __launch_bounds__(256) void cdist_count_kernel(
         const u64 um = wave_ballot(up);
         if (up) L.esc[ne + rank_below(um)] = idx;
         ne = uni(ne + __popcll(um));
-        ns = uni(base);
+        nsc = uni(base);
         wave_lds_sync();
         while (ne >= 64) escalate(64);
+    };
+    /* open entries first meet the 2-gram bound (when the choices carry
+     * 2-gram profiles); those it settles already hold k + 1, the rest are
+     * compacted onto the scan stack, so scans run 64 lanes at a time */
+    auto drain = [&](int cnt) {
+        const int base = ns - cnt;
+        bool keep = false;
+        int64_t idx = 0;
+        if (lane < cnt) {
+            idx = L.open[base + lane];
+            keep = true;
+            if (ccnt2) {
+                const int64_t q = idx / nc;
+                const int64_t c = idx - q * nc;
+                const int mq = (int)(oq[q + 1] - oq[q]);
+                const int kq = kvec ? clamp_k_dev(kvec[q]) : k_all;
+                keep = !count2_over(qcnt2[q], ccnt2[c], mq, clen[c], kq);
+            }
+        }
+        const u64 km = wave_ballot(keep);
+        if (keep) L.scan[nsc + rank_below(km)] = idx;
+        nsc = uni(nsc + __popcll(km));
+        ns = uni(base);
+        wave_lds_sync();
+        while (nsc >= 64) scan_drain(64);
     };
     Tickets tk;
     for (;;) {
@@ -1217,6 +1255,7 @@ __global__ __launch_bounds__(256) void cdist_count_kernel(
             }
         }
         while (ns > 0) drain(ns < 64 ? ns : 64);
+        while (nsc > 0) scan_drain(nsc < 64 ? nsc : 64);
         while (ne > 0) escalate(ne < 64 ? ne : 64);
     }
 }
@@ -1638,7 +1677,7 @@ static int cdist_nwmax(int max_qlen)
 }
 
 /* Workspace: the work-queue counter (256 bytes), the queries' Peq tables,
- * then one 16-byte class profile per query. */
+ * then one 16-byte class profile and one 2-gram profile per query. */
 static size_t cdist_peq_end(int64_t nq, int max_qlen)
 {
     return 256 + (size_t)nq * 256 * cdist_nwmax(max_qlen) * sizeof(u64);
@@ -1647,7 +1686,7 @@ static size_t cdist_peq_end(int64_t nq, int max_qlen)
 size_t bl_cdist_workspace_size(int64_t nq, int max_qlen)
 {
     if (nq < 0 || max_qlen < 0) return 0;
-    return cdist_peq_end(nq, max_qlen) + (size_t)nq * sizeof(Profile);
+    return cdist_peq_end(nq, max_qlen) + 2 * (size_t)nq * sizeof(Profile);
 }
 
 }  /* extern "C" */
@@ -1688,20 +1727,22 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
     /* query class profiles after the Peq tables, when the choices carry theirs */
     const Profile *ccnt = (const Profile *)choices->counts;
     Profile *qcnt = ccnt ? (Profile *)(ws + cdist_peq_end(nq, max_qlen)) : nullptr;
+    const Profile *ccnt2 = ccnt ? (const Profile *)choices->counts2 : nullptr;
+    Profile *qcnt2 = ccnt2 ? (Profile *)(ws + cdist_peq_end(nq, max_qlen)) + nq : nullptr;
     int rc = launch_clear(ctr, 1, s);
     if (rc == BL_OK && extra_clear) rc = launch_clear(extra_clear, 1, s);
     if (rc == BL_OK) {
-        peq_build_kernel<<<(unsigned)nq, 256, 0, s>>>(d_buf_q, d_off_q, nwmax, gpeq, qcnt);
+        peq_build_kernel<<<(unsigned)nq, 256, 0, s>>>(d_buf_q, d_off_q, nwmax, gpeq, qcnt, qcnt2);
         auto go = [&](auto nwc) {
             constexpr int NW = decltype(nwc)::value;
             if (mode == 4 && qgroup <= CD_GL)
                 cdist_count_kernel<NW, CD_GL><<<(unsigned)blocks, 256, sizeof(CountLds<CD_GL>) * 4, s>>>(
-                    d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, nc,
-                    sink, k, d_kvec, tile, grab, qgroup, ctr);
+                    d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, ccnt2,
+                    qcnt2, nc, sink, k, d_kvec, tile, grab, qgroup, ctr);
             else if (mode == 4)
                 cdist_count_kernel<NW, 0><<<(unsigned)blocks, 256, sizeof(CountLds<0>) * 4, s>>>(
-                    d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, nc,
-                    sink, k, d_kvec, tile, grab, qgroup, ctr);
+                    d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, ccnt2,
+                    qcnt2, nc, sink, k, d_kvec, tile, grab, qgroup, ctr);
             else
                 cdist_batched_kernel<NW><<<(unsigned)blocks, 256, lds, s>>>(
                     d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, nc,

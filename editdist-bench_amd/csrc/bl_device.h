@@ -1173,6 +1173,29 @@ BL_HD bool count_over(const Profile &qc, const Profile &cc, int m, int n, int k)
     return twice > 2u * (u32)k;              /* k <= 2^30: no overflow */
 }
 
+/* A second bound from 2-grams. Byte b falls into class c2(b) = (b >> 1) & 3
+ * (A, C, G, T apart), a 2-gram (x, y) into class 4 c2(x) + c2(y); the
+ * profile holds the saturated counts of the 16 classes over the string's
+ * max(n - 1, 0) 2-grams. With P and N the excess and the deficit of a's
+ * counts against b's: a substitution removes two 2-grams and adds two, an
+ * insertion removes one and adds two, a deletion removes two and adds one,
+ * so each edit lowers P by at most two and N by at most two, and d >=
+ * max(P, N) / 2 = (L1 + |t_a - t_b|) / 4 with t the 2-gram totals. Counts
+ * saturate only past 256 bytes; then d >= L1' / 4 alone. */
+BL_HD u32 class2(u32 b) { return (b >> 1) & 3u; }
+
+BL_HD bool count2_over(const Profile &qc, const Profile &cc, int m, int n, int k)
+{
+    u32 l1 = sad_u8(qc.x, cc.x, 0u);
+    l1 = sad_u8(qc.y, cc.y, l1);
+    l1 = sad_u8(qc.z, cc.z, l1);
+    l1 = sad_u8(qc.w, cc.w, l1);
+    const int ta = m > 0 ? m - 1 : 0, tb = n > 0 ? n - 1 : 0;
+    const u32 gap = (u32)(ta > tb ? ta - tb : tb - ta);
+    const u32 four = l1 + (m <= 256 && n <= 256 ? gap : 0u);
+    return (u64)four > 4ull * (u64)k;       /* 4k overflows 32 bits at k = 2^30 */
+}
+
 /* The profile of s[0..n): 16 class counters of 16 bits in 8 dwords while
  * counting, then saturated to bytes. */
 BL_HD Profile class_profile(const u8 *s, int n)
@@ -1180,6 +1203,31 @@ BL_HD Profile class_profile(const u8 *s, int n)
     u32 c16[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = 0; i < n; i++) {
         const u32 cls = s[i] & 15u;
+#pragma unroll
+        for (int t = 0; t < 8; t++)
+            c16[t] += (cls >> 1) == (u32)t ? 1u << (16 * (cls & 1u)) : 0u;
+    }
+    u32 w[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        u32 v = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int cls = 4 * j + i;
+            const u32 cnt = (c16[cls >> 1] >> (16 * (cls & 1))) & 0xFFFFu;
+            v |= (cnt > 255u ? 255u : cnt) << (8 * i);
+        }
+        w[j] = v;
+    }
+    return Profile{w[0], w[1], w[2], w[3]};
+}
+
+/* The 2-gram profile of s[0..n) (classes as for count2_over). */
+BL_HD Profile class2_profile(const u8 *s, int n)
+{
+    u32 c16[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i + 1 < n; i++) {
+        const u32 cls = 4u * class2(s[i]) + class2(s[i + 1]);
 #pragma unroll
         for (int t = 0; t < 8; t++)
             c16[t] += (cls >> 1) == (u32)t ? 1u << (16 * (cls & 1u)) : 0u;

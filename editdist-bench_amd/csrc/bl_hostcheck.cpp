@@ -395,6 +395,17 @@ int blh_count_over(const u8 *a, int m, const u8 *b, int n, int k)
     return count_over(class_profile(a, m), class_profile(b, n), m, n, k) ? 1 : 0;
 }
 
+void blh_class2_profile(const u8 *s, int n, uint32_t *out)
+{
+    const Profile p = class2_profile(s, n);
+    out[0] = p.x; out[1] = p.y; out[2] = p.z; out[3] = p.w;
+}
+
+int blh_count2_over(const u8 *a, int m, const u8 *b, int n, int k)
+{
+    return count2_over(class2_profile(a, m), class2_profile(b, n), m, n, k) ? 1 : 0;
+}
+
 uint32_t blh_swar_over(const u8 *q, int m, const u8 *t, const int *n, int k)
 {
     std::vector<u8> lo8(256, 0);

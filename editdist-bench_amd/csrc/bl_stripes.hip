@@ -29,7 +29,8 @@ __global__ __launch_bounds__(256) void stripes_build_kernel(const u8 *__restrict
                                                             const int64_t *__restrict__ gbase,
                                                             u32 *__restrict__ words,
                                                             int32_t *__restrict__ len,
-                                                            Profile *__restrict__ counts)
+                                                            Profile *__restrict__ counts,
+                                                            Profile *__restrict__ counts2)
 {
     const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = (int)(threadIdx.x & 63);
@@ -45,6 +46,7 @@ __global__ __launch_bounds__(256) void stripes_build_kernel(const u8 *__restrict
         s = buf + o;
         len[c] = ln;
         if (counts) counts[c] = class_profile(s, ln);
+        if (counts2) counts2[c] = class2_profile(s, ln);
     }
     for (int64_t j = 0; j < width; j++) words[gb + 64 * j + lane] = stripe_word(s, ln, j);
 }
@@ -107,7 +109,8 @@ int64_t bl_stripes_layout(const int64_t *h_off, int64_t n, int64_t *h_gbase)
 }
 
 int bl_stripes_build(const uint8_t *d_buf, const int64_t *d_off, int64_t n, const int64_t *d_gbase,
-                     uint32_t *d_words, int32_t *d_len, uint32_t *d_counts, void *stream)
+                     uint32_t *d_words, int32_t *d_len, uint32_t *d_counts, uint32_t *d_counts2,
+                     void *stream)
 {
     if (n < 0) return fail(BL_EINVAL, "bl_stripes_build: negative count");
     if (n == 0) return BL_OK;
@@ -116,10 +119,10 @@ int bl_stripes_build(const uint8_t *d_buf, const int64_t *d_off, int64_t n, cons
     const int64_t groups = (n + 63) / 64;
     const int64_t blocks = (groups + 3) / 4;
     if (blocks > 0x7fffffff) return fail(BL_EINVAL, "bl_stripes_build: too many strings");
-    if (((uintptr_t)d_counts & 15) != 0)
-        return fail(BL_EINVAL, "bl_stripes_build: class profiles not 16-byte aligned");
+    if (((uintptr_t)d_counts & 15) != 0 || ((uintptr_t)d_counts2 & 15) != 0)
+        return fail(BL_EINVAL, "bl_stripes_build: profiles not 16-byte aligned");
     stripes_build_kernel<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(
-        d_buf, d_off, n, d_gbase, d_words, d_len, (Profile *)d_counts);
+        d_buf, d_off, n, d_gbase, d_words, d_len, (Profile *)d_counts, (Profile *)d_counts2);
     BL_LAUNCHED("stripes_build_kernel");
     return BL_OK;
 }

@@ -109,6 +109,13 @@ typedef struct {
     const int32_t *len;      /* n lengths */
     int64_t n;               /* number of choices */
     const uint32_t *counts;  /* 4 * n dwords of class profiles, or NULL */
+    const uint32_t *counts2; /* 4 * n dwords of 2-gram profiles, or NULL: byte i of
+                                dword 4c + j counts the 2-grams (x, y) of choice c
+                                with 4 ((x >> 1) & 3) + ((y >> 1) & 3) == 4j + i,
+                                saturated at 255; a second lower bound, d >=
+                                (L1 + |t_q - t_c|) / 4 over the two (t: the
+                                2-gram totals; L1 / 4 past 256 bytes), applied to
+                                what the first leaves open */
 } bl_stripes;
 
 /* Host: group bases of the stripes of n strings with host offsets
@@ -116,12 +123,12 @@ typedef struct {
  * words array needs (h_gbase[ceil(n/64)]), or a negative status. */
 int64_t bl_stripes_layout(const int64_t *h_off, int64_t n, int64_t *h_gbase);
 
-/* Device: fill words, len and (unless d_counts is NULL) the class profiles
+/* Device: fill words, len and (unless NULL) the class and 2-gram profiles
  * from packed strings (d_buf, d_off) using the group bases d_gbase that
  * bl_stripes_layout computed for the same offsets. */
 int bl_stripes_build(const uint8_t *d_buf, const int64_t *d_off, int64_t n,
                      const int64_t *d_gbase, uint32_t *d_words, int32_t *d_len,
-                     uint32_t *d_counts, void *stream);
+                     uint32_t *d_counts, uint32_t *d_counts2, void *stream);
 
 /* dst[r][i] = src[r][inv[i]] for r < rows, i < cols (dtype BL_INT32 /
  * BL_UINT16 / BL_UINT8; src and dst distinct). No counterpart in the

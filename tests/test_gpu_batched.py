@@ -170,7 +170,8 @@ def test_class_count_bound_in_searches():
     distance), choices differing from the query only inside one class
     (profile equal, d > 0), queries of 255 and 256 bytes of one byte (the
     count saturates at 256), empty choices; the filter, gap and compacted
-    modes, dense and match-list output, with the profiles and without them."""
+    modes, dense and match-list output, with the class and 2-gram profiles,
+    with the class profiles only and without profiles."""
     rng = np.random.default_rng(53)
     base = bytes(rng.integers(97, 123, 64).astype(np.uint8))
     Qs = [base, b"\x41" * 255, b"\x41" * 256, base[:20], bytes(rng.integers(0, 256, 200).astype(np.uint8)),
@@ -214,11 +215,12 @@ def test_class_count_bound_in_searches():
     Q, C = hip.Packed(*pq), hip.Packed(*pc)
     exact = O.cross(pq, pc, threads=8)
     S = C.stripes()
-    assert S.desc.counts
-    saved = S.desc.counts
+    assert S.desc.counts and S.desc.counts2
+    saved, saved2 = S.desc.counts, S.desc.counts2
     try:
-        for with_profiles in (True, False):
+        for with_profiles, with_2grams in ((True, True), (True, False), (False, False)):
             S.desc.counts = saved if with_profiles else None
+            S.desc.counts2 = saved2 if with_2grams else None
             for k in (0, 1, 2, 3, 5, 8, 300):
                 want = np.minimum(exact, k + 1)
                 for mode, extra in (("swar", {}), ("gap", {}), ("compacted", {}), ("count", {}),
@@ -235,7 +237,7 @@ def test_class_count_bound_in_searches():
                     assert got == sorted(zip(wq.tolist(), wc.tolist(), exact[wq, wc].tolist())), \
                         (k, mode, with_profiles)
     finally:
-        S.desc.counts = saved
+        S.desc.counts, S.desc.counts2 = saved, saved2
 
 
 def test_filter_mode_with_every_choice_open():

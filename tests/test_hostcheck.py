@@ -499,3 +499,88 @@ def test_class_profiles_and_count_bound(hc):
     assert not over(a, b, 0)
     # same counts in every class, far apart: the bound cannot prune (it is a bound, not a filter)
     assert not over(b"ab" * 20, b"ba" * 20, 0)
+
+
+def test_two_gram_profiles_and_bound(hc):
+    """The 2-gram bound: profiles are the saturated counts of the 16 classes
+    4 c2(x) + c2(y), c2(b) = (b >> 1) & 3, over the string's 2-grams; every
+    pair it prunes has an oracle distance above k (random, edited, shuffled
+    and DNA pairs); it is reached on pairs that make it tight (one
+    substitution in the middle of a string moves two 2-grams out and two in;
+    a run of one letter against a run of another), so a dropped gap term, a
+    factor of 2 for 4 or an off-by-one fails; past 256 bytes the gap term is
+    left out."""
+    hc.blh_class2_profile.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)]
+    hc.blh_count2_over.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                   ctypes.c_int]
+    rng = np.random.default_rng(78)
+    for n in (0, 1, 2, 17, 256, 600):
+        s = bytes(rng.integers(0, 256, n).astype(np.uint8)) if n < 600 else b"A" * 600
+        out = (ctypes.c_uint32 * 4)()
+        hc.blh_class2_profile(s, n, out)
+        a = np.frombuffer(s, np.uint8).astype(np.int64)
+        cls = 4 * ((a[:-1] >> 1) & 3) + ((a[1:] >> 1) & 3) if n > 1 else np.zeros(0, np.int64)
+        want = np.minimum(np.bincount(cls, minlength=16), 255)
+        assert np.frombuffer(bytes(out), np.uint8).tolist() == want.tolist(), n
+
+    def over(a, b, k):
+        return bool(hc.blh_count2_over(a, len(a), b, len(b), k))
+
+    A, B = [], []
+    for _ in range(4000):
+        alpha = rng.choice([b"ACGT", b"ab", b"abcdefghijklmnopqrstuvwxyz", bytes(range(256))])
+        m = int(rng.integers(0, 300))
+        a = bytes(rng.choice(np.frombuffer(alpha, np.uint8), m)) if m else b""
+        r = rng.random()
+        if r < 0.4:
+            b = bytearray(a)
+            for _ in range(int(rng.integers(0, 14))):
+                op = rng.integers(0, 3)
+                pos = int(rng.integers(0, len(b) + 1))
+                ch = int(rng.choice(np.frombuffer(alpha, np.uint8)))
+                if op == 0 and pos < len(b):
+                    b[pos] = ch
+                elif op == 1:
+                    b.insert(pos, ch)
+                elif pos < len(b):
+                    del b[pos]
+            b = bytes(b)
+        elif r < 0.6:
+            x = np.frombuffer(a, np.uint8).copy()
+            rng.shuffle(x)
+            b = x.tobytes()
+        else:
+            b = bytes(rng.choice(np.frombuffer(alpha, np.uint8), int(rng.integers(0, 300))))
+        A.append(a)
+        B.append(b)
+    d = O.pairs(O.pack(A), O.pack(B), threads=8)
+    pruned = 0
+    for a, b, dist in zip(A, B, d):
+        for k in (0, 1, 2, 3, 5, 8, 13, 30):
+            if over(a, b, k):
+                assert dist > k, (a, b, k, dist)
+                pruned += 1
+    assert pruned > 3000
+    # tight: d(a, b) = 1 by one substitution in the middle; the 2-gram L1 is 4
+    a = b"ACGTACGTAC"
+    b = b"ACGTTCGTAC"
+    assert int(O.pairs(O.pack([a]), O.pack([b]))[0]) == 1
+    prof_a = (ctypes.c_uint32 * 4)(); prof_b = (ctypes.c_uint32 * 4)()
+    hc.blh_class2_profile(a, len(a), prof_a); hc.blh_class2_profile(b, len(b), prof_b)
+    l1 = int(np.abs(np.frombuffer(bytes(prof_a), np.uint8).astype(int)
+                    - np.frombuffer(bytes(prof_b), np.uint8).astype(int)).sum())
+    assert l1 == 4 and over(a, b, 0) and not over(a, b, 1)
+    # runs of two letters: d = 10; 2-gram L1 = 18 (9 and 9), gap 0 -> bound 4.5
+    a, b = b"A" * 10, b"C" * 10
+    assert over(a, b, 4) and not over(a, b, 5)
+    # lengths differ: a = x, b = x + 6 more; 2-gram L1 >= 6, gap 6 -> bound 3
+    a = b"ACGTACGTACGT"
+    b = a + b"AAAAAA"
+    assert int(O.pairs(O.pack([a]), O.pack([b]))[0]) == 6
+    assert not over(a, b, 6)
+    # the gap term decides: 9 vs 29 AA 2-grams, L1 = 20, gap 20 -> bound 10 (d = 20)
+    a, b = b"A" * 10, b"A" * 30
+    assert over(a, b, 9) and not over(a, b, 10)
+    # past 256 bytes the gap term is left out
+    a, b = b"A" * 300, b"A" * 280
+    assert not over(a, b, 0)
