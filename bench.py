@@ -649,33 +649,51 @@ def cpu_leg(res, args, wl, k, dt, hx, hy, out, plain):
               file=sys.stderr)
 
 
-def class_profiles(packed, chunk=1 << 18):
-    """(n, 16) byte-class counts (class = byte & 15, saturated at 255) of a
-    packed set, as the search kernels' profiles hold them."""
+def class_profiles(packed, chunk=1 << 18, grams=1):
+    """(n, 16) saturated counts of a packed set as the search kernels'
+    profiles hold them: byte classes b & 15 (grams=1) or 2-gram classes
+    4 ((x >> 1) & 3) + ((y >> 1) & 3) over each string's adjacent bytes
+    (grams=2)."""
     buf, off = packed
     n = len(off) - 1
     out = np.empty((n, 16), np.int16)
     for lo in range(0, n, chunk):
         hi = min(n, lo + chunk)
-        seg = np.asarray(buf[off[lo]:off[hi]]) & 15
-        ids = np.repeat(np.arange(hi - lo, dtype=np.int32), np.diff(off[lo:hi + 1]))
-        cnt = np.bincount(ids * 16 + seg.astype(np.int32), minlength=(hi - lo) * 16)
+        seg = np.asarray(buf[off[lo]:off[hi]]).astype(np.int32)
+        lens = np.diff(off[lo:hi + 1])
+        ids = np.repeat(np.arange(hi - lo, dtype=np.int32), lens)
+        if grams == 1:
+            cls = seg & 15
+        else:
+            c2 = (seg >> 1) & 3
+            cls = 4 * c2[:-1] + c2[1:]
+            ids = ids[:-1]
+            same = ids == np.repeat(np.arange(hi - lo, dtype=np.int32), lens)[1:]
+            cls, ids = cls[same], ids[same]      # no 2-gram across two strings
+        cnt = np.bincount(ids * 16 + cls, minlength=(hi - lo) * 16)
         out[lo:hi] = np.minimum(cnt.reshape(hi - lo, 16), 255)
     return out
 
 
 def count_open_mask(sq, hy, k):
-    """(rows, nc) mask of the pairs the length and class-count bounds leave for
-    a scan (the count kernel's survivors), for the sampled queries sq."""
+    """(rows, nc) mask of the pairs the length, class-count and 2-gram bounds
+    leave for a scan (the count kernel's survivors), for the sampled queries
+    sq."""
     pq, pc = class_profiles(sq), class_profiles(hy)
+    pq2, pc2 = class_profiles(sq, grams=2), class_profiles(hy, grams=2)
     mq = np.diff(sq[1]).astype(np.int64)
     nc_len = np.diff(hy[1]).astype(np.int64)
+    t_c = np.maximum(nc_len - 1, 0)
     mask = np.empty((len(mq), len(nc_len)), bool)
     for i in range(len(mq)):
         l1 = np.abs(pc - pq[i]).sum(1, dtype=np.int64)
         gap = np.abs(nc_len - mq[i])
         small = (mq[i] <= 255) & (nc_len <= 255)
-        mask[i] = (gap <= k) & (l1 + np.where(small, gap, 0) <= 2 * k)
+        m1 = (gap <= k) & (l1 + np.where(small, gap, 0) <= 2 * k)
+        l2 = np.abs(pc2 - pq2[i]).sum(1, dtype=np.int64)
+        gap2 = np.abs(t_c - max(mq[i] - 1, 0))
+        small2 = (mq[i] <= 256) & (nc_len <= 256)
+        mask[i] = m1 & (l2 + np.where(small2, gap2, 0) <= 4 * k)
     return mask
 
 
