@@ -830,7 +830,6 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
                                               const Profile *__restrict__ ccnt, int64_t c0,
                                               int64_t c1, int64_t nc, int64_t q0, int gq, u64 okmask,
                                               int lm, int lk, bool uniform_k, const Profile &lp,
-
                                               OutT *out, bool rows_aligned, int &ns, Drain &drain)
 {
     const int lane = lane_id();
