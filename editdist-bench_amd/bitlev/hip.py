@@ -338,10 +338,27 @@ def run_pairs(A: Packed, B: Packed, out: DeviceArray, k: int = -1, trim: bool = 
     raise ValueError("form must be 'batched' or 'plain'")
 
 
+def default_cdist_params(nq: int, nc: int, k: int, cus: int = 256):
+    """Launch shape of an untuned batched search: with a threshold, the count
+    kernel (query groups of up to 64; blocks sized so the 16 waves per CU get
+    about two tiles each -- the shape the autotuner picks for the
+    1,024 x 1e6 and 256 x 4.2e6 searches); exact searches keep the library's
+    defaults."""
+    if k < 0 or nq == 0 or nc == 0:
+        return None
+    g = min(64, nq)
+    waves = 16 * cus
+    tile = nc * ((nq + g - 1) // g) // (2 * waves)
+    tile = min(8192, max(1024, tile // 256 * 256))
+    return {"mode": "count", "waves_per_cu": 16, "tile": tile, "qgroup": g}
+
+
 def run_cdist(Q: Packed, C: Packed, out: DeviceArray, k: int = -1, stream=None,
               form: str | None = None, plan=None, workspace=None, params=None):
     form = form or FORM
     if form == "batched":
+        if params is None and C.stripes(stream).desc.counts:
+            params = default_cdist_params(Q.n, C.n, k)
         return cdist_batched_into(Q, C, out, k, stream, workspace, params, plan)
     if form == "plain":
         return cdist_into(Q, C, out, k, stream, plan)
