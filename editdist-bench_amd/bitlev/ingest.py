@@ -340,7 +340,8 @@ def pairwise(a, b, k: int = -1, dtype=np.int32, trim: bool = True, chunk: int = 
     `dtype`, through the staged pipeline. a, b: equal-length sequences of
     str/bytes (lists and tuples are read in place, without the GIL, and must
     not be changed while the call runs).
-    params: a bl_pairs launch shape (bitlev.autotune), or None for defaults.
+    params: a bl_pairs launch shape (bitlev.autotune), or None for the
+    untuned default (hip.default_pairs_params).
     The stager (pinned buffers sized by the largest chunk seen) is kept for
     the next call; release() frees it."""
     if not isinstance(a, (list, tuple)):
@@ -351,6 +352,8 @@ def pairwise(a, b, k: int = -1, dtype=np.int32, trim: bool = True, chunk: int = 
     if len(b) != n:
         raise ValueError(f"length mismatch: {n} vs {len(b)}")
     dt = check_dtype(dtype)
+    if params is None:
+        params = hip.default_pairs_params(int(k))
     T = threads or effective_threads()
     with _idle_lock:
         st = next((s for s in _idle if s.threads == T), None)
