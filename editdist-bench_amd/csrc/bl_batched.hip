@@ -1113,7 +1113,7 @@ __global__ __launch_bounds__(256) void cdist_count_kernel(
             const int kq = kvec ? clamp_k_dev(kvec[q]) : k_all;
             const u64 *qp = gpeq + (size_t)q * 256 * NWMAX;
             const StripeText text{words + gbase[c >> 6] + (c & 63)};
-            sink.put(idx, peq_lane<NWMAX, 1>((const u32 *)nullptr, qp, mq, text, clen[c], kq), kq);
+            sink.put(idx, peq_lane<NWMAX, 1, false>((const u32 *)nullptr, qp, mq, text, clen[c], kq), kq);
         }
         ne = uni(base);
         wave_lds_sync();
@@ -1217,7 +1217,7 @@ __global__ __launch_bounds__(256) void cdist_count_kernel(
                 const GlobalLo32<NWMAX> lo{qp};
                 for (int64_t c = c0 + lane; c < c1; c += 64) {
                     const StripeText text{words + gbase[c >> 6] + (c & 63)};
-                    sink.put(orow + c, peq_lane<NWMAX, 0>(lo, qp, mq, text, clen[c], kq), kq);
+                    sink.put(orow + c, peq_lane<NWMAX, 0, false>(lo, qp, mq, text, clen[c], kq), kq);
                 }
             }
         }

@@ -1021,8 +1021,10 @@ BL_HD int band_pair(const u8 *a, int la, const u8 *b, int lb, int k, int d0)
  * stage). Semantics of text_distance_peq, csrc/gpu_myers.cuh:124-145, and
  * run_text / run_text_k, csrc/myers.c:58-77, :107-130.
  * FROM: the first stage to run (0: 32 rows, 1: 64 rows); a caller that already
- * ran the 32-row stage passes 1 (lo32 is then not read). */
-template <int NW, int FROM, typename Lo32Ptr, typename PeqPtr, typename Text>
+ * ran the 32-row stage passes 1 (lo32 is then not read). EAGER: the whole
+ * stage fetches its four columns' Eq words before stepping (false: column by
+ * column, fewer live registers). */
+template <int NW, int FROM, bool EAGER = true, typename Lo32Ptr, typename PeqPtr, typename Text>
 BL_HD int peq_lane(Lo32Ptr lo32, PeqPtr peq, int m, const Text &t, int n, int k)
 {
     if (k >= 0 && (n > m ? n - m : m - n) > k) return k + 1;
@@ -1042,7 +1044,7 @@ BL_HD int peq_lane(Lo32Ptr lo32, PeqPtr peq, int m, const Text &t, int n, int k)
         if (r >= 0) return r;
     }
     const int nw = (m + 63) >> 6;
-    return scan_text<u64, NW, true>(m, nw, 64 * nw, t, n, k, eq64);
+    return scan_text<u64, NW, EAGER>(m, nw, 64 * nw, t, n, k, eq64);
 }
 
 /* Empty cross query: d = n, or k + 1 when k >= 0 and n > k
