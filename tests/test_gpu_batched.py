@@ -403,3 +403,48 @@ def test_count_kernel_fast_paths_every_output():
                 got = sorted(zip(qi.get().tolist(), ci.get().tolist(), di.get().tolist()))
                 wq, wc = np.nonzero(exact <= k)
                 assert got == sorted(zip(wq.tolist(), wc.tolist(), exact[wq, wc].tolist())), (nc, k, p)
+
+
+def test_count_kernel_long_queries_rescans():
+    """The count kernel's full-pattern rescans for queries of 129-256 and
+    257-512 bytes (four- and eight-word Peq tables): near copies of each query
+    (a few substitutions, insertions and deletions) reach the escalation
+    stage, mixed with unrelated choices; per-query thresholds include an exact
+    query (-1, run lane-strided) and thresholds above 255; dense and
+    match-list output against the oracle."""
+    rng = np.random.default_rng(71)
+    Qs = [bytes(rng.integers(97, 101, m).astype(np.uint8)) for m in (130, 256, 257, 400, 512)]
+    Cs = []
+    for i in range(3000):
+        q = Qs[i % len(Qs)]
+        b = bytearray(q)
+        if i % 4 == 3:
+            b = bytearray(rng.integers(97, 101, int(rng.integers(100, 520))).astype(np.uint8))
+        else:
+            for _ in range(int(rng.integers(0, 12))):
+                op = int(rng.integers(0, 3))
+                p_ = int(rng.integers(0, len(b)))
+                ch = 97 + int(rng.integers(0, 4))
+                if op == 0:
+                    b[p_] = ch
+                elif op == 1:
+                    b.insert(p_, ch)
+                else:
+                    del b[p_]
+        Cs.append(bytes(b))
+    pq, pc = O.pack(Qs), O.pack(Cs)
+    Q, C = hip.Packed(*pq), hip.Packed(*pc)
+    exact = O.cross(pq, pc, threads=8)
+    for kq in ([6, 6, 6, 6, 6], [3, 9, -1, 300, 6]):
+        kv = np.array(kq, np.int32)
+        want = np.where(kv[:, None] < 0, exact, np.minimum(exact, kv[:, None] + 1))
+        for p in ({"mode": "count", "tile": 256, "qgroup": 2}, {"mode": "count", "qgroup": 64},
+                  {"mode": "count", "tile": 1024, "qgroup": 4, "waves_per_cu": 1}):
+            out = hip.DeviceArray.empty((Q.n, C.n), np.int32)
+            hip.cdist_batched_into(Q, C, out, -1, None, None, p, kvec=hip.DeviceBuffer.from_host(kv))
+            np.testing.assert_array_equal(out.get(), want, err_msg=f"{kq} {p}")
+            if min(kq) >= 0:
+                qi, ci, di = hip.cdist_matches_into(Q, C, -1, params=p, kvec=hip.DeviceBuffer.from_host(kv))
+                got = sorted(zip(qi.get().tolist(), ci.get().tolist(), di.get().tolist()))
+                wq, wc = np.nonzero(exact <= kv[:, None])
+                assert got == sorted(zip(wq.tolist(), wc.tolist(), exact[wq, wc].tolist())), (kq, p)
