@@ -47,7 +47,7 @@ def test_every_pairs_launch_shape_gives_the_same_results():
             np.testing.assert_array_equal(out.get(), want[k], err_msg=str(p))
 
 
-@pytest.mark.parametrize("band", [1, 3, 5])
+@pytest.mark.parametrize("band", [1, 2, 3, 5])
 def test_band_tiles_on_uneven_pairs(band):
     """Band tiles on the uneven workload's kind of pairs (lengths 1-256, bytes
     0-255, edit rates 5-30%) and on harder ones (rates up to 45%, so the window
@@ -448,3 +448,23 @@ def test_count_kernel_long_queries_rescans():
                 got = sorted(zip(qi.get().tolist(), ci.get().tolist(), di.get().tolist()))
                 wq, wc = np.nonzero(exact <= kv[:, None])
                 assert got == sorted(zip(wq.tolist(), wc.tolist(), exact[wq, wc].tolist())), (kq, p)
+
+
+def test_untuned_thresholded_pairwise_default_shape():
+    """Public pairwise calls without a launch shape: thresholds 0-30 run band
+    tiles from two units (the default shape), others the library default; on
+    near-duplicate pairs of lengths 1-256 (where the band decides most pairs)
+    plus empty and long strings, every dtype, against the oracle."""
+    pa, pb = W.pairs(3, 6000, seed=12)
+    A = [bytes(pa[0][pa[1][i]:pa[1][i + 1]]) for i in range(len(pa[1]) - 1)]
+    B = [bytes(pb[0][pb[1][i]:pb[1][i + 1]]) for i in range(len(pb[1]) - 1)]
+    A += [b"", b"x" * 300, b"ab" * 150]
+    B += [b"abc", b"", b"ab" * 149 + b"a"]
+    assert hip.default_pairs_params(8) is not None and hip.default_pairs_params(31) is None
+    pk = O.pack(A), O.pack(B)
+    for k in (0, 1, 8, 30, 31, None):
+        kk = -1 if k is None else k
+        want = O.pairs(*pk, k=kk, threads=8)
+        for dt in (np.int32, np.uint8, np.uint16):
+            got = hip.pairwise(A, B, max_dist=k, dtype=dt)
+            np.testing.assert_array_equal(got, O.saturate(want, dt), err_msg=f"k={k} {dt}")
