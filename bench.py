@@ -1,26 +1,44 @@
 #!/usr/bin/env python3
 """Throughput of the bitlev MI355X backend on one GPU; prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--workload NAME] [--form plain|batched|both]
+    python bench.py [--steps K] [--warmup W] [--workload NAME] [--configs all|none|a,b]
 
 A step is one pass of the hot path over one batch with inputs already
-resident in HBM: for the default workload, one cdist of 1,024 queries x
+resident in HBM. The headline workload: one cdist of 1,024 queries x
 1,000,000 choices (length 64, 'a'-'z', 30% near duplicates), max_dist=2,
-uint8 results kept on the device. Other workloads: see bitlev.workloads.
+uint8 results kept on the device, timed in both forms:
 
-Besides the timed steps the run
-  * checks a sample of GPU rows against the reference C core (or the oracle);
-  * times that CPU core on the same sample, 1 thread and all cores;
-  * counts the useful work of the sample with the oracle's column counter
-    (columns the reference's threshold exit rule evaluates x pattern rows) and
-    reports the kernel against the INT32 VALU roofline -- or, for the count
-    kernel, which settles pairs without scanning cells, against HBM (the
-    result matrix it must write).
+  * plain   -- the reference's launches one for one: the host splits queries
+               into word classes and runs one bl_cdist_class launch per class
+               over the packed choices (bitlev/gpu.py:116-130);
+  * batched -- one bl_cdist call (per-query Peq pre-pass + one persistent
+               launch) with the launch shape the autotuner picks.
+
+`value` is the faster form (what a caller gets). Besides the timed steps the
+run
+  * copies each form's whole result back and compares it with the reference
+    C core (built from the reference's own csrc/myers.c into oracle/_ref, or
+    the oracle when that is absent), query rows in chunks, every pair;
+  * times that CPU core on the same inputs: all cores over the parity chunks,
+    one thread on a prefix;
+  * reports the dominant kernel against its roofline (HBM bytes for the count
+    kernel, which settles pairs by exact lower bounds; INT32 VALU cells for
+    the scanning kernels) and the measured HBM traffic from the committed
+    counter profiles;
+  * times a cold call (the choice set's stripes and profiles built inside the
+    timed region) and the match-list output;
+  * adds compact records of the other benchmark workloads (65,536 ACGT pairs
+    of 32 bytes; 16.8M pairs of 1-256 bytes; 256 x 4.19M DNA search at
+    max_dist=8) and of two exact searches (the headline shape without
+    max_dist; one serving batch against a resident set), each with both
+    forms, per-form parity, CPU rates and its roofline;
+  * runs the serving mix (p50/p99 latency at a fixed offered load).
 """
 from __future__ import annotations
 
 import argparse
 import ctypes
+import gc
 import json
 import os
 import sys
@@ -34,6 +52,9 @@ for _p in (ROOT, os.path.join(ROOT, "editdist-bench_amd")):
 import numpy as np  # noqa: E402
 
 METRIC = "M pairs/s (cdist L64 k=2 uint8), batched vs plain; p50/p99 request latency"
+HEADLINE = "cdist-L64-k2"
+OTHER_CONFIGS = ("pairs-L32-acgt", "pairs-uneven-1-256", "cdist-L200-256-k8",
+                 "cdist-L64-exact", "cdist-serving-exact")
 # INT32 VALU peak of MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz.
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 # Floor of the Myers recurrence: 10 INT32 ops per 32 DP cells (3-input bit ops).
@@ -48,11 +69,15 @@ def parse():
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="cdist-L64-k2")
+    ap.add_argument("--workload", default=HEADLINE)
     ap.add_argument("--form", default="both", choices=["plain", "batched", "both"])
+    ap.add_argument("--configs", default=None,
+                    help="compact records of other workloads: 'all' (default with the headline "
+                         "workload), 'none', or a comma-separated list of workload names")
     ap.add_argument("--cpu-rows", type=int, default=None,
-                    help="sample size for parity / CPU baseline (queries for cdist, pairs/1024 for pairs)")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity sample")
+                    help="parity / all-core CPU sample: queries for cdist, pairs for pairs "
+                         "(default: every query, every pair; a prefix for the exact searches)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity")
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--no-tune", action="store_true", help="batched form with default launch shape")
     ap.add_argument("--retune", action="store_true", help="ignore the autotune cache")
@@ -67,9 +92,9 @@ def parse():
                     help="offered load as a fraction of the measured batched-offline throughput")
     ap.add_argument("--serving-slots", type=int, default=4)
     ap.add_argument("--ingest-pairs", type=int, default=1 << 21,
-                    help="pairs workloads: end-to-end leg from Python lists of this many pairs "
-                         "(packing, PCIe copies, kernels, results back), staged pipeline vs one "
-                         "whole-batch pack; 0 skips it")
+                    help="pairs workloads run on their own: end-to-end leg from Python lists of "
+                         "this many pairs (packing, PCIe copies, kernels, results back), staged "
+                         "pipeline vs one whole-batch pack; 0 skips it")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 3 times (a kernel of known traffic "
                          "for calibrating the HBM byte counters) and print the byte count")
@@ -101,7 +126,7 @@ class Events:
 
     def new(self):
         e = ctypes.c_void_p()
-        self.capi.call("bl_event_create", ctypes.byref(e))
+        self.capi.call("bl_event_create", e)
         return e.value
 
     def record(self, e):
@@ -110,18 +135,24 @@ class Events:
     def ms(self, a, b):
         self.capi.call("bl_event_sync", b)
         v = ctypes.c_float()
-        self.capi.call("bl_event_elapsed_ms", a, b, ctypes.byref(v))
+        self.capi.call("bl_event_elapsed_ms", a, b, v)
         return float(v.value)
+
+    def destroy(self, *es):
+        for e in es:
+            self.capi.call("bl_event_destroy", e)
 
 
 def device_info(capi):
     name = ctypes.create_string_buffer(256)
     cu, ma, mi = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-    capi.call("bl_device_info", name, 256, ctypes.byref(cu), ctypes.byref(ma), ctypes.byref(mi))
+    capi.call("bl_device_info", name, 256, cu, ma, mi)
     return {"name": name.value.decode(), "cus": cu.value}
 
 
 def timed_steps(step, ev, steps, warmup):
+    """Warm-up, then `steps` steps bracketed by device syncs; per-step HIP
+    event times on the launch stream."""
     for _ in range(warmup):
         step()
     ev.capi.call("bl_device_sync")
@@ -135,9 +166,570 @@ def timed_steps(step, ev, steps, warmup):
     ev.capi.call("bl_device_sync")
     wall = time.perf_counter() - t0
     per = [ev.ms(a, b) for a, b in zip(starts, stops)]
-    for e in starts + stops:
-        ev.capi.call("bl_event_destroy", e)
+    ev.destroy(*starts, *stops)
     return wall, per
+
+
+def rounded(d):
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
+
+
+def config_of(name, wl):
+    c = {"workload": name}
+    if wl["kind"] == "cdist":
+        c.update(queries=wl["nq"], choices=wl["nc"])
+    else:
+        c.update(pairs=wl["n"])
+    c.update(max_dist=wl["k"], out_dtype=wl["dtype"], return_device=True, gpus=1)
+    return c
+
+
+# ============================================================ one workload --
+class Workload:
+    """One benchmark workload resident on the device, its timed forms and
+    their results copied back for parity."""
+
+    def __init__(self, name, args, seed, stream):
+        from bitlev import hip
+        from bitlev import workloads as W
+        self.name, self.args, self.seed = name, args, seed
+        self.wl = wl = W.WORKLOADS[name]
+        self.kind = wl["kind"]
+        self.k = -1 if wl["k"] is None else int(wl["k"])
+        self.dt = np.dtype(wl["dtype"])
+        self.stream = stream
+        sp = stream.ptr
+        t0 = time.perf_counter()
+        if self.kind == "cdist":
+            self.hx, self.hy = W.cdist(wl["gen"], wl["nq"], wl["nc"], seed)
+            self.X, self.Y = hip.Packed(*self.hx, stream=sp), hip.Packed(*self.hy, stream=sp)
+            self.Y.stripes(sp)          # the resident search layout of the choice set
+            self.out = hip.DeviceArray.empty((self.X.n, self.Y.n), self.dt)
+            self.plan = hip.cdist_plan(self.X, self.Y)
+            self.pairs_total = self.X.n * self.Y.n
+        else:
+            self.hx, self.hy = W.pairs(wl["gen"], wl["n"], seed)
+            self.X, self.Y = hip.Packed(*self.hx, stream=sp), hip.Packed(*self.hy, stream=sp)
+            self.out = hip.DeviceArray.empty((self.X.n,), self.dt)
+            self.plan = hip.pairs_plan(self.X, self.Y)
+            self.pairs_total = self.X.n
+        stream.sync()
+        self.setup_s = time.perf_counter() - t0
+        self.results = {}
+        self.params = None
+
+    def step(self, form, params=None, ws=None):
+        from bitlev import hip
+        sp = self.stream.ptr
+        X, Y, out, k, plan = self.X, self.Y, self.out, self.k, self.plan
+        if self.kind == "cdist":
+            if form == "plain-stripes":
+                return lambda: hip.cdist_into(X, Y, out, k, sp, plan, stripes=True)
+            return lambda: hip.run_cdist(X, Y, out, k, sp, form, plan, ws, params)
+        return lambda: hip.run_pairs(X, Y, out, k, True, sp, form, plan, ws, params)
+
+    def tune(self):
+        from bitlev import autotune
+        a = self.args
+        if a.params:
+            return json.loads(a.params)
+        if a.no_tune:
+            return None
+        sp = self.stream.ptr
+        if self.kind == "cdist":
+            return autotune.tune_cdist(self.X, self.Y, self.out, self.k, sp, use_cache=not a.retune,
+                                       log=None)
+        return autotune.tune_pairs(self.X, self.Y, self.out, self.k, True, sp,
+                                   use_cache=not a.retune, log=None)
+
+    def measure(self, form, ev, steps, warmup):
+        from bitlev import hip
+        extra = {}
+        if form == "batched":
+            t1 = time.perf_counter()
+            self.params = self.tune()
+            extra["tune_s"] = round(time.perf_counter() - t1, 2)
+            extra["params"] = self.params
+            step = self.step(form, self.params, hip.Workspace())
+            step()
+            self.stream.sync()
+            w1, _ = timed_steps(step, ev, 1, 0)
+            if not self.args.no_graph and w1 < 2e-3:
+                # graph replay or eager launches, whichever is faster: a graph
+                # saves per-launch host work on multi-kernel pipelines but adds
+                # ~10 us, which dominates a batch that runs in ~10 us
+                g = hip.Graph(step, self.stream)
+                we, _ = timed_steps(step, ev, 20, 2)
+                wg, _ = timed_steps(g.launch, ev, 20, 2)
+                extra["probe_ms_per_step"] = {"eager": round(we * 50, 4), "graph": round(wg * 50, 4)}
+                extra["graph"] = bool(wg <= we)
+                if extra["graph"]:
+                    step = g.launch
+        else:
+            step = self.step(form)
+        # a sentinel the step must overwrite everywhere
+        from bitlev import _capi
+        _capi.call("bl_memset", self.out.ptr, 0xEE, self.out.nbytes, self.stream.ptr)
+        wall, per = timed_steps(step, ev, steps, warmup)
+        ms_step = wall * 1e3 / steps
+        self.results[form] = self.out.get(self.stream.ptr)
+        return {
+            "value": self.pairs_total / (ms_step / 1e3) / 1e6,
+            "ms_per_step": ms_step,
+            "kernel_ms_mean": float(np.mean(per)),
+            "kernel_ms_min": float(np.min(per)),
+            **extra,
+        }
+
+    def free(self):
+        for a in ("X", "Y", "out", "plan"):
+            setattr(self, a, None)
+        self.results = {}
+        gc.collect()
+        from bitlev.device import POOL
+        POOL.release_cached()
+
+
+def run_forms(W_, forms, ev, args):
+    measured = {}
+    for form in forms:
+        _log(f"{W_.name}: timing {form}")
+        measured[form] = W_.measure(form, ev, args.steps, args.warmup)
+    return measured
+
+
+# ============================================================ CPU core, parity --
+def span(packed, lo, hi):
+    """Strings lo..hi-1 of a packed host set (a view of the buffer)."""
+    buf, off = packed
+    return buf[off[lo]:max(off[hi], off[lo] + 1)], off[lo:hi + 1] - off[lo]
+
+
+def cpu_leg(W_, args, dominant_s):
+    """The reference C core on the same inputs: every GPU form's result
+    compared with it (query rows or pairs in chunks; every pair unless
+    --cpu-rows or the exact searches' prefix cuts it), the all-core rate over
+    those chunks, the 1-thread rate on a prefix, and the work counts of the
+    roofline (cells the reference's exit rule evaluates)."""
+    from bitlev import workloads as W
+    from oracle import oracle as O
+
+    ref = O.Ref()
+    use_ref = ref.driver is not None
+    cpus = O.effective_cpus()
+    threads = max(1, min(cpus["effective"], 64))
+    wl, k, dt = W_.wl, W_.k, W_.dt
+    forms = list(W_.results)
+    equal = {f: True for f in forms}
+    first_bad = {}
+
+    def timed_once(fn):
+        t = time.perf_counter()
+        r = fn()
+        return r, time.perf_counter() - t
+
+    def timed(fn, min_s=0.5):
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            r = fn()
+            reps += 1
+            el = time.perf_counter() - t0
+            if el >= min_s:
+                return r, el / reps
+
+    extra = {}
+    if W_.kind == "cdist":
+        nq, nc = wl["nq"], wl["nc"]
+        exact = k < 0
+        rows = min(nq, args.cpu_rows or (32 if exact else nq))
+        chunk = max(1, min(64, (256 << 20) // (4 * nc)))
+
+        def run_cpu(sample_q, th):
+            return ref.cross(sample_q, W_.hy, k, th) if use_ref else O.cross(sample_q, W_.hy, k, th)
+
+        t_all = 0.0
+        for lo in range(0, rows, chunk):
+            hi = min(rows, lo + chunk)
+            sq = span(W_.hx, lo, hi)
+            d, t = timed_once(lambda: run_cpu(sq, threads))
+            t_all += t
+            want = O.saturate(d, dt)
+            for f in forms:
+                if equal[f] and not np.array_equal(want, W_.results[f][lo:hi]):
+                    equal[f] = False
+                    bad = np.argwhere(want != W_.results[f][lo:hi])[0]
+                    first_bad[f] = [int(bad[0]) + lo, int(bad[1])]
+            del d, want
+        rows1 = min(rows, 2 if exact else (4 if nc > 1_000_000 else 64))
+        _, t_one = timed(lambda: run_cpu(W.head(W_.hx, rows1), 1))
+        pairs_sample, pairs_one = rows * nc, rows1 * nc
+        sample_txt = (f"first {rows} of {nq} queries x all {nc:,} choices (all cores: parity of "
+                      f"every form, {chunk}-query chunks); first {rows1} queries (1 thread)")
+        # work counts: cells the reference's exit rule evaluates (SURVEY-style
+        # definition: m x the columns scanned), counted on a query prefix
+        mq = np.diff(W_.hx[1]).astype(np.float64)
+        nlen = np.diff(W_.hy[1]).astype(np.float64)
+        if exact:
+            extra["cells_reference_rule"] = float(mq.sum() * nlen.sum())
+        else:
+            rows_cnt = min(nq, 16 if nc > 1_000_000 else 128)
+            sc = W.head(W_.hx, rows_cnt)
+            O.count_rule("reference")
+            _, cols = O.cross(sc, W_.hy, k, threads, cols=True)
+            extra["cells_reference_rule"] = float((cols.astype(np.float64) * mq[:rows_cnt, None]).sum()) \
+                * nq / rows_cnt
+            O.count_rule("staged")
+            _, cols = O.cross(sc, W_.hy, k, threads, cols=True)
+            open_mask = count_open_mask(sc, W_.hy, k)
+            extra["cells_staged_open"] = float((cols.astype(np.float64) * open_mask).sum()) * nq / rows_cnt
+            extra["open_pairs"] = float(open_mask.sum()) * nq / rows_cnt
+            O.count_rule("reference")
+            extra["cells_counted_on_queries"] = rows_cnt
+            del cols
+    else:
+        n = wl["n"]
+        rows = min(n, args.cpu_rows or n)
+        chunk = 1 << 21
+
+        def run_cpu(a, b, th):
+            return ref.pairs(a, b, k, True, th) if use_ref else O.pairs(a, b, k, True, th)
+
+        t_all = 0.0
+        for lo in range(0, rows, chunk):
+            hi = min(rows, lo + chunk)
+            sa, sb = span(W_.hx, lo, hi), span(W_.hy, lo, hi)
+            d, t = timed_once(lambda: run_cpu(sa, sb, threads))
+            t_all += t
+            want = O.saturate(d, dt)
+            for f in forms:
+                if equal[f] and not np.array_equal(want, W_.results[f][lo:hi]):
+                    equal[f] = False
+                    first_bad[f] = int(np.argwhere(want != W_.results[f][lo:hi])[0][0]) + lo
+        rows1 = min(rows, 2_000_000)
+        _, t_one = timed(lambda: run_cpu(W.head(W_.hx, rows1), W.head(W_.hy, rows1), 1))
+        pairs_sample, pairs_one = rows, rows1
+        sample_txt = (f"first {rows:,} of {n:,} pairs (all cores: parity of every form); "
+                      f"first {rows1:,} (1 thread)")
+        la, lb = np.diff(W_.hx[1]).astype(np.float64), np.diff(W_.hy[1]).astype(np.float64)
+        extra["cells_reference_rule"] = float((la * lb).sum())
+
+    parity = {f: {"checked": pairs_sample, "equal": equal[f],
+                  **({"first_mismatch": first_bad[f]} if f in first_bad else {})} for f in forms}
+    parity["against"] = "reference C core (csrc/myers.c)" if use_ref else "oracle"
+    cpu = {
+        "value": round(pairs_sample / t_all / 1e6, 3),
+        "unit": "M pairs/s",
+        "cores": threads,
+        "kind": "reference" if use_ref else "port",
+        "sample": sample_txt,
+        "single_thread": round(pairs_one / t_one / 1e6, 3),
+        "host_cpus": cpus,
+        "build": "-O3 -std=c11 (portable, no -march=native)",
+    }
+    for f in forms:
+        if not equal[f]:
+            _log(f"WARNING: {W_.name} {f}: GPU results differ from the CPU reference at "
+                 f"{first_bad[f]}")
+    return parity, cpu, extra
+
+
+# ============================================================ rooflines --
+def roofline(W_, form, kernel_ms, work, traffic):
+    """The dominant kernel of `form` against its roofline, with the kernel's
+    mean launch time from HIP events on its stream.
+
+    * count kernel (thresholded searches, batched): it settles most pairs by
+      exact lower bounds without scanning a cell, so the floor it cannot skip
+      is HBM: the dense result it writes plus every choice's length and
+      profiles and the queries, once. The reference-rule cell rate
+      (cells x 10/32 INT32 ops / 39.3 T) is reported beside it and exceeds 1
+      where the bounds skip the DP; the kernel's own VALU work (6 ops per
+      pair for the bound test + 10/32 per staged cell of the pairs the bounds
+      leave open) is `other_bound`.
+    * scanning kernels: INT32 VALU, cells = the reference's definition
+      (untrimmed m x n for exact workloads, m x the columns its exit rule
+      evaluates with max_dist), 10/32 ops per cell."""
+    wl, dt = W_.wl, W_.dt
+    t = kernel_ms / 1e3
+    cells = work["cells_reference_rule"]
+    cell_tops = cells * OPS_PER_CELL / t / 1e12
+    cells_view = {"cells_per_launch": cells, "ops_per_cell": OPS_PER_CELL,
+                  "gcups": round(cells / t / 1e9, 1),
+                  "achieved_tops": round(cell_tops, 3),
+                  "frac": round(cell_tops / VALU_PEAK_TOPS, 4)}
+    if W_.kind == "cdist":
+        nq, nc = wl["nq"], wl["nc"]
+        alg_bytes = nq * nc * dt.itemsize + nc * 4 + int(W_.hx[1][-1]) + nq * 8
+        profiles = W_.params and W_.params.get("mode") == "count" and form == "batched"
+        if profiles:
+            alg_bytes += nc * 32 + nq * 32
+    else:
+        n = wl["n"]
+        alg_bytes = int(W_.hx[1][-1]) + int(W_.hy[1][-1]) + 16 * n + n * dt.itemsize
+        profiles = False
+    gbs = alg_bytes / t / 1e9
+    hbm = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_launch": alg_bytes}
+    count_mode = profiles and W_.k >= 0
+    if count_mode:
+        alg_ops = W_.pairs_total * COUNT_OPS_PER_PAIR + work.get("cells_staged_open", 0.0) * OPS_PER_CELL
+        tops = alg_ops / t / 1e12
+        r = dict(hbm)
+        r["cells_view"] = dict(cells_view, note=(
+            "reference-rule cells at 10/32 INT32 ops per cell; above 1 because the kernel "
+            "settles pairs by exact lower bounds (length, byte-class and 2-gram counts) "
+            "without scanning their cells"))
+        r["other_bound"] = {"bound": "valu", "achieved": round(tops, 3),
+                            "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
+                            "frac": round(tops / VALU_PEAK_TOPS, 4),
+                            "algorithmic_ops_per_launch": alg_ops,
+                            "ops_per_pair_bound_test": COUNT_OPS_PER_PAIR,
+                            "cells_staged_open_per_launch": work.get("cells_staged_open")}
+    else:
+        r = {"bound": "valu", "achieved": round(cell_tops, 3), "peak": round(VALU_PEAK_TOPS, 2),
+             "unit": "TOP/s", "frac": round(cell_tops / VALU_PEAK_TOPS, 4),
+             "algorithmic_ops_per_launch": cells * OPS_PER_CELL,
+             "cells_per_launch": cells, "ops_per_cell": OPS_PER_CELL,
+             "gcups": cells_view["gcups"], "other_bound": hbm}
+    r["traffic"], r["traffic_source"] = traffic
+    r["kernel"] = kernel_name(W_, form)
+    r["kernel_ms"] = round(kernel_ms, 4)
+    return r
+
+
+def class_profiles(packed, chunk=1 << 18, grams=1):
+    """(n, 16) saturated counts of a packed set as the search kernels'
+    profiles hold them: byte classes b & 15 (grams=1) or 2-gram classes
+    4 ((x >> 1) & 3) + ((y >> 1) & 3) over each string's adjacent bytes
+    (grams=2)."""
+    buf, off = packed
+    n = len(off) - 1
+    out = np.empty((n, 16), np.int16)
+    for lo in range(0, n, chunk):
+        hi = min(n, lo + chunk)
+        seg = np.asarray(buf[off[lo]:off[hi]]).astype(np.int32)
+        lens = np.diff(off[lo:hi + 1])
+        ids = np.repeat(np.arange(hi - lo, dtype=np.int32), lens)
+        if grams == 1:
+            cls = seg & 15
+        else:
+            c2 = (seg >> 1) & 3
+            cls = 4 * c2[:-1] + c2[1:]
+            ids = ids[:-1]
+            same = ids == np.repeat(np.arange(hi - lo, dtype=np.int32), lens)[1:]
+            cls, ids = cls[same], ids[same]      # no 2-gram across two strings
+        cnt = np.bincount(ids * 16 + cls, minlength=(hi - lo) * 16)
+        out[lo:hi] = np.minimum(cnt.reshape(hi - lo, 16), 255)
+    return out
+
+
+def count_open_mask(sq, hy, k):
+    """(rows, nc) mask of the pairs the length, class-count and 2-gram bounds
+    leave for a scan (the count kernel's survivors), for the sampled queries
+    sq."""
+    pq, pc = class_profiles(sq), class_profiles(hy)
+    pq2, pc2 = class_profiles(sq, grams=2), class_profiles(hy, grams=2)
+    mq = np.diff(sq[1]).astype(np.int64)
+    nc_len = np.diff(hy[1]).astype(np.int64)
+    t_c = np.maximum(nc_len - 1, 0)
+    mask = np.empty((len(mq), len(nc_len)), bool)
+    for i in range(len(mq)):
+        l1 = np.abs(pc - pq[i]).sum(1, dtype=np.int64)
+        gap = np.abs(nc_len - mq[i])
+        small = (mq[i] <= 255) & (nc_len <= 255)
+        m1 = (gap <= k) & (l1 + np.where(small, gap, 0) <= 2 * k)
+        l2 = np.abs(pc2 - pq2[i]).sum(1, dtype=np.int64)
+        gap2 = np.abs(t_c - max(mq[i] - 1, 0))
+        small2 = (mq[i] <= 256) & (nc_len <= 256)
+        mask[i] = m1 & (l2 + np.where(small2, gap2, 0) <= 4 * k)
+    return mask
+
+
+def measured_traffic(workload, form):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    profiles (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, each scaled by what
+    the counter reads for a device copy of known size), or (None, None)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        e = t[workload][form]
+        return int(e["bytes"]), f"profiles/traffic.json: {e['kernel']}"
+    except (OSError, ValueError, KeyError, TypeError):
+        return None, None
+
+
+def kernel_name(W_, form):
+    if W_.kind == "cdist":
+        mq = int(np.diff(W_.hx[1]).max())
+        nw = 1 if mq <= 64 else 2 if mq <= 128 else 4 if mq <= 256 else 8
+        if form.startswith("plain"):
+            return f"cdist_class_kernel<{nw}>"
+        if W_.params and W_.params.get("mode") == "count" and W_.k >= 0:
+            return f"cdist_count_kernel<{nw}>"
+        return f"cdist_batched_kernel<{nw}>"
+    return ("pairs_class_kernel (one launch per length class)" if form == "plain"
+            else "pairs pipeline (classify, scan, scatter, pairs_batched_kernel) or "
+                 "pairs_direct_kernel")
+
+
+# ============================================================ extra legs --
+def cold_call(W_, ev):
+    """A one-shot search on packed choices already in HBM: the choice set's
+    stripes and profiles built inside the timed region (host layout pass,
+    build kernel, sync), then the batched search. Median of 3 calls, each
+    building fresh stripes."""
+    from bitlev import hip
+    sp = W_.stream.ptr
+    ws = hip.Workspace()
+    walls, evs, builds = [], [], []
+    for _ in range(3):
+        Yc = hip.Packed(*W_.hy, stream=sp)
+        W_.stream.sync()
+        a, b, c = ev.new(), ev.new(), ev.new()
+        t0 = time.perf_counter()
+        ev.record(a)
+        Yc.stripes(sp)
+        ev.record(b)
+        hip.run_cdist(W_.X, Yc, W_.out, W_.k, sp, "batched", None, ws, W_.params)
+        ev.record(c)
+        W_.stream.sync()
+        walls.append((time.perf_counter() - t0) * 1e3)
+        builds.append(ev.ms(a, b))
+        evs.append(ev.ms(a, c))
+        ev.destroy(a, b, c)
+        del Yc
+    w = float(np.median(walls))
+    return {"ms": round(w, 4), "value": round(W_.pairs_total / (w / 1e3) / 1e6, 3),
+            "gpu_ms": round(float(np.median(evs)), 4),
+            "stripes_build_ms": round(float(np.median(builds)), 4),
+            "note": "packed choices resident; stripes + profiles built inside the timed call"}
+
+
+def matches_leg(W_, ev, args):
+    """The same search with the threshold match-list output (bl_cdist_matches):
+    no dense nq x nc result, only the (q, c, d) of cells at or below k. The
+    list is checked against the dense result of the batched form."""
+    from bitlev import _capi, hip
+    X, Y, k, params = W_.X, W_.Y, W_.k, W_.params
+    dq, dc, dd = hip.cdist_matches_into(X, Y, k, None, None, params)
+    n = dq.shape[0]
+    dense = W_.results.get("batched")
+    ok = None
+    if dense is not None:
+        hq, hc, hd = dq.get(), dc.get(), dd.get()
+        qi, ci = np.nonzero(dense <= k)
+        order = np.lexsort((hc, hq))
+        ok = bool(len(qi) == n and np.array_equal(hq[order], qi) and np.array_equal(hc[order], ci)
+                  and np.array_equal(hd[order], dense[qi, ci].astype(np.int32)))
+    cap = max(n, 1)
+    bq, bc, bd = (hip.DeviceArray.empty((cap,), t) for t in (np.int32, np.int64, np.int32))
+    count = hip.DeviceBuffer(8)
+    ws = hip.Workspace()
+    mq = min(X.max_len, hip.BATCHED_MAX_QUERY)
+    w = ws.get(_capi.lib().bl_cdist_workspace_size(X.n, mq))
+    sp = ev.stream
+    pr = hip._cdist_params(params)
+
+    def step():
+        _capi.call("bl_cdist_matches", X.d_buf.ptr, X.d_off.ptr, X.n, Y.stripes(sp).ref, mq, int(k),
+                   None, bq.ptr, bc.ptr, bd.ptr, cap, count.ptr, w.ptr, w.nbytes, pr, sp)
+
+    wall, per = timed_steps(step, ev, args.steps, args.warmup)
+    ms = wall * 1e3 / args.steps
+    return {"value": round(W_.pairs_total / (ms / 1e3) / 1e6, 3), "ms_per_step": round(ms, 4),
+            "kernel_ms_mean": round(float(np.mean(per)), 4), "matches": int(n),
+            "output_bytes": int(n * 16), "equal_to_dense": ok}
+
+
+def ingest_leg(W_, n):
+    """End-to-end pairwise from Python lists of bytes (the first n pairs of the
+    workload): the staged pipeline (chunks packed on a thread pool into pinned
+    buffers, copies and kernels overlapped) against one whole-batch pack and
+    pageable copy. Host-clock wall time; results checked against the
+    resident run's."""
+    from bitlev import hip, ingest
+    n = min(n, len(W_.hx[1]) - 1)
+    (ba, oa), (bb, ob) = W_.hx, W_.hy
+    A = [ba[oa[i]:oa[i + 1]].tobytes() for i in range(n)]
+    B = [bb[ob[i]:ob[i + 1]].tobytes() for i in range(n)]
+    want = W_.results["batched"][:n]
+    md = None if W_.k < 0 else W_.k
+
+    def timed(fn):
+        fn()                                   # warm: pinned slots, workspace, tuning cache
+        best, r = 1e9, None
+        for _ in range(2):
+            t = time.perf_counter()
+            r = fn()
+            best = min(best, time.perf_counter() - t)
+        return best, r
+
+    t_staged, r1 = timed(lambda: hip.pairwise(A, B, max_dist=md, dtype=W_.dt))
+    saved = hip.STAGED_PAIRS
+    hip.STAGED_PAIRS = 1 << 62
+    try:
+        t_whole, r2 = timed(lambda: hip.pairwise(A, B, max_dist=md, dtype=W_.dt))
+    finally:
+        hip.STAGED_PAIRS = saved
+    if not (np.array_equal(r1, want) and np.array_equal(r2, want)):
+        raise RuntimeError("ingest leg: end-to-end results differ from the resident run")
+    return {"pairs": n, "threads": ingest.effective_threads(),
+            "staged_mpairs_s": round(n / t_staged / 1e6, 2),
+            "whole_batch_mpairs_s": round(n / t_whole / 1e6, 2),
+            "staged_over_whole": round(t_whole / t_staged, 2), "results_match": True}
+
+
+# ============================================================ drivers --
+def bench_workload(name, args, seed, ev, stream, full):
+    """All measurements of one workload. full: the headline record (also the
+    plain-stripes form, the cold call, the match list and the ingest leg)."""
+    W_ = Workload(name, args, seed, stream)
+    forms = ["plain", "batched"] if args.form == "both" else [args.form]
+    measured = run_forms(W_, forms, ev, args)
+    if full and W_.kind == "cdist" and "plain" in forms:
+        measured["plain-stripes"] = W_.measure("plain-stripes", ev, args.steps, args.warmup)
+    best = max(("plain", "batched"), key=lambda f: measured[f]["value"] if f in measured else -1)
+    rec = {"config": config_of(name, W_.wl), "form": best, "setup_s": round(W_.setup_s, 2)}
+    for f, m in measured.items():
+        rec[f.replace("-", "_")] = rounded(m)
+    if "plain" in measured and "batched" in measured:
+        rec["batched_over_plain"] = round(measured["plain"]["ms_per_step"]
+                                          / measured["batched"]["ms_per_step"], 3)
+    if not args.no_cpu:
+        parity, cpu, work = cpu_leg(W_, args, None)
+        rec["parity"] = parity
+        rec["cpu_baseline"] = cpu
+        for f in ("batched", "plain"):
+            if f in measured:
+                traffic = measured_traffic(name, f)
+                rec.setdefault("rooflines", {})[f] = roofline(W_, f, measured[f]["kernel_ms_mean"],
+                                                              work, traffic)
+        rec["roofline"] = rec["rooflines"][best]
+    if full and W_.kind == "cdist":
+        if "batched" in measured:
+            rec["cold_call"] = cold_call(W_, ev)
+        if W_.k >= 0 and "batched" in measured:
+            rec["matches"] = matches_leg(W_, ev, args)
+    if full and W_.kind == "pairs" and args.ingest_pairs > 0 and "batched" in measured:
+        rec["ingest"] = ingest_leg(W_, args.ingest_pairs)
+    W_.free()
+    return rec, measured[best]
+
+
+def compact(rec):
+    """The fields of a workload record kept in the headline line's `configs`."""
+    keep = ("config", "form", "plain", "batched", "batched_over_plain", "parity", "cpu_baseline",
+            "roofline")
+    out = {k: rec[k] for k in keep if k in rec}
+    for f in ("plain", "batched"):
+        if f in out:
+            out[f] = {k: v for k, v in out[f].items() if k in ("value", "ms_per_step", "kernel_ms_mean",
+                                                                "params", "graph")}
+    if "rooflines" in rec and "plain" in rec["rooflines"]:
+        p = rec["rooflines"]["plain"]
+        out["plain_roofline"] = {k: p[k] for k in ("bound", "achieved", "unit", "frac", "kernel")}
+    return out
 
 
 def main():
@@ -157,103 +749,19 @@ def main():
         copy_probe(args.copy_probe)
         return
     seed = args.seed if args.seed is not None else W.SEED
+    info = device_info(_capi)
     if args.workload == "serving":
         res = {"metric": METRIC, "n_gpus": 1, "data": "synthetic", "dtype": "u32",
-               "config": {"workload": "serving", **serving_config(args)},
-               "device": device_info(_capi)}
+               "config": {"workload": "serving", **serving_config(args)}, "device": info}
         sv = serving_leg(args, seed)
         res.update(value=sv["achieved_mpairs_s"], unit="M pairs/s", higher_is_better=True,
                    scaling="weak", vs_baseline=None, serving=sv)
         print(json.dumps(res), flush=True)
         return
-    wl = W.WORKLOADS[args.workload]
-    k = -1 if wl["k"] is None else int(wl["k"])
-    dt = np.dtype(wl["dtype"])
-    info = device_info(_capi)
-
-    from bitlev import autotune
 
     stream = hip.Stream()
-    sp = stream.ptr
-    t0 = time.perf_counter()
-    if wl["kind"] == "cdist":
-        hq, hc = W.cdist(wl["gen"], wl["nq"], wl["nc"], seed)
-        X, Y = hip.Packed(*hq, stream=sp), hip.Packed(*hc, stream=sp)
-        Y.stripes(sp)                      # the resident search layout of the choice set
-        out = hip.DeviceArray.empty((X.n, Y.n), dt)
-        plan = hip.cdist_plan(X, Y)
-        pairs_total = X.n * Y.n
-
-        def make_step(form, params=None, ws=None):
-            return lambda: hip.run_cdist(X, Y, out, k, sp, form, plan, ws, params)
-
-        def tune():
-            return autotune.tune_cdist(X, Y, out, k, sp, use_cache=not args.retune, log=_log)
-    else:
-        ha, hb = W.pairs(wl["gen"], wl["n"], seed)
-        X, Y = hip.Packed(*ha, stream=sp), hip.Packed(*hb, stream=sp)
-        out = hip.DeviceArray.empty((X.n,), dt)
-        plan = hip.pairs_plan(X, Y)
-        pairs_total = X.n
-
-        def make_step(form, params=None, ws=None):
-            return lambda: hip.run_pairs(X, Y, out, k, True, sp, form, plan, ws, params)
-
-        def tune():
-            return autotune.tune_pairs(X, Y, out, k, True, sp, use_cache=not args.retune, log=_log)
-    stream.sync()
-    setup_s = time.perf_counter() - t0
-
-    ev = Events(_capi, sp)
-    forms = ["plain", "batched"] if args.form == "both" else [args.form]
-    measured = {}
-    tuned = None
-    for form in forms:
-        step = make_step(form)
-        extra = {}
-        if form == "batched":
-            t1 = time.perf_counter()
-            if args.params:
-                tuned = json.loads(args.params)
-            else:
-                tuned = tune() if not args.no_tune else None
-            extra["tune_s"] = round(time.perf_counter() - t1, 2)
-            extra["params"] = tuned
-            ws = hip.Workspace()
-            step = make_step(form, tuned, ws)
-            if not args.no_graph:
-                # graph replay or eager launches, whichever is faster: a graph
-                # saves per-launch host work on multi-kernel pipelines but adds
-                # ~10 us, which dominates a batch that runs in ~10 us
-                g = hip.Graph(step, stream)
-                we, _ = timed_steps(step, ev, 20, 2)
-                wg, _ = timed_steps(g.launch, ev, 20, 2)
-                extra["probe_ms_per_step"] = {"eager": round(we * 50, 4), "graph": round(wg * 50, 4)}
-                extra["graph"] = bool(wg <= we)
-                if extra["graph"]:
-                    step = g.launch
-        wall, per = timed_steps(step, ev, args.steps, args.warmup)
-        ms_step = wall * 1e3 / args.steps
-        measured[form] = {
-            "value": pairs_total / (ms_step / 1e3) / 1e6,
-            "ms_per_step": ms_step,
-            "kernel_ms_mean": float(np.mean(per)),
-            "kernel_ms_min": float(np.min(per)),
-            **extra,
-        }
-    if wl["kind"] == "cdist" and k >= 0 and "batched" in measured:
-        measured_matches = matches_leg(hip, _capi, X, Y, k, tuned, ev, args)
-    else:
-        measured_matches = None
-    # the faster form is what a caller of this build gets (both are reported)
-    best = max(measured, key=lambda f: measured[f]["value"])
-    main_m = measured[best]
-    ms_step = main_m["ms_per_step"]
-    if "plain" in measured and "batched" in measured:
-        speedup = measured["plain"]["ms_per_step"] / measured["batched"]["ms_per_step"]
-    else:
-        speedup = None
-
+    ev = Events(_capi, stream.ptr)
+    rec, main_m = bench_workload(args.workload, args, seed, ev, stream, full=True)
     res = {
         "metric": METRIC,
         "value": round(main_m["value"], 3),
@@ -261,74 +769,34 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": round(main_m["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic",
-        "config": config_of(args.workload, wl),
-        "form": best,
-        "plain": rounded(measured["plain"]) if "plain" in measured else None,
-        "batched": rounded(measured["batched"]) if "batched" in measured else None,
-        "batched_over_plain": round(speedup, 3) if speedup else None,
-        "matches": measured_matches,
+        **rec,
         "device": info,
-        "setup_s": round(setup_s, 2),
     }
-    plain = main_m
-
-    if wl["kind"] == "pairs" and args.ingest_pairs > 0:
-        res["ingest"] = ingest_leg(hip, ha, hb, k, dt, out, args.ingest_pairs, stream)
-    if not args.no_cpu:
-        cpu_leg(res, args, wl, k, dt, hq if wl["kind"] == "cdist" else ha,
-                hc if wl["kind"] == "cdist" else hb, out, plain)
-    if not args.no_serving and args.workload == "cdist-L64-k2":
+    sel = args.configs if args.configs is not None else ("all" if args.workload == HEADLINE else "none")
+    names = [] if sel == "none" else (list(OTHER_CONFIGS) if sel == "all" else sel.split(","))
+    if names:
+        cargs = argparse.Namespace(**vars(args))
+        cargs.params = None
+        res["configs"] = {}
+        for nm in names:
+            if nm == args.workload:
+                continue
+            _log(f"config {nm}")
+            crec, _ = bench_workload(nm, cargs, seed, ev, stream, full=False)
+            res["configs"][nm] = compact(crec)
+    if not args.no_serving and args.workload == HEADLINE:
         # the metric's latency half: the serving mix at a fixed offered load
-        del out, X, Y
         res["serving"] = serving_leg(args, seed)
     print(json.dumps(res), flush=True)
 
 
-def ingest_leg(hip, ha, hb, k, dt, out, n, stream):
-    """End-to-end pairwise from Python lists of bytes (the first n pairs of the
-    workload): the staged pipeline (chunks packed on a thread pool into pinned
-    buffers, copies and kernels overlapped) against one whole-batch pack and
-    pageable copy. Host-clock wall time; results checked against the
-    resident run's."""
-    from bitlev import ingest
-    n = min(n, len(ha[1]) - 1)
-    (ba, oa), (bb, ob) = ha, hb
-    A = [ba[oa[i]:oa[i + 1]].tobytes() for i in range(n)]
-    B = [bb[ob[i]:ob[i + 1]].tobytes() for i in range(n)]
-    want = out.get(stream.ptr)[:n]
-    md = None if k < 0 else k
-
-    def timed(fn):
-        fn()                                   # warm: pinned slots, workspace, tuning cache
-        best, r = 1e9, None
-        for _ in range(2):
-            t = time.perf_counter()
-            r = fn()
-            best = min(best, time.perf_counter() - t)
-        return best, r
-
-    t_staged, r1 = timed(lambda: hip.pairwise(A, B, max_dist=md, dtype=dt))
-    saved = hip.STAGED_PAIRS
-    hip.STAGED_PAIRS = 1 << 62
-    try:
-        t_whole, r2 = timed(lambda: hip.pairwise(A, B, max_dist=md, dtype=dt))
-    finally:
-        hip.STAGED_PAIRS = saved
-    if not (np.array_equal(r1, want) and np.array_equal(r2, want)):
-        raise RuntimeError("ingest leg: end-to-end results differ from the resident run")
-    _log(f"ingest: {n} pairs staged {t_staged:.3f}s, whole batch {t_whole:.3f}s")
-    return {"pairs": n, "threads": ingest.effective_threads(),
-            "staged_mpairs_s": round(n / t_staged / 1e6, 2),
-            "whole_batch_mpairs_s": round(n / t_whole / 1e6, 2),
-            "staged_over_whole": round(t_whole / t_staged, 2), "results_match": True}
-
-
+# ============================================================ serving --
 def serving_config(args):
     from bitlev import workloads as W
     S = W.SERVING
@@ -448,289 +916,6 @@ def _serving_oracle(reqs, sizes, got, sets):
         ok = ok and bool(np.array_equal(O.saturate(want, np.dtype(r["dtype"])), got[i]))
         checked += 1
     return ok, checked
-
-
-def matches_leg(hip, capi, X, Y, k, params, ev, args):
-    """The same search with the threshold match-list output (bl_cdist_matches):
-    no dense nq x nc result, only the (q, c, d) of cells at or below k."""
-    dq, dc, dd = hip.cdist_matches_into(X, Y, k, None, None, params)
-    n = dq.shape[0]
-    cap = max(n, 1)
-    bq, bc, bd = (hip.DeviceArray.empty((cap,), t) for t in (np.int32, np.int64, np.int32))
-    count = hip.DeviceBuffer(8)
-    ws = hip.Workspace()
-    mq = min(X.max_len, hip.BATCHED_MAX_QUERY)
-    w = ws.get(capi.lib().bl_cdist_workspace_size(X.n, mq))
-    sp = ev.stream
-    pr = hip._cdist_params(params)
-
-    def step():
-        capi.call("bl_cdist_matches", X.d_buf.ptr, X.d_off.ptr, X.n, Y.stripes(sp).ref, mq, int(k),
-                  None, bq.ptr, bc.ptr, bd.ptr, cap, count.ptr, w.ptr, w.nbytes, pr, sp)
-
-    wall, per = timed_steps(step, ev, args.steps, args.warmup)
-    ms = wall * 1e3 / args.steps
-    return {"value": round(X.n * Y.n / (ms / 1e3) / 1e6, 3), "ms_per_step": round(ms, 4),
-            "kernel_ms_mean": round(float(np.mean(per)), 4), "matches": int(n),
-            "output_bytes": int(n * 16)}
-
-
-def rounded(d):
-    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
-
-
-def _flt(v):
-    return round(v, 4) if isinstance(v, float) else v
-
-
-def config_of(name, wl):
-    c = {"workload": name}
-    if wl["kind"] == "cdist":
-        c.update(queries=wl["nq"], choices=wl["nc"])
-    else:
-        c.update(pairs=wl["n"])
-    c.update(max_dist=wl["k"], out_dtype=wl["dtype"], return_device=True, gpus=1)
-    return c
-
-
-def cpu_leg(res, args, wl, k, dt, hx, hy, out, plain):
-    """Reference C core on a bounded sample: parity of those GPU results, the
-    1-thread and all-core rates, and the useful work (band cells) of the
-    sample for the roofline. The sample is a prefix of the same workload."""
-    from bitlev import workloads as W
-    from oracle import oracle as O
-
-    ref = O.Ref()
-    use_ref = ref.driver is not None
-    cpus = O.effective_cpus()
-    threads = max(1, min(cpus["effective"], 64))
-
-    def timed(fn, min_s=0.5):
-        """Wall seconds per call of fn, repeated until min_s has passed."""
-        reps, t0 = 0, time.perf_counter()
-        while True:
-            r = fn()
-            reps += 1
-            el = time.perf_counter() - t0
-            if el >= min_s:
-                return r, el / reps
-
-    if wl["kind"] == "cdist":
-        big = wl["nc"] > 1_000_000
-        rows = min(wl["nq"], args.cpu_rows or (wl["nq"] // 4 if big else wl["nq"]))
-        rows1 = min(rows, 4 if big else 64)
-        rows_cnt = min(rows, 16 if big else 128)
-        sq = W.head(hx, rows)
-        gpu_rows = np.empty((rows, wl["nc"]), dt)
-        _copy_rows(out, gpu_rows)
-
-        def run_cpu(sample_q, th):
-            return ref.cross(sample_q, hy, k, th) if use_ref else O.cross(sample_q, hy, k, th)
-
-        d, t_all = timed(lambda: run_cpu(sq, threads))
-        equal = bool(np.array_equal(O.saturate(d, dt), gpu_rows))
-        del d
-        _, t_one = timed(lambda: run_cpu(W.head(hx, rows1), 1))
-        sc = W.head(hx, rows_cnt)
-        m = np.diff(sc[1]).astype(np.float64)
-        cells = {}
-        open_mask = count_open_mask(sc, hy, k) if k >= 0 else None
-        for rule in ("staged", "reference"):
-            O.count_rule(rule)
-            _, cols = O.cross(sc, hy, k, threads, cols=True)
-            c = cols.astype(np.float64)
-            cells[rule] = float(c.sum() if rule == "staged" else (c * m[:, None]).sum()) \
-                * wl["nq"] / rows_cnt
-            if rule == "staged" and open_mask is not None:
-                cells["open"] = float((c * open_mask).sum()) * wl["nq"] / rows_cnt
-            del cols, c
-        O.count_rule("reference")
-        cells_total, cells_ref = cells["staged"], cells["reference"]
-        cells_extra = {"open": cells["open"]} if "open" in cells else {}
-        pairs_sample, pairs_one = rows * wl["nc"], rows1 * wl["nc"]
-        sample_txt = (f"first {rows} of {wl['nq']} queries x all {wl['nc']:,} choices "
-                      f"(all cores, parity); first {rows1} queries (1 thread); cells counted "
-                      f"on the first {rows_cnt} queries")
-    else:
-        n = wl["n"]
-        rows = min(n, (args.cpu_rows or 1 << 24))
-        sa, sb = W.head(hx, rows), W.head(hy, rows)
-        gpu_part = np.empty(rows, dt)
-        _copy_prefix(out, gpu_part)
-
-        def run_cpu(a, b, th):
-            return ref.pairs(a, b, k, True, th) if use_ref else O.pairs(a, b, k, True, th)
-
-        d, t_all = timed(lambda: run_cpu(sa, sb, threads))
-        equal = bool(np.array_equal(O.saturate(d, dt), gpu_part))
-        rows1 = min(rows, 2_000_000)
-        _, t_one = timed(lambda: run_cpu(W.head(hx, rows1), W.head(hy, rows1), 1))
-        O.count_rule("staged")
-        _, cols = O.pairs(sa, sb, k, True, threads, cols=True)
-        O.count_rule("reference")
-        cells_total = float(cols.astype(np.float64).sum()) * n / rows
-        la, lb = np.diff(hx[1]).astype(np.float64), np.diff(hy[1]).astype(np.float64)
-        cells_ref = float((la * lb).sum())   # the reference's GCUPS cells: untrimmed m x n
-        cells_extra = {}
-        pairs_sample, pairs_one = rows, rows1
-        sample_txt = f"first {rows:,} of {n:,} pairs (all cores, parity); first {rows1:,} (1 thread)"
-
-    res["parity"] = {"checked": pairs_sample, "equal": equal,
-                     "against": "reference C core" if use_ref else "oracle"}
-    res["cpu_baseline"] = {
-        "value": round(pairs_sample / t_all / 1e6, 3),
-        "unit": "M pairs/s",
-        "cores": threads,
-        "kind": "reference" if use_ref else "port",
-        "sample": sample_txt,
-        "single_thread": round(pairs_one / t_one / 1e6, 3),
-        "host_cpus": cpus,
-        "build": "-O3 -std=c11 (portable, no -march=native)",
-    }
-    # Dominant kernel: the timed step's event time; for the batched form the
-    # step is the persistent launch (plus, for pairs, the bucketing kernels,
-    # which the rocprof summaries break out).
-    kernel_s = plain["kernel_ms_mean"] / 1e3
-    achieved = cells_total * OPS_PER_CELL / kernel_s / 1e12
-    form = res.get("form", "plain")
-    params = (res.get("batched") or {}).get("params") if form == "batched" else None
-    count_mode = wl["kind"] == "cdist" and bool(params) and params.get("mode") == "count"
-    traffic, traffic_src = measured_traffic(args.workload, form)
-    cells_view = {
-        "cells_per_launch": cells_total,
-        "cells_reference_rule": cells_ref,
-        "gcups": round(cells_total / kernel_s / 1e9, 1),
-        "gcups_reference_rule": round(cells_ref / kernel_s / 1e9, 1),
-        "ops_per_cell": OPS_PER_CELL,
-    }
-    if count_mode:
-        # The count kernel settles most pairs by the length and class-count
-        # bounds without scanning a cell. Its algorithmic work: the bound test
-        # on every pair (5 v_sad_u8 and a compare, 6 INT32 ops) plus the staged
-        # cells of the pairs the bounds leave open; its algorithmic bytes: the
-        # result matrix plus every choice's length and profile once. The
-        # roofline reported is the one the kernel is nearer.
-        nq, nc = len(hx[1]) - 1, len(hy[1]) - 1
-        alg_bytes = nq * nc * dt.itemsize + nc * (4 + 16) + int(hx[1][-1]) + nq * 24
-        gbs = alg_bytes / kernel_s / 1e9
-        cells_open = cells_extra.get("open", 0.0)
-        alg_ops = nq * nc * COUNT_OPS_PER_PAIR + cells_open * OPS_PER_CELL
-        tops = alg_ops / kernel_s / 1e12
-        hbm = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-               "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_launch": alg_bytes}
-        valu = {"bound": "valu", "achieved": round(tops, 3), "peak": round(VALU_PEAK_TOPS, 2),
-                "unit": "TOP/s", "frac": round(tops / VALU_PEAK_TOPS, 4),
-                "algorithmic_ops_per_launch": alg_ops, "ops_per_pair_bound_test": COUNT_OPS_PER_PAIR,
-                "cells_open_per_launch": cells_open}
-        main, other = (hbm, valu) if hbm["frac"] >= valu["frac"] else (valu, hbm)
-        res["roofline"] = {
-            **main,
-            "traffic": traffic,
-            "traffic_source": traffic_src,
-            "other_bound": other,
-            "staged_cell_view": {"achieved_top_s": round(achieved, 3), **cells_view,
-                                 "note": "counts cells of pairs the bounds settle without a scan"},
-            "kernel": kernel_name(wl, form, params),
-        }
-    else:
-        res["roofline"] = {
-            "bound": "valu",
-            "achieved": round(achieved, 3),
-            "peak": round(VALU_PEAK_TOPS, 2),
-            "unit": "TOP/s",
-            "frac": round(achieved / VALU_PEAK_TOPS, 4),
-            "traffic": traffic,
-            "traffic_source": traffic_src,
-            **cells_view,
-            "kernel": kernel_name(wl, form, params),
-        }
-    if not equal:
-        print("bench.py: WARNING: GPU results differ from the CPU reference on the sample",
-              file=sys.stderr)
-
-
-def class_profiles(packed, chunk=1 << 18, grams=1):
-    """(n, 16) saturated counts of a packed set as the search kernels'
-    profiles hold them: byte classes b & 15 (grams=1) or 2-gram classes
-    4 ((x >> 1) & 3) + ((y >> 1) & 3) over each string's adjacent bytes
-    (grams=2)."""
-    buf, off = packed
-    n = len(off) - 1
-    out = np.empty((n, 16), np.int16)
-    for lo in range(0, n, chunk):
-        hi = min(n, lo + chunk)
-        seg = np.asarray(buf[off[lo]:off[hi]]).astype(np.int32)
-        lens = np.diff(off[lo:hi + 1])
-        ids = np.repeat(np.arange(hi - lo, dtype=np.int32), lens)
-        if grams == 1:
-            cls = seg & 15
-        else:
-            c2 = (seg >> 1) & 3
-            cls = 4 * c2[:-1] + c2[1:]
-            ids = ids[:-1]
-            same = ids == np.repeat(np.arange(hi - lo, dtype=np.int32), lens)[1:]
-            cls, ids = cls[same], ids[same]      # no 2-gram across two strings
-        cnt = np.bincount(ids * 16 + cls, minlength=(hi - lo) * 16)
-        out[lo:hi] = np.minimum(cnt.reshape(hi - lo, 16), 255)
-    return out
-
-
-def count_open_mask(sq, hy, k):
-    """(rows, nc) mask of the pairs the length, class-count and 2-gram bounds
-    leave for a scan (the count kernel's survivors), for the sampled queries
-    sq."""
-    pq, pc = class_profiles(sq), class_profiles(hy)
-    pq2, pc2 = class_profiles(sq, grams=2), class_profiles(hy, grams=2)
-    mq = np.diff(sq[1]).astype(np.int64)
-    nc_len = np.diff(hy[1]).astype(np.int64)
-    t_c = np.maximum(nc_len - 1, 0)
-    mask = np.empty((len(mq), len(nc_len)), bool)
-    for i in range(len(mq)):
-        l1 = np.abs(pc - pq[i]).sum(1, dtype=np.int64)
-        gap = np.abs(nc_len - mq[i])
-        small = (mq[i] <= 255) & (nc_len <= 255)
-        m1 = (gap <= k) & (l1 + np.where(small, gap, 0) <= 2 * k)
-        l2 = np.abs(pc2 - pq2[i]).sum(1, dtype=np.int64)
-        gap2 = np.abs(t_c - max(mq[i] - 1, 0))
-        small2 = (mq[i] <= 256) & (nc_len <= 256)
-        mask[i] = m1 & (l2 + np.where(small2, gap2, 0) <= 4 * k)
-    return mask
-
-
-def measured_traffic(workload, form):
-    """HBM bytes per launch of the dominant kernel from the committed counter
-    profiles (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, each scaled by what
-    the counter reads for a device copy of known size), or (None, None)."""
-    path = os.path.join(ROOT, "profiles", "traffic.json")
-    try:
-        with open(path) as f:
-            t = json.load(f)
-        e = t[workload][form]
-        return int(e["bytes"]), f"profiles/traffic.json: {e['kernel']}"
-    except (OSError, ValueError, KeyError):
-        return None, None
-
-
-def kernel_name(wl, form, params=None):
-    if wl["kind"] == "cdist":
-        nw = {2: 1, 4: 4}.get(wl["gen"], 0)
-        if form == "plain":
-            return f"cdist_class_kernel<{nw}>"
-        if params and params.get("mode") == "count":
-            return f"cdist_count_kernel<{nw}>"
-        return f"cdist_batched_kernel<{nw}>"
-    return ("pairs_class_kernel<1,2,4> (5 class launches)" if form == "plain"
-            else "pairs pipeline (classify, scan, scatter, pairs_batched_kernel)")
-
-
-def _copy_rows(out, host):
-    from bitlev import _capi
-    _capi.call("bl_memcpy_dtoh", host.ctypes.data, out.ptr, host.nbytes, None)
-    _capi.call("bl_stream_sync", None)
-
-
-def _copy_prefix(out, host):
-    _copy_rows(out, host)
 
 
 if __name__ == "__main__":

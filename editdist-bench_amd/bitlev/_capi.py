@@ -1,8 +1,14 @@
-"""ctypes binding of libbitlev_hip.so (C ABI declared in include/bitlev_hip.h).
+"""Binding of libbitlev_hip.so (C ABI declared in include/bitlev_hip.h)
+through the package's C extension: `bitlev._edpy` loads the library with
+dlopen and exposes each entry point as a function of the same name and
+argument order, called with the GIL released (csrc/edpy_hip.c).
 
-The library is loaded from this package directory. A missing or unloadable
-library raises BackendUnavailable: there is no CPU fallback anywhere in the
-backend, so nothing can silently compute distances without the GPU.
+Pointer arguments take an int address, None, or a buffer-exporting object
+(ctypes instances serve as out-parameters and as the bl_stripes /
+bl_*_params structures). The library is loaded from this package directory;
+a missing or unloadable library raises BackendUnavailable: there is no CPU
+fallback anywhere in the backend, so nothing can silently compute distances
+without the GPU.
 """
 from __future__ import annotations
 
@@ -10,62 +16,15 @@ import ctypes
 import os
 import threading
 
+from . import _edpy
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbitlev_hip.so")
 
 BL_OK, BL_EALLOC, BL_EINVAL, BL_EHIP, BL_ENODEV = 0, -1, -2, -3, -4
 
 _c = ctypes
-_vp, _i64, _i32, _int, _sz = _c.c_void_p, _c.c_int64, _c.c_int32, _c.c_int, _c.c_size_t
-
-# name -> argtypes (restype int unless listed in _RESTYPE)
-SIGNATURES = {
-    "bl_version": [],
-    "bl_last_error": [],
-    "bl_device_count": [],
-    "bl_set_device": [_int],
-    "bl_device_info": [_c.c_char_p, _sz, _c.POINTER(_int), _c.POINTER(_int), _c.POINTER(_int)],
-    "bl_malloc": [_c.POINTER(_vp), _sz],
-    "bl_free": [_vp],
-    "bl_host_malloc": [_c.POINTER(_vp), _sz],
-    "bl_host_free": [_vp],
-    "bl_memcpy_htod": [_vp, _vp, _sz, _vp],
-    "bl_memcpy_dtoh": [_vp, _vp, _sz, _vp],
-    "bl_memcpy_dtod": [_vp, _vp, _sz, _vp],
-    "bl_memset": [_vp, _int, _sz, _vp],
-    "bl_stream_create": [_c.POINTER(_vp)],
-    "bl_stream_destroy": [_vp],
-    "bl_stream_sync": [_vp],
-    "bl_device_sync": [],
-    "bl_event_create": [_c.POINTER(_vp)],
-    "bl_event_destroy": [_vp],
-    "bl_event_record": [_vp, _vp],
-    "bl_stream_wait_event": [_vp, _vp],
-    "bl_event_sync": [_vp],
-    "bl_event_query": [_vp],
-    "bl_event_elapsed_ms": [_vp, _vp, _c.POINTER(_c.c_float)],
-    "bl_graph_begin": [_vp],
-    "bl_graph_end": [_vp, _c.POINTER(_vp)],
-    "bl_graph_launch": [_vp, _vp],
-    "bl_graph_destroy": [_vp],
-    "bl_pairs_class": [_int, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _int, _int, _int, _vp],
-    "bl_stripes_layout": [_vp, _i64, _vp],
-    "bl_stripes_build": [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
-    "bl_gather_cols": [_vp, _vp, _vp, _i64, _i64, _int, _vp],
-    "bl_cdist_class": [_int, _vp, _vp, _vp, _i64, _vp, _vp, _int, _int, _vp],
-    "bl_long": [_int, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _int, _int, _int, _i64, _vp],
-    "bl_cdist_workspace_size": [_i64, _int],
-    "bl_cdist": [_vp, _vp, _i64, _vp, _int, _vp, _int, _int, _vp, _sz, _vp],
-    "bl_cdist_tuned": [_vp, _vp, _i64, _vp, _int, _vp, _int, _int, _vp, _vp, _sz, _vp, _vp],
-    "bl_cdist_matches": [_vp, _vp, _i64, _vp, _int, _int, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _sz,
-                         _vp, _vp],
-    "bl_pairs_workspace_size": [_i64, _i64],
-    "bl_pairs": [_vp, _vp, _vp, _vp, _i64, _i64, _vp, _int, _int, _int, _vp, _sz, _vp],
-    "bl_pairs_tuned": [_vp, _vp, _vp, _vp, _i64, _i64, _vp, _int, _int, _int, _vp, _vp, _sz, _vp,
-                       _vp],
-}
-_RESTYPE = {"bl_last_error": _c.c_char_p, "bl_cdist_workspace_size": _sz,
-            "bl_pairs_workspace_size": _sz, "bl_stripes_layout": _i64}
+_vp, _i64, _int = _c.c_void_p, _c.c_int64, _c.c_int
 
 
 class Stripes(ctypes.Structure):
@@ -97,7 +56,8 @@ _lock = threading.Lock()
 
 
 def lib():
-    """The loaded library (raises BackendUnavailable if it cannot be loaded)."""
+    """The extension module with the library bound (raises BackendUnavailable
+    if it cannot be loaded)."""
     global _lib
     if _lib is not None:
         return _lib
@@ -107,15 +67,16 @@ def lib():
                 raise BackendUnavailable(
                     f"{LIB_PATH} is not built; run `make -C editdist-bench_amd` (hipcc, gfx950)")
             try:
-                L = ctypes.CDLL(LIB_PATH)
+                _edpy.hip_open(LIB_PATH)
             except OSError as e:
-                raise BackendUnavailable(f"cannot load {LIB_PATH}: {e}") from e
-            for name, args in SIGNATURES.items():
-                f = getattr(L, name)
-                f.argtypes = args
-                f.restype = _RESTYPE.get(name, _int)
-            _lib = L
+                raise BackendUnavailable(str(e)) from e
+            _lib = _edpy
     return _lib
+
+
+def symbols() -> tuple:
+    """The entry points the extension binds."""
+    return _edpy.hip_symbols()
 
 
 def check(rc: int, what: str = "") -> int:

@@ -81,7 +81,7 @@ def _store(key, value):
 def _device_tag():
     name = ctypes.create_string_buffer(256)
     cu = ctypes.c_int()
-    call("bl_device_info", name, 256, ctypes.byref(cu), None, None)
+    call("bl_device_info", name, 256, cu, None, None)
     return f"{name.value.decode().strip()}/{cu.value}cu"
 
 
@@ -93,9 +93,10 @@ def _grid(space):
 
 def _time(fn, stream, reps=3):
     """Median ms of fn() on stream, measured with HIP events."""
-    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
-    call("bl_event_create", ctypes.byref(e0))
-    call("bl_event_create", ctypes.byref(e1))
+    h0, h1 = ctypes.c_void_p(), ctypes.c_void_p()
+    call("bl_event_create", h0)
+    call("bl_event_create", h1)
+    e0, e1 = h0.value, h1.value
     try:
         fn()
         call("bl_stream_sync", stream)
@@ -106,7 +107,7 @@ def _time(fn, stream, reps=3):
             call("bl_event_record", e1, stream)
             call("bl_event_sync", e1)
             v = ctypes.c_float()
-            call("bl_event_elapsed_ms", e0, e1, ctypes.byref(v))
+            call("bl_event_elapsed_ms", e0, e1, v)
             ts.append(v.value)
         return float(np.median(ts))
     finally:

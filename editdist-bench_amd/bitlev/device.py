@@ -50,10 +50,10 @@ class DevicePool:
                 return ptr, cap
         p = ctypes.c_void_p()
         try:
-            call("bl_malloc", ctypes.byref(p), cap)
+            call("bl_malloc", p, cap)
         except MemoryError:
             self.release_cached()
-            call("bl_malloc", ctypes.byref(p), cap)
+            call("bl_malloc", p, cap)
         with self._lock:
             self.live += cap
         return p.value, cap

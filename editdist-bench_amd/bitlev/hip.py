@@ -157,7 +157,7 @@ class ChoiceStripes:
 
     @property
     def ref(self):
-        return ctypes.byref(self.desc)
+        return self.desc
 
 
 def _idx_buffers(groups):
@@ -212,18 +212,26 @@ def pairs_into(A: Packed, B: Packed, out: DeviceArray, k: int = -1, trim: bool =
 
 
 def cdist_into(Q: Packed, C: Packed, out: DeviceArray, k: int = -1, stream=None,
-               plan: Plan | None = None) -> Plan:
-    """Launch the plain cdist kernels into the row-major (Q.n, C.n) `out`."""
+               plan: Plan | None = None, stripes: bool = False) -> Plan:
+    """Launch the plain cdist kernels into the row-major (Q.n, C.n) `out`:
+    one bl_cdist_class launch per query class over the packed choices, the
+    reference's launches one for one (bitlev/gpu.py:121-130); stripes=True
+    reads the choice set's resident stripes instead (bl_cdist_class_stripes)."""
     if plan is None:
         plan = cdist_plan(Q, C)
     if Q.n == 0 or C.n == 0:
         return plan
     dt = DTYPE_CODE[out.dtype]
-    st = C.stripes(stream)
+    st = C.stripes(stream) if stripes else None
     for nw, g, b in zip(CROSS_NWS, plan.groups, plan.bufs):
-        if b is not None:
-            call("bl_cdist_class", nw, Q.d_buf.ptr, Q.d_off.ptr, b.ptr, len(g), st.ref, out.ptr,
-                 dt, int(k), stream)
+        if b is None:
+            continue
+        if st is not None:
+            call("bl_cdist_class_stripes", nw, Q.d_buf.ptr, Q.d_off.ptr, b.ptr, len(g), st.ref,
+                 out.ptr, dt, int(k), stream)
+        else:
+            call("bl_cdist_class", nw, Q.d_buf.ptr, Q.d_off.ptr, b.ptr, len(g), C.d_buf.ptr,
+                 C.d_off.ptr, C.n, out.ptr, dt, int(k), stream)
     if plan.rest_buf is not None:
         call("bl_long", 0, Q.d_buf.ptr, Q.d_off.ptr, C.d_buf.ptr, C.d_off.ptr, plan.rest_buf.ptr,
              len(plan.rest), C.n, out.ptr, dt, 0, int(k), plan.max_len, stream)
@@ -263,7 +271,7 @@ def _cdist_params(params):
     p.reserved[0] = {"strided": 1, "swar": 2, "gap": 3, "count": 4}.get(params.get("mode"), 0)
     p.reserved[2] = int(params.get("qgroup", 0))
     p.reserved[1] = int(params.get("grab", 0))
-    return ctypes.byref(p)
+    return p
 
 
 def _pairs_params(params):
@@ -277,7 +285,7 @@ def _pairs_params(params):
     p.reserved[1] = 1 if params.get("mode") == "direct" else 0
     p.reserved[2] = int(params.get("grab", 0))
     p.reserved[3] = int(params.get("band", 0))
-    return ctypes.byref(p)
+    return p
 
 
 def pairs_batched_into(A: Packed, B: Packed, out: DeviceArray, k: int = -1, trim: bool = True,
@@ -383,7 +391,7 @@ class Stream:
 
     def __init__(self):
         p = ctypes.c_void_p()
-        call("bl_stream_create", ctypes.byref(p))
+        call("bl_stream_create", p)
         self.ptr = p.value
 
     def sync(self):
@@ -413,7 +421,7 @@ class Graph:
         try:
             fn()
         finally:
-            rc = lib().bl_graph_end(stream.ptr, ctypes.byref(ge))
+            rc = lib().bl_graph_end(stream.ptr, ge)
         from ._capi import check
         check(rc, "bl_graph_end")
         self.exec = ge.value

@@ -101,7 +101,7 @@ class _Pinned:
     def __init__(self, nbytes: int):
         self.nbytes = max(int(nbytes), 16)
         p = ctypes.c_void_p()
-        call("bl_host_malloc", ctypes.byref(p), self.nbytes)
+        call("bl_host_malloc", p, self.nbytes)
         self.ptr = p.value
         self.arr = np.ctypeslib.as_array((ctypes.c_uint8 * self.nbytes).from_address(self.ptr))
 
@@ -123,10 +123,10 @@ class _Slot:
         self.h = {}
         self.d = {}
         ev = ctypes.c_void_p()
-        call("bl_event_create", ctypes.byref(ev))
+        call("bl_event_create", ev)
         self.copied = ev.value
         ev = ctypes.c_void_p()
-        call("bl_event_create", ctypes.byref(ev))
+        call("bl_event_create", ev)
         self.done = ev.value
         self.chunk = None                  # (lo, hi) of the chunk whose results are in flight
 
@@ -240,7 +240,7 @@ class Stager:
         self.streams = []
         for _ in range(2):
             s = ctypes.c_void_p()
-            call("bl_stream_create", ctypes.byref(s))
+            call("bl_stream_create", s)
             self.streams.append(s.value)
         self.ws = _RetainingWorkspace()
 

@@ -93,10 +93,10 @@ class _Slot:
     def __init__(self, idx):
         self.idx = idx
         s = ctypes.c_void_p()
-        call("bl_stream_create", ctypes.byref(s))      # default priority
+        call("bl_stream_create", s)      # default priority
         self.stream = s.value
         e = ctypes.c_void_p()
-        call("bl_event_create", ctypes.byref(e))
+        call("bl_event_create", e)
         self.event = e.value
         self.workspace = hip.Workspace()
         self.busy = False

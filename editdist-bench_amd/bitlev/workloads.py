@@ -23,6 +23,10 @@ WORKLOADS = {
     "cdist-L64-k2": dict(gen=2, kind="cdist", nq=1_024, nc=1_000_000, k=2, dtype="uint8"),
     "pairs-uneven-1-256": dict(gen=3, kind="pairs", n=16_777_216, k=None, dtype="uint16"),
     "cdist-L200-256-k8": dict(gen=4, kind="cdist", nq=256, nc=4_194_304, k=8, dtype="uint8"),
+    # exact searches (no max_dist, int32): the shape of the reference's published
+    # GPU cdist figures, and one serving batch against a resident set
+    "cdist-L64-exact": dict(gen=2, kind="cdist", nq=1_024, nc=1_000_000, k=None, dtype="int32"),
+    "cdist-serving-exact": dict(gen=5, kind="cdist", nq=64, nc=1_000_000, k=None, dtype="int32"),
 }
 
 

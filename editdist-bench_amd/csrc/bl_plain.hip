@@ -6,7 +6,10 @@
  *                            Reference: ed_pairs_c{CAP}, csrc/gpu_kernels.cu:6-32.
  *   cdist_class_kernel<NW>   one workgroup per (query, choice block); the query's
  *                            256 x NW Peq table (and its low-32-row copy) in LDS;
- *                            lanes stride over choices.
+ *                            lanes stride over choices, read from the packed
+ *                            buffer (bl_cdist_class, the reference's arguments)
+ *                            or from a resident stripes layout
+ *                            (bl_cdist_class_stripes).
  *                            Reference: ed_cross_w{NW}, csrc/gpu_kernels.cu:36-69.
  *   long_kernel              one wave per long (pattern, text), anti-diagonal
  *                            wavefront over 64-row words; replaces the reference's
@@ -45,11 +48,29 @@ __global__ __launch_bounds__(THREADS) void pairs_class_kernel(
     store_out(out, i, d, dtype);
 }
 
-template <int NW>
+/* Choices straight from the packed buffer (the reference's layout): each lane
+ * reads its own string with aligned dword loads. */
+struct PackedChoices {
+    const u8 *buf;
+    const int64_t *off;
+    BL_HDM int len(int64_t c) const { return (int)(off[c + 1] - off[c]); }
+    BL_HDM ByteText text(int64_t c) const { return ByteText{buf + off[c]}; }
+};
+
+/* Choices from a resident stripes layout: the lanes of a wave read 256
+ * contiguous bytes per dword step. */
+struct StripeChoices {
+    const u32 *words;
+    const int64_t *gbase;
+    const int32_t *clen;
+    BL_HDM int len(int64_t c) const { return clen[c]; }
+    BL_HDM StripeText text(int64_t c) const { return StripeText{words + gbase[c >> 6] + (c & 63)}; }
+};
+
+template <int NW, typename Choices>
 __global__ __launch_bounds__(THREADS) void cdist_class_kernel(
     const u8 *__restrict__ bq, const int64_t *__restrict__ oq, const int32_t *__restrict__ qidx,
-    const u32 *__restrict__ words, const int64_t *__restrict__ gbase,
-    const int32_t *__restrict__ clen, int64_t nc, void *out, int dtype, int k)
+    const Choices ch, int64_t nc, void *out, int dtype, int k)
 {
     __shared__ u64 peq[256 * NW];
     __shared__ u32 lo32[256];               /* rows 0..31: the 32-row stage's table */
@@ -67,9 +88,8 @@ __global__ __launch_bounds__(THREADS) void cdist_class_kernel(
     __syncthreads();
     const int64_t stride = (int64_t)gridDim.y * THREADS;
     for (int64_t c = (int64_t)blockIdx.y * THREADS + threadIdx.x; c < nc; c += stride) {
-        const int n = clen[c];
-        const StripeText text{words + gbase[c >> 6] + (c & 63)};
-        const int d = !fits ? -1 : m == 0 ? empty_pattern(n, k) : peq_lane<NW, 0>(lo32, peq, m, text, n, k);
+        const int n = ch.len(c);
+        const int d = !fits ? -1 : m == 0 ? empty_pattern(n, k) : peq_lane<NW, 0>(lo32, peq, m, ch.text(c), n, k);
         store_out(out, q * nc + c, d, dtype);
     }
 }
@@ -169,49 +189,65 @@ int bl_pairs_class(int cap, const uint8_t *d_buf_a, const int64_t *d_off_a,
     return BL_OK;
 }
 
-int bl_cdist_class(int nw, const uint8_t *d_buf_q, const int64_t *d_off_q,
-                   const int32_t *d_qidx, int64_t nqc, const bl_stripes *choices, void *d_out,
-                   int dtype, int k, void *stream)
+}  /* extern "C" */
+
+namespace {
+
+template <typename Choices>
+int cdist_class_launch(const char *who, int nw, const uint8_t *d_buf_q, const int64_t *d_off_q,
+                       const int32_t *d_qidx, int64_t nqc, const Choices &ch, int64_t nc,
+                       void *d_out, int dtype, int k, void *stream)
 {
-    if (!choices) return fail(BL_EINVAL, "bl_cdist_class: null choices");
-    const int64_t nc = choices->n;
-    if (nqc < 0 || nc < 0) return fail(BL_EINVAL, "bl_cdist_class: negative count");
-    if (nqc == 0 || nc == 0) return BL_OK;
-    if (int rc = check_common(d_out, dtype, "bl_cdist_class")) return rc;
-    if (!d_buf_q || !d_off_q || !d_qidx || !choices->words || !choices->gbase || !choices->len)
-        return fail(BL_EINVAL, "bl_cdist_class: null input");
-    if (nqc > 0x7fffffff) return fail(BL_EINVAL, "bl_cdist_class: too many queries");
     k = clamp_k(k);
     int64_t gy = (nc + THREADS - 1) / THREADS;
     if (gy > 64) gy = 64;
     const dim3 grid((unsigned)nqc, (unsigned)gy);
     hipStream_t s = (hipStream_t)stream;
     switch (nw) {
-    case 1:
-        cdist_class_kernel<1><<<grid, THREADS, 0, s>>>(d_buf_q, d_off_q, d_qidx, choices->words,
-                                                       choices->gbase, choices->len, nc, d_out,
-                                                       dtype, k);
-        break;
-    case 2:
-        cdist_class_kernel<2><<<grid, THREADS, 0, s>>>(d_buf_q, d_off_q, d_qidx, choices->words,
-                                                       choices->gbase, choices->len, nc, d_out,
-                                                       dtype, k);
-        break;
-    case 4:
-        cdist_class_kernel<4><<<grid, THREADS, 0, s>>>(d_buf_q, d_off_q, d_qidx, choices->words,
-                                                       choices->gbase, choices->len, nc, d_out,
-                                                       dtype, k);
-        break;
-    case 8:
-        cdist_class_kernel<8><<<grid, THREADS, 0, s>>>(d_buf_q, d_off_q, d_qidx, choices->words,
-                                                       choices->gbase, choices->len, nc, d_out,
-                                                       dtype, k);
-        break;
+    case 1: cdist_class_kernel<1><<<grid, THREADS, 0, s>>>(d_buf_q, d_off_q, d_qidx, ch, nc, d_out, dtype, k); break;
+    case 2: cdist_class_kernel<2><<<grid, THREADS, 0, s>>>(d_buf_q, d_off_q, d_qidx, ch, nc, d_out, dtype, k); break;
+    case 4: cdist_class_kernel<4><<<grid, THREADS, 0, s>>>(d_buf_q, d_off_q, d_qidx, ch, nc, d_out, dtype, k); break;
+    case 8: cdist_class_kernel<8><<<grid, THREADS, 0, s>>>(d_buf_q, d_off_q, d_qidx, ch, nc, d_out, dtype, k); break;
     default:
-        return fail(BL_EINVAL, "bl_cdist_class: nw must be 1, 2, 4 or 8 (got %d)", nw);
+        return fail(BL_EINVAL, "%s: nw must be 1, 2, 4 or 8 (got %d)", who, nw);
     }
     BL_LAUNCHED("cdist_class_kernel");
     return BL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bl_cdist_class(int nw, const uint8_t *d_buf_q, const int64_t *d_off_q,
+                   const int32_t *d_qidx, int64_t nqc, const uint8_t *d_buf_c,
+                   const int64_t *d_off_c, int64_t nc, void *d_out, int dtype, int k, void *stream)
+{
+    if (nqc < 0 || nc < 0) return fail(BL_EINVAL, "bl_cdist_class: negative count");
+    if (nqc == 0 || nc == 0) return BL_OK;
+    if (int rc = check_common(d_out, dtype, "bl_cdist_class")) return rc;
+    if (!d_buf_q || !d_off_q || !d_qidx || !d_buf_c || !d_off_c)
+        return fail(BL_EINVAL, "bl_cdist_class: null input");
+    if (nqc > 0x7fffffff) return fail(BL_EINVAL, "bl_cdist_class: too many queries");
+    return cdist_class_launch("bl_cdist_class", nw, d_buf_q, d_off_q, d_qidx, nqc,
+                              PackedChoices{d_buf_c, d_off_c}, nc, d_out, dtype, k, stream);
+}
+
+int bl_cdist_class_stripes(int nw, const uint8_t *d_buf_q, const int64_t *d_off_q,
+                           const int32_t *d_qidx, int64_t nqc, const bl_stripes *choices,
+                           void *d_out, int dtype, int k, void *stream)
+{
+    if (!choices) return fail(BL_EINVAL, "bl_cdist_class_stripes: null choices");
+    const int64_t nc = choices->n;
+    if (nqc < 0 || nc < 0) return fail(BL_EINVAL, "bl_cdist_class_stripes: negative count");
+    if (nqc == 0 || nc == 0) return BL_OK;
+    if (int rc = check_common(d_out, dtype, "bl_cdist_class_stripes")) return rc;
+    if (!d_buf_q || !d_off_q || !d_qidx || !choices->words || !choices->gbase || !choices->len)
+        return fail(BL_EINVAL, "bl_cdist_class_stripes: null input");
+    if (nqc > 0x7fffffff) return fail(BL_EINVAL, "bl_cdist_class_stripes: too many queries");
+    return cdist_class_launch("bl_cdist_class_stripes", nw, d_buf_q, d_off_q, d_qidx, nqc,
+                              StripeChoices{choices->words, choices->gbase, choices->len}, nc,
+                              d_out, dtype, k, stream);
 }
 
 int bl_long(int is_pair, const uint8_t *d_buf_a, const int64_t *d_off_a,

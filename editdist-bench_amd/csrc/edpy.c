@@ -14,6 +14,10 @@
  *   pack_range(seq, lo, hi, buf, offsets) -> (total_bytes, max_len)
  *       items lo..hi-1 of a list or tuple, entirely without the GIL, so that
  *       several threads pack chunks of one batch at once
+ *
+ * The same module carries the C ABI of libbitlev_hip.so (edpy_hip.c):
+ * hip_open(path) binds the library, then bl_*(...) call its entry points with
+ * the GIL released.
  */
 #define PY_SSIZE_T_CLEAN
 #include <Python.h>
@@ -262,4 +266,15 @@ static PyMethodDef methods[] = {
 static struct PyModuleDef module = {PyModuleDef_HEAD_INIT, "_edpy", NULL, -1, methods,
                                     NULL, NULL, NULL, NULL};
 
-PyMODINIT_FUNC PyInit__edpy(void) { return PyModule_Create(&module); }
+/* The C ABI of libbitlev_hip.so, one function per entry point (edpy_hip.c). */
+extern PyMethodDef edpy_hip_methods[];
+
+PyMODINIT_FUNC PyInit__edpy(void)
+{
+    PyObject *m = PyModule_Create(&module);
+    if (m && PyModule_AddFunctions(m, edpy_hip_methods) < 0) {
+        Py_DECREF(m);
+        return NULL;
+    }
+    return m;
+}

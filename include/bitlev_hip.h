@@ -139,14 +139,25 @@ int bl_gather_cols(const void *d_src, const int64_t *d_inv, void *d_dst, int64_t
                    int dtype, void *stream);
 
 /* One cdist class: queries d_qidx[0..nqc) (each at most 64*nw bytes, nw in
- * {1, 2, 4, 8}) against all choices; out is the row-major nq x nc matrix of
- * the call (out[q * nc + c], nc = choices->n). One workgroup per (query,
- * choice block), the query's Peq table in LDS, lanes over consecutive choices
- * of the stripes. Replaces ed_cross_w{1,2,4,8} (csrc/gpu_kernels.cu:59-69) and
- * their launch in bitlev/gpu.py:121-130. */
+ * {1, 2, 4, 8}) against the nc packed choices (d_buf_c, d_off_c); out is the
+ * row-major nq x nc matrix of the call (out[q * nc + c], q the query's index
+ * in d_off_q). One workgroup per (query, choice block), the query's Peq table
+ * in LDS, lanes over consecutive choices, each reading its own string from
+ * the packed buffer. Same arguments as ed_cross_w{1,2,4,8}(buf_q, off_q, qidx,
+ * buf_c, off_c, nc, out, k, out8) (csrc/gpu_kernels.cu:59-65), with nw for
+ * the kernel name's suffix, nqc for the grid size and dtype for out8;
+ * replaces those kernels and their launch in bitlev/gpu.py:121-130. */
 int bl_cdist_class(int nw, const uint8_t *d_buf_q, const int64_t *d_off_q,
-                   const int32_t *d_qidx, int64_t nqc, const bl_stripes *choices, void *d_out,
-                   int dtype, int k, void *stream);
+                   const int32_t *d_qidx, int64_t nqc, const uint8_t *d_buf_c,
+                   const int64_t *d_off_c, int64_t nc, void *d_out, int dtype, int k, void *stream);
+
+/* bl_cdist_class over a resident choice set in the stripes layout
+ * (bl_stripes_build; nc = choices->n): the lanes of a wave read 256
+ * contiguous bytes per step instead of 64 scattered strings. No counterpart
+ * in the reference, which searches the packed buffer on every call. */
+int bl_cdist_class_stripes(int nw, const uint8_t *d_buf_q, const int64_t *d_off_q,
+                           const int32_t *d_qidx, int64_t nqc, const bl_stripes *choices,
+                           void *d_out, int dtype, int k, void *stream);
 
 /* Long strings on the GPU: pairs whose shorter string exceeds 256 bytes
  * (is_pair = 1: d_idx lists pair indices; trim applies) or cdist queries
@@ -195,9 +206,11 @@ typedef struct {
 } bl_pairs_params;
 
 /* Bytes of device workspace bl_cdist needs for nq queries of at most
- * max_qlen bytes: the work-queue counter and one Peq table per query
- * (256 x ceil(max_qlen / 64) words, rounded up to 1, 2, 4 or 8). 0 for a
- * negative argument. */
+ * max_qlen bytes: the work-queue counter (256 bytes), one Peq table per query
+ * (256 x ceil(max_qlen / 64) u64 words, the word count rounded up to 1, 2, 4
+ * or 8), then two 16-byte profiles per query (its byte-class and 2-gram
+ * profiles). Size workspaces from the return value, not from this list. 0
+ * for a negative argument. */
 size_t bl_cdist_workspace_size(int64_t nq, int max_qlen);
 
 /* All queries (each at most max_qlen <= 512 bytes) against all nc choices:

@@ -30,12 +30,43 @@ def declared_symbols():
 
 
 def test_library_exports_every_declared_symbol():
-    L = ctypes.CDLL(_capi.LIB_PATH)
+    L = ctypes.CDLL(_capi.LIB_PATH)          # only to read the export table
     names = declared_symbols()
     assert len(names) >= 25
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
-    assert set(names) == set(_capi.SIGNATURES), set(names) ^ set(_capi.SIGNATURES)
+    _capi.lib()
+    assert set(names) == set(_capi.symbols()), set(names) ^ set(_capi.symbols())
+
+
+def test_extension_binds_every_entry_point():
+    """The package reaches the library through its C extension (bitlev._edpy,
+    the reference's binding route, csrc/edpy.c:12, :231-239), one function per
+    entry point; no product module loads it with ctypes."""
+    mod = _capi.lib()
+    assert mod is _edpy
+    for n in declared_symbols():
+        assert callable(getattr(_edpy, n)), n
+    for f in glob.glob(os.path.join(PKG, "bitlev", "*.py")):
+        if os.path.basename(f) == "workloads.py":      # synthetic-input generator, not the ABI
+            continue
+        assert "CDLL" not in open(f).read(), f
+    with pytest.raises(TypeError):
+        _edpy.bl_free("not an address")
+    assert _edpy.bl_cdist_workspace_size(-1, 0) == 0
+    assert _edpy.bl_cdist_workspace_size(4, 64) == 256 + 4 * 2048 + 4 * 32
+
+
+def test_class_entry_points_reject_bad_arguments():
+    """Argument errors come back as BL_EINVAL before anything is launched (no
+    GPU needed: every case below fails the checks)."""
+    _capi.lib()
+    assert _edpy.bl_cdist_class(3, 1, 1, 1, 1, 1, 1, 1, 1, 0, -1, None) == -2     # nw not in {1,2,4,8}
+    assert _edpy.bl_cdist_class(1, 1, 1, 1, -1, 1, 1, 1, 1, 0, -1, None) == -2    # negative count
+    assert _edpy.bl_cdist_class(1, 1, 1, 1, 1, 1, 1, 1, 1, 7, -1, None) == -2     # bad dtype
+    assert _edpy.bl_cdist_class(1, 0, 1, 1, 1, 1, 1, 1, 1, 0, -1, None) == -2     # null input
+    assert _edpy.bl_cdist_class(1, 1, 1, 1, 0, 1, 1, 1, 1, 0, -1, None) == 0      # nothing to do
+    assert _edpy.bl_pairs_class(48, 1, 1, 1, 1, 1, 1, 1, 0, 1, -1, None) == -2    # cap
 
 
 def test_library_is_gfx950_code_object():
