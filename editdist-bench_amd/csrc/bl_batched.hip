@@ -29,6 +29,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <type_traits>
 
 #include "bl_device.h"
@@ -1812,6 +1813,64 @@ int bl_cdist(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq, const b
 {
     return bl_cdist_tuned(d_buf_q, d_off_q, nq, choices, max_qlen, d_out, dtype, k, nullptr, d_work,
                           work_bytes, nullptr, stream);
+}
+
+}  /* extern "C" */
+
+/* ------------------------------------------------ search without allocation */
+/* The workspace of bl_cdist_noalloc: one array per device, held until the
+ * process ends. In host code `scratch` only names the symbol; its device
+ * address comes from hipGetSymbolAddress, once per device, with that device
+ * current (noalloc_scratch). */
+static __device__ __align__(256) unsigned char scratch[BL_NOALLOC_SCRATCH_BYTES];
+
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static void *noalloc_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex noalloc_lock;
+
+/* The calling thread's current device's copy of `scratch` (its address
+ * resolved on the first call on that device). */
+static int noalloc_scratch(void **out)
+{
+    int dev = -1;
+    BL_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return fail(BL_EINVAL, "bl_cdist_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> g(noalloc_lock);
+    if (!noalloc_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(BL_EHIP, "bl_cdist_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                        hipGetErrorString(e));
+        noalloc_ptr[dev] = p;
+    }
+    *out = noalloc_ptr[dev];
+    return BL_OK;
+}
+
+extern "C" {
+
+int bl_cdist_noalloc(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
+                     const bl_stripes *choices, int max_qlen, void *d_out, int dtype, int k,
+                     const int32_t *d_kvec, const bl_cdist_params *prm, void *stream)
+{
+    if (int rc = cdist_check("bl_cdist_noalloc", d_buf_q, d_off_q, nq, choices, max_qlen)) return rc;
+    if (nq == 0 || choices->n == 0) return BL_OK;
+    if (dtype < BL_INT32 || dtype > BL_UINT8)
+        return fail(BL_EINVAL, "bl_cdist_noalloc: bad dtype %d", dtype);
+    if (!d_out) return fail(BL_EINVAL, "bl_cdist_noalloc: null pointer");
+    const size_t need = bl_cdist_workspace_size(nq, max_qlen);
+    if (need > (size_t)BL_NOALLOC_SCRATCH_BYTES)
+        return fail(BL_EINVAL,
+                    "bl_cdist_noalloc: %lld queries of up to %d bytes need %zu bytes of scratch, above "
+                    "the %d-byte static scratch (split the queries, or call bl_cdist_tuned)",
+                    (long long)nq, max_qlen, need, (int)BL_NOALLOC_SCRATCH_BYTES);
+    void *ws = nullptr;
+    if (int rc = noalloc_scratch(&ws)) return rc;
+    DenseSink sink{d_out, dtype};
+    return cdist_launch("bl_cdist_noalloc", d_buf_q, d_off_q, nq, choices, max_qlen, clamp_k_dev(k),
+                        d_kvec, ws, (size_t)BL_NOALLOC_SCRATCH_BYTES, prm, stream, sink, nullptr);
 }
 
 static int64_t pairs_chunk(int64_t n, const bl_pairs_params *prm)

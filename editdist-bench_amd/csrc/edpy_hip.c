@@ -37,7 +37,7 @@
     X(bl_graph_destroy) X(bl_pairs_class) X(bl_stripes_layout) X(bl_stripes_build)            \
     X(bl_gather_cols) X(bl_cdist_class) X(bl_cdist_class_stripes) X(bl_long)                  \
     X(bl_cdist_workspace_size) X(bl_cdist) X(bl_cdist_tuned) X(bl_cdist_matches)              \
-    X(bl_pairs_workspace_size) X(bl_pairs) X(bl_pairs_tuned)
+    X(bl_pairs_workspace_size) X(bl_pairs) X(bl_pairs_tuned) X(bl_cdist_noalloc)
 
 #define BL_DECL(n) __typeof__(&n) n;
 static struct { BL_FNS(BL_DECL) } H;
@@ -328,6 +328,16 @@ FN(bl_cdist_tuned)
     PARSE("O&O&LO&iO&iiO&O&O&O&O&", P(bq), P(oq), &nq, P(st), &mq, P(out), &dtype, &k, P(kvec),
           P(work), Z(wb), P(prm), P(s));
     RUN(H.bl_cdist_tuned(bq, oq, nq, st, mq, out, dtype, k, kvec, work, wb, prm, s));
+}
+
+FN(bl_cdist_noalloc)
+{
+    void *bq, *oq, *st, *out, *kvec, *prm, *s;
+    long long nq;
+    int mq, dtype, k;
+    PARSE("O&O&LO&iO&iiO&O&O&", P(bq), P(oq), &nq, P(st), &mq, P(out), &dtype, &k, P(kvec), P(prm),
+          P(s));
+    RUN(H.bl_cdist_noalloc(bq, oq, nq, st, mq, out, dtype, k, kvec, prm, s));
 }
 
 FN(bl_cdist_matches)

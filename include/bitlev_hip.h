@@ -236,6 +236,26 @@ int bl_cdist_tuned(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
                    const int32_t *d_kvec, void *d_work, size_t work_bytes,
                    const bl_cdist_params *params, void *stream);
 
+/* Device scratch held by bl_cdist_noalloc on each GPU it runs on (until the
+ * process ends): 8 MiB. */
+#define BL_NOALLOC_SCRATCH_BYTES (8 << 20)
+
+/* bl_cdist_tuned without any allocation: the workspace (work-queue counter,
+ * Peq tables, query profiles) is a static device array of
+ * BL_NOALLOC_SCRATCH_BYTES per GPU instead of a caller buffer or a
+ * stream-ordered allocation; the call makes no hipMalloc, hipFree or other
+ * allocating call. Returns BL_EINVAL when the queries need more than that
+ * (bl_cdist_workspace_size(nq, max_qlen) > BL_NOALLOC_SCRATCH_BYTES: about
+ * 4,000 queries of up to 64 bytes, 500 of up to 512). Every call on one GPU
+ * shares that GPU's array, so calls that overlap on one GPU (on different
+ * streams or from different threads) race; calls ordered on one stream do
+ * not. No counterpart in the reference: for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). */
+int bl_cdist_noalloc(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
+                     const bl_stripes *choices, int max_qlen, void *d_out, int dtype, int k,
+                     const int32_t *d_kvec, const bl_cdist_params *params, void *stream);
+
 /* Threshold match list instead of the dense matrix: every (query q, choice c)
  * with d(q, c) <= k is appended as d_q[i] = q, d_c[i] = c, d_d[i] = d, in no
  * particular order, for i < min(*d_count, capacity); *d_count receives the
