@@ -2,10 +2,12 @@
 
 For a resident workload it times every candidate launch shape of the batched
 pipeline on a sample of the batch (HIP events, median of a few runs), times
-the few fastest again on the whole batch (a sample's length mix and its
-tail of short tiles can rank close shapes differently), keeps the fastest and
-caches it per device and workload signature in a JSON file
-($BITLEV_TUNE_CACHE, default ~/.cache/bitlev/autotune.json).
+the fastest of each family (lane mode; direct, bucketed or band tiles) again
+on the whole batch (a sample's length mix and its tail of short tiles can
+rank shapes differently), walks the winner's parameters one at a time on the
+whole batch, keeps the fastest and caches it per device and workload
+signature in a JSON file ($BITLEV_TUNE_CACHE, default
+~/.cache/bitlev/autotune.json).
 
 cdist: lane mode (compacted lockstep passes, strided lanes, gap windows for
 k >= 0, or for k <= 3 the four-choice byte filter), persistent waves per CU,
@@ -115,27 +117,58 @@ def _time(fn, stream, reps=3):
         call("bl_event_destroy", e1)
 
 
-def _refine(res, full_fn, stream, refine, log, tag, wpc_values):
-    """Re-time the `refine` fastest sampled candidates on the whole batch, then
-    the winner at its other waves-per-CU values (a sample holds too few tiles
-    per wave to rank occupancy the way the whole batch does)."""
-    top = sorted(res, key=lambda r: r[0])[:max(1, refine)]
+def _family(p):
+    """The structural choice a launch shape makes (what a small sample ranks
+    least reliably): the lane mode for cdist; direct, bucketed or band tiles
+    for pairs."""
+    if p.get("mode") == "direct":
+        return "direct"
+    if "mode" in p:
+        return p["mode"]
+    return "band" if p.get("band") else "bucketed"
+
+
+def _refine(res, full_fn, stream, refine, log, tag, axes):
+    """Re-time on the whole batch the `refine` fastest sampled candidates of
+    every family (on a sample, tiles per wave are few and the end-of-batch
+    imbalance hides what band tiles or a lane mode save on the whole batch:
+    on the 16.8M uneven pairs a 1M-pair sample put band tiles 2% behind,
+    the whole batch 34% ahead), then walk the winner's parameters one axis
+    at a time on the whole batch, keeping each change that is faster."""
+    fams = {}
+    for ms, p in sorted(res, key=lambda r: r[0]):
+        lst = fams.setdefault(_family(p), [])
+        if len(lst) < max(1, refine):
+            lst.append(p)
     out = []
 
     def run(p):
+        for ms, q in out:
+            if q == p:
+                return ms
         ms = _time(lambda: full_fn(p), stream)
         out.append((ms, p))
         if log:
             log(f"{tag} full {p} {ms:.3f} ms")
+        return ms
 
-    for _, p in top:
-        run(p)
-    best = min(out, key=lambda r: r[0])[1]
-    for w in wpc_values:
-        q = dict(best, waves_per_cu=w)
-        if all(q != p for _, p in out):
-            run(q)
-    return min(out, key=lambda r: r[0])
+    for lst in fams.values():
+        for p in lst:
+            run(p)
+    best_ms, best = min(out, key=lambda r: r[0])
+    for axis, values in axes.items():
+        if axis == "band" and not best.get("band"):
+            continue
+        if axis in ("static", "gram") and best.get("mode") != "count":
+            continue
+        if axis not in best and axis not in ("band", "static", "gram"):
+            continue
+        for v in values:
+            q = dict(best, **{axis: v})
+            ms = run(q)
+            if ms < best_ms:
+                best_ms, best = ms, q
+    return best_ms, best
 
 
 def _length_sig(packed: hip.Packed):
@@ -180,7 +213,10 @@ def tune_cdist(Q: hip.Packed, C: hip.Packed, out: hip.DeviceArray, k: int = -1, 
             log(f"cdist {p} {ms:.3f} ms")
     if nq < Q.n and refine > 0:
         ms, best = _refine(res, lambda p: hip.cdist_batched_into(Q, C, out, k, stream, ws, p),
-                           stream, refine, log, "cdist", CDIST_SPACE["waves_per_cu"])
+                           stream, 1, log, "cdist",
+                           {"waves_per_cu": [8, 16, 32], "tile": [1024, 2048, 4096, 8192, 32768],
+                            "qgroup": [16, 32, 64], "static": [False, True],
+                            "gram": [False, True]})
     else:
         ms, best = min(res, key=lambda r: r[0])
     _store(sig, {"params": best, "ms": ms, "sample_queries": nq})
@@ -219,7 +255,10 @@ def tune_pairs(A: hip.Packed, B: hip.Packed, out: hip.DeviceArray, k: int = -1, 
                 run(dict(p, band=b))
     if n < A.n and refine > 0:
         ms, best = _refine(res, lambda p: hip.pairs_batched_into(A, B, out, k, trim, stream, ws, p),
-                           stream, refine, log, "pairs", PAIRS_SPACE["waves_per_cu"])
+                           stream, 2, log, "pairs",
+                           {"band": PAIRS_BAND_UNITS, "waves_per_cu": [8, 16, 32],
+                            "chunk": [16384, 65536, 262144], "bin_width": [8, 16, 32],
+                            "block": [64, 256]})
     else:
         ms, best = min(res, key=lambda r: r[0])
     _store(sig, {"params": best, "ms": ms, "sample_pairs": n})

@@ -271,6 +271,7 @@ def _cdist_params(params):
     p.reserved[0] = {"strided": 1, "swar": 2, "gap": 3, "count": 4}.get(params.get("mode"), 0)
     p.reserved[2] = int(params.get("qgroup", 0))
     p.reserved[1] = int(params.get("grab", 0))
+    p.reserved[3] = (0 if params.get("gram", True) else 1) | (0 if params.get("static", True) else 2)
     return p
 
 

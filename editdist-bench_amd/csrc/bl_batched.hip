@@ -41,6 +41,10 @@ namespace {
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
+typedef int v16i __attribute__((ext_vector_type(16)));   /* one 32x32 MFMA accumulator */
+typedef u32 v4u __attribute__((ext_vector_type(4)));     /* one 16-byte store */
+typedef int v4i __attribute__((ext_vector_type(4)));     /* one 16-byte MFMA operand */
+
 __host__ __device__ __forceinline__ int clamp_k_dev(int k)
 {
     return k < 0 ? -1 : (k > (1 << 30) ? (1 << 30) : k);
@@ -89,15 +93,28 @@ __device__ __forceinline__ long long wave_take(unsigned long long *counter, long
     return (long long)(((u64)hi << 32) | lo);
 }
 
-/* Work-queue tickets taken `grab` at a time: one atomic per grab (the
- * counter is device-wide, so every atomic leaves the XCD's L2), then the
- * wave walks its run. next / end are wave-uniform. */
+/* Work-queue tickets taken `grab` at a time. With `first_static` every wave's
+ * first run is its own (wave w takes tiles w * grab ..), so a launch's waves
+ * do not all queue on the counter at once (the 1,024 x 1e6 search at k = 2:
+ * 0.34 -> 0.30 ms; tiles of uneven cost can prefer the dynamic start: the
+ * 256 x 4.2e6 one 2.13 -> 2.36 ms); later runs come from the counter, one
+ * atomic per run. The counter is device-wide: same-address atomics are
+ * served one after another (about 16 ns apart here), which bounds how many
+ * tickets per microsecond a launch can take. next / end are wave-uniform. */
 struct Tickets {
     long long next = 0, end = 0;
+    bool started = false;
+    bool first_static = true;
     __device__ __forceinline__ long long get(unsigned long long *counter, int grab)
     {
         if (next >= end) {
-            next = wave_take(counter, grab);
+            const long long waves = ((long long)gridDim.x * blockDim.x) >> 6;
+            if (first_static && !started) {
+                started = true;
+                next = uni64((((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * grab);
+            } else {
+                next = (first_static ? waves * grab : 0) + wave_take(counter, grab);
+            }
             end = next + grab;
         }
         return next++;
@@ -694,6 +711,14 @@ struct CountLds {
 };
 constexpr int CD_GL = 4;        /* query groups up to this size keep their tables in LDS */
 
+/* One wave's LDS for the matrix-core tiles: the group's int8 operands (two
+ * 32-query row blocks) and one block's accumulator when it has open pairs,
+ * so that nothing of the tile path stays in registers while rescans run. */
+struct GramLds {
+    v4i a[2][64];
+    unsigned short m[16][64];   /* per 32 x 32 block of a window: lane l's open rows, bit r */
+};
+
 /* Persistent cdist over queries of at most 64*NWMAX bytes whose Peq tables
  * peq_build_kernel wrote to gpeq. Waves pull (choice block, query) tiles,
  * block-major, from the work-queue counter. */
@@ -826,12 +851,13 @@ __global__ __launch_bounds__(256) void cdist_batched_kernel(
  * (L1 >= gap, so L1 + gap > 2k already covers gap > k). Settled choices get
  * k + 1 (DENSE: the row at out + q * nc); the rest are pushed on the open
  * stack as output indices for the per-lane rescans (drain). */
-template <bool DENSE, bool PAIRS8, typename OutT, typename Drain>
+template <bool DENSE, bool PAIRS8, bool GRAM, typename OutT, typename Drain>
 __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t *__restrict__ clen,
                                               const Profile *__restrict__ ccnt, int64_t c0,
                                               int64_t c1, int64_t nc, int64_t q0, int gq, u64 okmask,
                                               int lm, int lk, bool uniform_k, const Profile &lp,
-                                              OutT *out, bool rows_aligned, int &ns, Drain &drain)
+                                              OutT *out, bool rows_aligned, bool rows16, int &ns,
+                                              Drain &drain, GramLds *gl)
 {
     const int lane = lane_id();
     const int k_uni = __builtin_amdgcn_readfirstlane(lk);
@@ -844,9 +870,137 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
     const u32 k2_uni = 2u * (u32)k_uni;
     const u32 fill_uni = out_k1<OutT>(k_uni) * (sizeof(OutT) == 1 ? 0x01010101u
                                                 : sizeof(OutT) == 2 ? 0x00010001u : 1u);
+    /* Matrix-core tiles (GRAM): with every query of the group and every choice
+     * of a window at most 64 bytes (class counts <= 64, doubled counts within
+     * int8) and k <= 63, the whole window is tested with the class-count L2
+     * bound (count_l2_over): per 32 queries x 32 choices one
+     * v_mfma_i32_32x32x32_i8 gives D = ||a - b||^2 over the 16 class counts,
+     * dims 0-15 carrying -2 a against b and dims 16-19 the norms in base-128
+     * digits, [-h_a, l_a, -128, 1] against [-128, 1, -h_b, l_b] (|x|^2 = 128 h
+     * + l, h <= 32); a pair stays open when D <= 2 k^2. Layout (checked on the device):
+     * A and B lane l hold row / column l % 32, dims 16 (l / 32) .. + 15; D lane
+     * l, register r holds column l % 32, row 8 (r / 4) + 4 (l / 32) + r % 4.
+     * The window's k + 1 fills are plain streaming stores. */
+    const int half = lane >> 5;
+    bool gram = false;
+    if (GRAM && fast_tile && k_uni <= 63 && wave_ballot(lane < gq && lm > 64) == 0) {
+        gram = true;
+        const u32 qn = lane < gq ? profile_dot(lp, lp) : 0u;
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++) {
+            const int j = 32 * rb + (lane & 31);
+            const u32 sx = (u32)__shfl((int)lp.x, j), sy = (u32)__shfl((int)lp.y, j);
+            const u32 sz = (u32)__shfl((int)lp.z, j), sw = (u32)__shfl((int)lp.w, j);
+            const u32 n2 = (u32)__shfl((int)qn, j);
+            if (half == 0) {
+                const u32 v[4] = {sx, sy, sz, sw};
+                u32 o[4];
+#pragma unroll
+                for (int d = 0; d < 4; d++) {
+                    o[d] = 0;
+#pragma unroll
+                    for (int b = 0; b < 32; b += 8) o[d] |= ((0u - 2u * ((v[d] >> b) & 255u)) & 255u) << b;
+                }
+                gl->a[rb][lane] = j < gq ? v4i{(int)o[0], (int)o[1], (int)o[2], (int)o[3]} : v4i{0, 0, 0, 0};
+            } else {
+                /* rows past the group: 128 * 100 > 2 * 63^2, never open */
+                const u32 h = j < gq ? n2 >> 7 : 100u, l = j < gq ? n2 & 127u : 0u;
+                gl->a[rb][lane] = v4i{(int)(((0u - h) & 255u) | (l << 8) | (0x80u << 16) | (1u << 24)),
+                                      0, 0, 0};
+            }
+        }
+        wave_lds_sync();
+    }
+    const int two_k2 = 2 * k_uni * k_uni;
+    auto gram_window = [&](int64_t w) {
+        if (DENSE) {
+            int64_t orow = q0 * nc;
+            if (rows16) {
+                /* 16 bytes per lane: 4 / sizeof(OutT) rows per store */
+                constexpr int LPR = 16 * (int)sizeof(OutT);      /* lanes per row segment */
+                constexpr int RPS = 64 / LPR;
+                const int rsub = lane / LPR;
+                const v4u f4 = {fill_uni, fill_uni, fill_uni, fill_uni};
+                for (int j = 0; j < gq; j += RPS) {
+                    if (j + rsub < gq) {
+                        v4u *p = (v4u *)__builtin_assume_aligned(
+                            out + orow + (int64_t)rsub * nc + w + (lane % LPR) * (16 / sizeof(OutT)), 16);
+                        __builtin_nontemporal_store(f4, p);
+                    }
+                    orow += RPS * nc;
+                }
+            } else {
+                for (int j = 0; j < gq; j++, orow += nc) {
+                    u32 *p = (u32 *)__builtin_assume_aligned(out + orow + w + 4 * lane, 4 * sizeof(OutT));
+#pragma unroll
+                    for (int x = 0; x < (int)sizeof(OutT); x++) __builtin_nontemporal_store(fill_uni, p + x);
+                }
+            }
+        }
+        /* the window's profiles all at once (eight 16-byte loads a lane in
+         * flight together), the group's operands from LDS */
+        Profile cpv[8];
+#pragma unroll
+        for (int cb = 0; cb < 8; cb++) cpv[cb] = ccnt[w + 32 * cb + (lane & 31)];
+        const v4i a0 = gl->a[0][lane], a1 = gl->a[1][lane];
+        const bool two = gq > 32;
+        /* blocks with open pairs: each lane's open rows as a 16-bit mask in
+         * LDS; pushes and rescans run after the window, once D and the
+         * profiles are dead */
+        u32 any = 0;
+#pragma unroll
+        for (int cb = 0; cb < 8; cb++) {
+            const Profile &cp = cpv[cb];
+            /* lanes 0-31: the counts; lanes 32-63: [-128, 1, -h_b, l_b] */
+            const u32 n2 = profile_dot(cp, cp);
+            const u32 dig = 0x80u | (1u << 8) | (((0u - (n2 >> 7)) & 255u) << 16) | ((n2 & 127u) << 24);
+            const v4i bop = half ? v4i{(int)dig, 0, 0, 0} : v4i{(int)cp.x, (int)cp.y, (int)cp.z, (int)cp.w};
+#pragma unroll
+            for (int rb = 0; rb < 2; rb++) {
+                if (rb == 1 && !two) continue;
+                const v16i D = __builtin_amdgcn_mfma_i32_32x32x32_i8(rb ? a1 : a0, bop, v16i{0}, 0, 0, 0);
+                int mn = D[0];
+#pragma unroll
+                for (int r = 1; r < 16; r++) mn = D[r] < mn ? D[r] : mn;
+                if (wave_ballot(mn <= two_k2) == 0) continue;
+                u32 bits = 0;
+#pragma unroll
+                for (int r = 0; r < 16; r++) bits |= (D[r] <= two_k2 ? 1u : 0u) << r;
+                gl->m[2 * cb + rb][lane] = (unsigned short)bits;
+                any |= 1u << (2 * cb + rb);
+            }
+        }
+        if (any == 0) return;
+        wave_lds_sync();
+        while (any) {
+            const int b = __builtin_ctz(any);
+            any &= any - 1;
+            const int cb = b >> 1, rb = b & 1;
+            const int64_t c = w + 32 * cb + (lane & 31);
+#pragma unroll 1
+            for (int r = 0; r < 16; r++) {
+                const u64 m = wave_ballot((gl->m[b][lane] >> r) & 1u);
+                if (m == 0) continue;
+                const int64_t q = q0 + 32 * rb + 8 * (r >> 2) + 4 * half + (r & 3);
+                if ((m >> lane) & 1u) open_stack[ns + rank_below(m)] = q * nc + c;
+                ns = uni(ns + __popcll(m));
+                if (ns >= 64) {
+                    wave_lds_sync();
+                    while (ns >= 64) drain(64);
+                }
+            }
+        }
+    };
     auto window = [&](int64_t w, auto fullc) {
         constexpr bool FULL = decltype(fullc)::value;
         const int64_t cl = w + 4 * lane;
+        if (GRAM && FULL && gram) {
+            const int4 l4 = *(const int4 *)(clen + cl);
+            if (wave_ballot(l4.x > 64 || l4.y > 64 || l4.z > 64 || l4.w > 64) == 0) {
+                gram_window(w);
+                return;
+            }
+        }
         int n[4];
         bool valid[4];
         Profile cp[4];
@@ -954,7 +1108,7 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
                     t = sad_u8(pq.y, cp[s].y, t);
                     t = sad_u8(pq.z, cp[s].z, t);
                     t = sad_u8(pq.w, cp[s].w, t);
-                    t += (mq <= 255) & (n[s] <= 255) ? gap : 0u;
+                    t += ((mq <= 255) & (n[s] <= 255)) ? gap : 0u;
                     om[s] = wave_ballot(valid[s] && gap <= (u32)kq && t <= k2);
                 }
             }
@@ -1066,7 +1220,7 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
             }
         }
     };
-    if (PAIRS8 && fast_tile)
+    if (PAIRS8 && fast_tile && !gram)
         for (; w + 512 <= c1; w += 512) window2(w);
     for (; w + 256 <= c1; w += 256) window(w, std::true_type());
     if (w < c1) window(w, std::false_type());
@@ -1083,18 +1237,21 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
  * escalation stack and through the wider stages 64 at a time. Empty,
  * too-long and exact queries run lane-strided. Needs the choices' and the
  * queries' class profiles. */
-template <int NWMAX, int GL, typename Sink>
-__global__ __launch_bounds__(256) void cdist_count_kernel(
+template <int NWMAX, int GL, typename Sink, bool GRAM = false>
+__global__ __launch_bounds__(256)
+/* the four-word instance sits at the 128-VGPR line: keep it at 4 waves/SIMD */
+__attribute__((amdgpu_waves_per_eu(NWMAX == 4 ? 4 : 1))) void cdist_count_kernel(
     const int64_t *__restrict__ oq, int64_t nq, const u64 *__restrict__ gpeq,
     const u32 *__restrict__ words, const int64_t *__restrict__ gbase,
     const int32_t *__restrict__ clen, const Profile *__restrict__ ccnt,
     const Profile *__restrict__ qcnt, const Profile *__restrict__ ccnt2,
     const Profile *__restrict__ qcnt2, int64_t nc, Sink sink,
     int k_all, const int32_t *__restrict__ kvec, int64_t tile, int grab, int qgroup,
-    unsigned long long *counter)
+    unsigned long long *counter, int first_static)
 {
     extern __shared__ u64 lds[];
     CountLds<GL> &L = ((CountLds<GL> *)lds)[threadIdx.x >> 6];
+    GramLds *gl = GRAM ? (GramLds *)((CountLds<GL> *)lds + 4) + (threadIdx.x >> 6) : nullptr;
     const int lane = lane_id();
     const int64_t nblk = (nc + tile - 1) / tile;
     const int64_t ng = (nq + qgroup - 1) / qgroup;
@@ -1179,6 +1336,17 @@ __global__ __launch_bounds__(256) void cdist_count_kernel(
             }
         }
         const u64 km = wave_ballot(keep);
+        if (GRAM) {
+            /* after the L2 prefilter the open pairs are mostly near matches,
+             * which the 32-row stage cannot settle: straight to the whole
+             * pattern (one 64-row word) */
+            if (keep) L.esc[ne + rank_below(km)] = idx;
+            ne = uni(ne + __popcll(km));
+            ns = uni(base);
+            wave_lds_sync();
+            while (ne >= 64) escalate(64);
+            return;
+        }
         if (keep) L.scan[nsc + rank_below(km)] = idx;
         nsc = uni(nsc + __popcll(km));
         ns = uni(base);
@@ -1186,6 +1354,7 @@ __global__ __launch_bounds__(256) void cdist_count_kernel(
         while (nsc >= 64) scan_drain(64);
     };
     Tickets tk;
+    tk.first_static = first_static != 0;
     for (;;) {
         const long long t = tk.get(counter, grab);
         if (t >= ntiles) break;
@@ -1204,7 +1373,22 @@ __global__ __launch_bounds__(256) void cdist_count_kernel(
         }
         const int64_t c0 = blk * tile;
         const int64_t c1 = c0 + tile < nc ? c0 + tile : nc;
-        for (int64_t q = q0; q < q1; q++) {
+        /* empty, too long and exact queries of the group (lane j looks at
+         * query q0 + j, so the loads run in parallel), then only those */
+        u64 special;
+        {
+            bool sp = false;
+            if (lane < (int)(q1 - q0)) {
+                const int64_t q = q0 + lane;
+                const int mq = (int)(oq[q + 1] - oq[q]);
+                const int kq = kvec ? clamp_k_dev(kvec[q]) : k_all;
+                sp = mq == 0 || mq > 64 * NWMAX || kq < 0;
+            }
+            special = wave_ballot(sp);
+        }
+        while (special) {
+            const int64_t q = q0 + __builtin_ctzll(special);
+            special &= special - 1;
             const int mq = (int)(oq[q + 1] - oq[q]);
             const int kq = kvec ? clamp_k_dev(kvec[q]) : k_all;
             const int64_t orow = q * nc;
@@ -1236,28 +1420,36 @@ __global__ __launch_bounds__(256) void cdist_count_kernel(
         const u64 okmask = wave_ballot(lane < gq && lm > 0 && lm <= 64 * NWMAX && lk >= 0);
         if (okmask) {
             if (!Sink::kDense) {
-                count_windows<false, NWMAX == 1>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
-                                     (u8 *)nullptr, false, ns, drain);
+                count_windows<false, NWMAX == 1, GRAM>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk,
+                                                       !kvec, lp, (u8 *)nullptr, false, false, ns, drain, gl);
             } else {
                 void *base = dense_out(sink);
                 const int dt = dense_dtype(sink);
                 const size_t esz = dt == DT_UINT8 ? 1 : dt == DT_UINT16 ? 2 : 4;
                 const bool aligned = (nc & 3) == 0 && ((uintptr_t)base & (4 * esz - 1)) == 0;
+                const bool a16 = ((nc * esz) & 15) == 0 && ((uintptr_t)base & 15) == 0;
                 if (dt == DT_UINT8)
-                    count_windows<true, NWMAX == 1>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
-                                        (u8 *)base, aligned, ns, drain);
+                    count_windows<true, NWMAX == 1, GRAM>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk,
+                                                          !kvec, lp, (u8 *)base, aligned, a16, ns, drain, gl);
                 else if (dt == DT_UINT16)
-                    count_windows<true, NWMAX == 1>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
-                                        (uint16_t *)base, aligned, ns, drain);
+                    count_windows<true, NWMAX == 1, GRAM>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk,
+                                                          !kvec, lp, (uint16_t *)base, aligned, a16, ns, drain, gl);
                 else
-                    count_windows<true, NWMAX == 1>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk, !kvec, lp,
-                                        (int32_t *)base, aligned, ns, drain);
+                    count_windows<true, NWMAX == 1, GRAM>(L.open, clen, ccnt, c0, c1, nc, q0, gq, okmask, lm, lk,
+                                                          !kvec, lp, (int32_t *)base, aligned, a16, ns, drain, gl);
             }
         }
-        while (ns > 0) drain(ns < 64 ? ns : 64);
-        while (nsc > 0) scan_drain(nsc < 64 ? nsc : 64);
-        while (ne > 0) escalate(ne < 64 ? ne : 64);
+        /* with the group's tables in LDS (GL > 0) the stacks empty before the
+         * next group; with global tables they carry over to the next tile */
+        if (GL > 0) {
+            while (ns > 0) drain(ns < 64 ? ns : 64);
+            while (nsc > 0) scan_drain(nsc < 64 ? nsc : 64);
+            while (ne > 0) escalate(ne < 64 ? ne : 64);
+        }
     }
+    while (ns > 0) drain(ns < 64 ? ns : 64);
+    while (nsc > 0) scan_drain(nsc < 64 ? nsc : 64);
+    while (ne > 0) escalate(ne < 64 ? ne : 64);
 }
 
 /* ======================================================================= pairs */
@@ -1651,6 +1843,20 @@ int launch_clear(unsigned long long *p, int n, hipStream_t s)
     return BL_OK;
 }
 
+int device_cus();
+
+/* Blocks of `kernel` the device holds at once. A persistent launch of more
+ * only adds waves that start when others have exited -- with the first run of
+ * tickets given to every wave statically, their tiles would then run last. */
+template <typename K>
+int64_t resident_blocks(K kernel, int block, size_t lds)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    return (int64_t)per_cu * device_cus();
+}
+
 int device_cus()
 {
     static int cus = 0;
@@ -1714,6 +1920,11 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
     const int qgroup = mode == 4 ? (prm->reserved[2] > 0 ? (prm->reserved[2] < 64 ? prm->reserved[2] : 64) : 16)
                                  : 1;
     const int nwmax = cdist_nwmax(max_qlen);
+    /* count mode over queries of at most 64 bytes: matrix-core tiles unless
+     * switched off (bit 0 of reserved[3]) */
+    const bool gram = mode == 4 && max_qlen <= 64 && !(prm && (prm->reserved[3] & 1));
+    /* count mode: each wave's first tile statically, unless switched off (bit 1) */
+    const int first_static = prm && (prm->reserved[3] & 2) ? 0 : 1;
     const size_t lds = sizeof(WaveLds) * 4;
     int64_t blocks = (int64_t)device_cus() * wpc / 4;
     const int64_t ntiles = ((nc + tile - 1) / tile) * ((nq + qgroup - 1) / qgroup);
@@ -1733,20 +1944,36 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
     if (rc == BL_OK && extra_clear) rc = launch_clear(extra_clear, 1, s);
     if (rc == BL_OK) {
         peq_build_kernel<<<(unsigned)nq, 256, 0, s>>>(d_buf_q, d_off_q, nwmax, gpeq, qcnt, qcnt2);
+        auto capped = [&](auto kernel, size_t lds_bytes) {
+            const int64_t cap = resident_blocks(kernel, 256, lds_bytes);
+            return (unsigned)(blocks < cap ? blocks : cap);
+        };
         auto go = [&](auto nwc) {
             constexpr int NW = decltype(nwc)::value;
-            if (mode == 4 && qgroup <= CD_GL)
-                cdist_count_kernel<NW, CD_GL><<<(unsigned)blocks, 256, sizeof(CountLds<CD_GL>) * 4, s>>>(
+            if (NW == 1 && mode == 4 && gram && qgroup > CD_GL) {
+                auto kern = cdist_count_kernel<1, 0, Sink, true>;
+                const size_t lb = (sizeof(CountLds<0>) + sizeof(GramLds)) * 4;
+                kern<<<capped(kern, lb), 256, lb, s>>>(
                     d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, ccnt2,
-                    qcnt2, nc, sink, k, d_kvec, tile, grab, qgroup, ctr);
-            else if (mode == 4)
-                cdist_count_kernel<NW, 0><<<(unsigned)blocks, 256, sizeof(CountLds<0>) * 4, s>>>(
+                    qcnt2, nc, sink, k, d_kvec, tile, grab, qgroup, ctr, first_static);
+            } else if (mode == 4 && qgroup <= CD_GL) {
+                auto kern = cdist_count_kernel<NW, CD_GL, Sink>;
+                const size_t lb = sizeof(CountLds<CD_GL>) * 4;
+                kern<<<capped(kern, lb), 256, lb, s>>>(
                     d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, ccnt2,
-                    qcnt2, nc, sink, k, d_kvec, tile, grab, qgroup, ctr);
-            else
-                cdist_batched_kernel<NW><<<(unsigned)blocks, 256, lds, s>>>(
+                    qcnt2, nc, sink, k, d_kvec, tile, grab, qgroup, ctr, first_static);
+            } else if (mode == 4) {
+                auto kern = cdist_count_kernel<NW, 0, Sink>;
+                const size_t lb = sizeof(CountLds<0>) * 4;
+                kern<<<capped(kern, lb), 256, lb, s>>>(
+                    d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, ccnt2,
+                    qcnt2, nc, sink, k, d_kvec, tile, grab, qgroup, ctr, first_static);
+            } else {
+                auto kern = cdist_batched_kernel<NW, Sink>;
+                kern<<<capped(kern, lds), 256, lds, s>>>(
                     d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, nc,
                     sink, k, d_kvec, tile, mode, grab, ctr);
+            }
         };
         switch (nwmax) {
         case 1: go(std::integral_constant<int, 1>()); break;
@@ -1987,6 +2214,8 @@ int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t
                                                                         tile_cnt);
             prefix_kernel<<<1, 1024, 0, s>>>(chunk_cnt, chunk_base, tile_cnt, tile_base, L.nchunks);
             pairs_scatter_kernel<<<cblocks, 1024, 0, s>>>(key, n, L.chunk, cursor, chunk_base, sorted);
+            const int64_t cap = resident_blocks(pairs_batched_kernel, wg, 0);
+            if (blocks > cap) blocks = cap;
             pairs_batched_kernel<<<(unsigned)blocks, wg, 0, s>>>(
                 d_buf_a, d_off_a, d_buf_b, d_off_b, trim, k, d_kvec, sorted, key, hist, chunk_base,
                 tile_base, L.nchunks, d_out, dtype, grab, ctr, scratch, slot);

@@ -1200,6 +1200,37 @@ BL_HD bool count2_over(const Profile &qc, const Profile &cc, int m, int n, int k
 
 /* The profile of s[0..n): 16 class counters of 16 bits in 8 dwords while
  * counting, then saturated to bytes. */
+/* A third bound over the byte-class counts, in the form the matrix cores
+ * evaluate. An insertion or a deletion moves the count vector by a unit
+ * vector, a substitution by the difference of two (L2 norm sqrt 2), so after
+ * d edits ||count(a) - count(b)||_2 <= sqrt(2) d, and d > k whenever
+ * ||count(a) - count(b)||^2 > 2 k^2. Weaker than the L1 bound (an L2 norm is
+ * at most the L1 norm), but ||a||^2 + ||b||^2 - 2 a.b needs only one int8
+ * matrix product for a block of query and choice profiles. Saturated counts
+ * only shrink the norm (clamping is 1-Lipschitz), so it holds at any length;
+ * the kernels use it for strings of at most 64 bytes, whose counts and
+ * doubled counts fit int8. */
+BL_HD u32 profile_dot(const Profile &a, const Profile &b)
+{
+    u32 s = 0;
+    const u32 A[4] = {a.x, a.y, a.z, a.w}, B[4] = {b.x, b.y, b.z, b.w};
+    for (int j = 0; j < 4; j++) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        s = __builtin_amdgcn_udot4(A[j], B[j], s, false);
+#else
+        for (int i = 0; i < 32; i += 8) s += ((A[j] >> i) & 255u) * ((B[j] >> i) & 255u);
+#endif
+    }
+    return s;
+}
+
+BL_HD bool count_l2_over(const Profile &qc, const Profile &cc, int k)
+{
+    const int64_t d2 = (int64_t)profile_dot(qc, qc) + (int64_t)profile_dot(cc, cc) -
+                       2 * (int64_t)profile_dot(qc, cc);
+    return d2 > 2 * (int64_t)k * (int64_t)k;    /* k <= 2^30: fits 64 bits */
+}
+
 BL_HD Profile class_profile(const u8 *s, int n)
 {
     u32 c16[8] = {0, 0, 0, 0, 0, 0, 0, 0};

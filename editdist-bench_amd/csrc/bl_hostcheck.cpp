@@ -395,6 +395,12 @@ int blh_count_over(const u8 *a, int m, const u8 *b, int n, int k)
     return count_over(class_profile(a, m), class_profile(b, n), m, n, k) ? 1 : 0;
 }
 
+/* The class-count L2 test of the search kernels' matrix-core filter. */
+int blh_count_l2_over(const u8 *a, int m, const u8 *b, int n, int k)
+{
+    return count_l2_over(class_profile(a, m), class_profile(b, n), k) ? 1 : 0;
+}
+
 void blh_class2_profile(const u8 *s, int n, uint32_t *out)
 {
     const Profile p = class2_profile(s, n);

@@ -188,7 +188,14 @@ typedef struct {
                               most choices, the rest are rescanned with per-lane
                               queries;
                               [1]: tiles per work-queue ticket (default 1);
-                              [2]: count mode: queries per tile (default 16, at most 64) */
+                              [2]: count mode: queries per tile (default 16, at most 64);
+                              [3]: count mode options: bit 0 switches off the
+                              matrix-core tiles (queries of at most 64 bytes and
+                              windows of choices of at most 64 bytes are otherwise
+                              tested 32 x 32 pairs at a time with an int8 MFMA over
+                              the class counts, a squared-L2 bound); bit 1 makes
+                              every tile come from the work queue (by default each
+                              wave's first tile is assigned statically) */
 } bl_cdist_params;
 
 typedef struct {

@@ -468,3 +468,76 @@ def test_untuned_thresholded_pairwise_default_shape():
         for dt in (np.int32, np.uint8, np.uint16):
             got = hip.pairwise(A, B, max_dist=k, dtype=dt)
             np.testing.assert_array_equal(got, O.saturate(want, dt), err_msg=f"k={k} {dt}")
+
+
+def _short_sets(seed, nq, nc, maxlen=64):
+    """Queries and choices of at most `maxlen` bytes over several alphabets
+    (26 letters, DNA, two letters, all bytes), with near copies of queries,
+    anagrams, runs and empty strings among the choices."""
+    rng = np.random.default_rng(seed)
+    alphas = [np.arange(97, 123), np.frombuffer(b"ACGT", np.uint8), np.array([97, 98]),
+              np.arange(256)]
+    Q = []
+    for i in range(nq):
+        a = alphas[i % 4]
+        Q.append(bytes(rng.choice(a, int(rng.integers(1, maxlen + 1))).astype(np.uint8)))
+    C = []
+    for i in range(nc):
+        r = rng.random()
+        q = bytearray(Q[int(rng.integers(nq))])
+        if r < 0.3:
+            for _ in range(int(rng.integers(0, 6))):
+                op = int(rng.integers(0, 3))
+                pos = int(rng.integers(0, len(q) + 1))
+                if op == 0 and pos < len(q):
+                    q[pos] = int(rng.integers(0, 256))
+                elif op == 1 and len(q) < maxlen:
+                    q.insert(pos, int(rng.integers(0, 256)))
+                elif pos < len(q):
+                    del q[pos]
+            C.append(bytes(q))
+        elif r < 0.4:
+            x = np.frombuffer(bytes(q), np.uint8).copy()
+            rng.shuffle(x)
+            C.append(x.tobytes())
+        elif r < 0.45:
+            C.append(bytes([int(rng.integers(0, 256))]) * int(rng.integers(0, maxlen + 1)))
+        else:
+            a = alphas[int(rng.integers(4))]
+            C.append(bytes(rng.choice(a, int(rng.integers(0, maxlen + 1))).astype(np.uint8)))
+    return O.pack(Q), O.pack(C)
+
+
+@pytest.mark.parametrize("nc", [5120, 5123])
+def test_count_kernel_matrix_core_tiles(nc):
+    """The count kernel's int8-MFMA tiles (queries and choice windows of at
+    most 64 bytes: the class-count L2 bound on 32 x 32 pairs at a time) for
+    thresholds up to and past where the bound can prune (k = 63, 64, 100),
+    partial query groups of 5, 40 and 64, every output dtype with 16-byte
+    rows and without, and the match list; the same launch with the tiles
+    switched off gives the same bytes, and both equal the oracle. Every
+    fifth choice window holds a 65-byte choice and falls back to the
+    byte-class path."""
+    pq, pc = _short_sets(31, 69, nc)
+    C = O.unpack(*pc)
+    for w in range(0, nc, 1280):                 # some windows not eligible for the tiles
+        C[min(w + 7, nc - 1)] = b"z" * 65
+    pc = O.pack(C)
+    Q, Cp = hip.Packed(*pq), hip.Packed(*pc)
+    exact = O.cross(pq, pc, threads=8)
+    for k in (0, 1, 2, 3, 8, 30, 63, 64, 100):
+        want = np.minimum(exact, k + 1)
+        for g in (5, 40, 64):
+            for gram in (True, False):
+                p = {"mode": "count", "tile": 1024, "qgroup": g, "gram": gram}
+                for dt in (np.uint8, np.uint16, np.int32):
+                    out = hip.DeviceArray.empty((Q.n, Cp.n), dt)
+                    hip.cdist_batched_into(Q, Cp, out, k, None, None, p)
+                    np.testing.assert_array_equal(out.get(), O.saturate(want, dt),
+                                                  err_msg=f"nc={nc} k={k} {p} {dt}")
+            if k <= 30:
+                p = {"mode": "count", "tile": 1024, "qgroup": g}
+                qi, ci, di = hip.cdist_matches_into(Q, Cp, k, params=p)
+                got = sorted(zip(qi.get().tolist(), ci.get().tolist(), di.get().tolist()))
+                wq, wc = np.nonzero(exact <= k)
+                assert got == sorted(zip(wq.tolist(), wc.tolist(), exact[wq, wc].tolist())), (k, p)

@@ -584,3 +584,73 @@ def test_two_gram_profiles_and_bound(hc):
     # past 256 bytes the gap term is left out
     a, b = b"A" * 300, b"A" * 280
     assert not over(a, b, 0)
+
+
+def test_count_l2_bound(hc):
+    """The matrix-core filter's bound: d > k whenever the squared L2 distance
+    of the two byte-class count vectors exceeds 2 k^2. Every pair it prunes
+    (random, edited, anagram pairs over several alphabets, 8 thresholds) has
+    an oracle distance above k; it is reached on pairs that make it tight (a
+    run of one letter against a run of another: d = 10, squared distance
+    200 = 2 * 10^2), so a factor 4 for 2 or >= for > fails; one cross-class
+    substitution (squared distance 2) is pruned at k = 0 only; it never prunes
+    more than the L1 bound does."""
+    hc.blh_count_l2_over.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                     ctypes.c_int]
+    hc.blh_count_over.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                  ctypes.c_int]
+
+    def over(a, b, k):
+        return bool(hc.blh_count_l2_over(a, len(a), b, len(b), k))
+
+    def over_l1(a, b, k):
+        return bool(hc.blh_count_over(a, len(a), b, len(b), k))
+
+    rng = np.random.default_rng(79)
+    A, B = [], []
+    for _ in range(4000):
+        alpha = int(rng.choice([2, 4, 16, 26, 256]))
+        m = int(rng.integers(0, 80))
+        a = bytes(rng.integers(0, alpha, m).astype(np.uint8) + (97 if alpha == 26 else 0))
+        r = rng.random()
+        if r < 0.5:
+            b = bytearray(a)
+            for _ in range(int(rng.integers(0, 10))):
+                op = rng.integers(0, 3)
+                pos = int(rng.integers(0, len(b) + 1))
+                ch = int(rng.integers(0, alpha)) + (97 if alpha == 26 else 0)
+                if op == 0 and pos < len(b):
+                    b[pos] = ch
+                elif op == 1:
+                    b.insert(pos, ch)
+                elif pos < len(b):
+                    del b[pos]
+            b = bytes(b)
+        elif r < 0.7:
+            x = np.frombuffer(a, np.uint8).copy()
+            rng.shuffle(x)
+            b = x.tobytes()
+        else:
+            b = bytes(rng.integers(0, alpha, int(rng.integers(0, 80))).astype(np.uint8)
+                      + (97 if alpha == 26 else 0))
+        A.append(a)
+        B.append(b)
+    d = O.pairs(O.pack(A), O.pack(B), threads=8)
+    pruned = 0
+    for a, b, dist in zip(A, B, d):
+        for k in (0, 1, 2, 3, 5, 8, 13, 30):
+            if over(a, b, k):
+                assert dist > k, (a, b, k, dist)
+                # the L2 norm is at most the L1 norm: the L1 bound prunes it too
+                assert over_l1(a, b, k), (a, b, k)
+                pruned += 1
+    assert pruned > 3000
+    a, b = b"a" * 10, b"b" * 10                  # classes 1 and 2: squared distance 200
+    assert int(O.pairs(O.pack([a]), O.pack([b]))[0]) == 10
+    assert over(a, b, 9) and not over(a, b, 10)
+    a, b = b"abcdefgh", b"abcdxfgh"              # one cross-class substitution: squared distance 2
+    assert over(a, b, 0) and not over(a, b, 1)
+    a, b = b"abcdefgh", b"abcdefghq"             # one insertion: squared distance 1
+    assert over(a, b, 0) and not over(a, b, 1)
+    assert not over(b"ab" * 20, b"ba" * 20, 0)   # same counts: the bound says nothing
+    assert not over(b"a" * 64, b"b" * 64, 64) and over(b"a" * 64, b"b" * 64, 63)
