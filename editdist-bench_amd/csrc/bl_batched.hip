@@ -93,28 +93,47 @@ __device__ __forceinline__ long long wave_take(unsigned long long *counter, long
     return (long long)(((u64)hi << 32) | lo);
 }
 
-/* Work-queue tickets taken `grab` at a time. With `first_static` every wave's
- * first run is its own (wave w takes tiles w * grab ..), so a launch's waves
- * do not all queue on the counter at once (the 1,024 x 1e6 search at k = 2:
- * 0.34 -> 0.30 ms; tiles of uneven cost can prefer the dynamic start: the
- * 256 x 4.2e6 one 2.13 -> 2.36 ms); later runs come from the counter, one
- * atomic per run. The counter is device-wide: same-address atomics are
- * served one after another (about 16 ns apart here), which bounds how many
- * tickets per microsecond a launch can take. next / end are wave-uniform. */
+/* Work queues: QUEUES counters on separate 256-byte lines (QUEUE_BYTES of
+ * workspace). Same-address device atomics are served one after another
+ * (about 16 ns apart here), so one counter shared by 4,096 waves is a queue
+ * in itself -- at the launch, at the end (every wave's last, failed
+ * ticket) and for workloads of many small tiles. Wave w draws from queue
+ * w % QUEUES only, and each queue owns an interleaved share of the tiles, so
+ * every queue is balanced dynamically among its waves while each address
+ * sees 1/QUEUES of the traffic. */
+constexpr int QUEUES = 16;
+constexpr int QUEUE_STRIDE = 32;                        /* u64 words: 256 bytes */
+constexpr int QUEUE_BYTES = QUEUES * QUEUE_STRIDE * 8;
+
+/* Tickets in runs of `grab` tiles; run r covers tiles r * grab .. + grab - 1.
+ * With `first_static` every wave's first run is its own (run w), so a
+ * launch's waves do not all queue at once (the 1,024 x 1e6 search at k = 2:
+ * 0.34 -> 0.30 ms with one counter); tiles of uneven cost can prefer the
+ * dynamic start. Later runs: queue j hands out runs S + j + Q i (S = the
+ * static round, Q = min(QUEUES, waves), so that every queue has a wave), one
+ * atomic per run. A wave takes its runs in increasing order. next / end are
+ * wave-uniform. */
 struct Tickets {
     long long next = 0, end = 0;
     bool started = false;
     bool first_static = true;
-    __device__ __forceinline__ long long get(unsigned long long *counter, int grab)
+    __device__ __forceinline__ long long get(unsigned long long *queues, int grab)
     {
         if (next >= end) {
             const long long waves = ((long long)gridDim.x * blockDim.x) >> 6;
+            const long long wid = uni64(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+            long long run;
             if (first_static && !started) {
                 started = true;
-                next = uni64((((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * grab);
+                run = wid;
             } else {
-                next = (first_static ? waves * grab : 0) + wave_take(counter, grab);
+                /* every queue in use needs a wave: a launch of fewer than
+                 * QUEUES waves uses as many queues as it has waves */
+                const long long nqs = waves < QUEUES ? waves : QUEUES;
+                const int j = (int)(wid % nqs);
+                run = (first_static ? waves : 0) + j + nqs * wave_take(queues + j * QUEUE_STRIDE, 1);
             }
+            next = run * grab;
             end = next + grab;
         }
         return next++;
@@ -1882,11 +1901,11 @@ static int cdist_nwmax(int max_qlen)
     return words <= 1 ? 1 : words <= 2 ? 2 : words <= 4 ? 4 : 8;
 }
 
-/* Workspace: the work-queue counter (256 bytes), the queries' Peq tables,
+/* Workspace: the work queues (QUEUE_BYTES), the queries' Peq tables,
  * then one 16-byte class profile and one 2-gram profile per query. */
 static size_t cdist_peq_end(int64_t nq, int max_qlen)
 {
-    return 256 + (size_t)nq * 256 * cdist_nwmax(max_qlen) * sizeof(u64);
+    return QUEUE_BYTES + (size_t)nq * 256 * cdist_nwmax(max_qlen) * sizeof(u64);
 }
 
 size_t bl_cdist_workspace_size(int64_t nq, int max_qlen)
@@ -1934,13 +1953,13 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
     unsigned char *ws = (unsigned char *)d_work;
     if (!ws) BL_HIP(hipMallocAsync((void **)&ws, need_ws, s));
     unsigned long long *ctr = (unsigned long long *)ws;
-    u64 *gpeq = (u64 *)(ws + 256);
+    u64 *gpeq = (u64 *)(ws + QUEUE_BYTES);
     /* query class profiles after the Peq tables, when the choices carry theirs */
     const Profile *ccnt = (const Profile *)choices->counts;
     Profile *qcnt = ccnt ? (Profile *)(ws + cdist_peq_end(nq, max_qlen)) : nullptr;
     const Profile *ccnt2 = ccnt ? (const Profile *)choices->counts2 : nullptr;
     Profile *qcnt2 = ccnt2 ? (Profile *)(ws + cdist_peq_end(nq, max_qlen)) + nq : nullptr;
-    int rc = launch_clear(ctr, 1, s);
+    int rc = launch_clear(ctr, QUEUE_BYTES / 8, s);
     if (rc == BL_OK && extra_clear) rc = launch_clear(extra_clear, 1, s);
     if (rc == BL_OK) {
         peq_build_kernel<<<(unsigned)nq, 256, 0, s>>>(d_buf_q, d_off_q, nwmax, gpeq, qcnt, qcnt2);
@@ -2125,7 +2144,7 @@ static PairsLayout pairs_layout(int64_t n, int64_t max_len, const bl_pairs_param
     const int64_t hc = header_chunks > L.nchunks ? header_chunks : L.nchunks;
     /* head: counter | hist | cursor | chunk_cnt | tile_cnt | chunk_base | tile_base */
     const size_t nk = (size_t)hc * PKEYS;
-    L.head = (64 + (2 * nk + 4 * ((size_t)hc + 1)) * 8 + 255) & ~(size_t)255;
+    L.head = (QUEUE_BYTES + (2 * nk + 4 * ((size_t)hc + 1)) * 8 + 255) & ~(size_t)255;
     L.key = L.head;
     L.sorted = (L.key + (size_t)n * sizeof(unsigned short) + 255) & ~(size_t)255;
     L.scratch = (L.sorted + (size_t)n * sizeof(int32_t) + 255) & ~(size_t)255;
@@ -2174,7 +2193,7 @@ int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t
     if (!w) BL_HIP(hipMallocAsync((void **)&w, L.total, s));
     const size_t nk = (size_t)L.nchunks * PKEYS;
     unsigned long long *ctr = (unsigned long long *)w;
-    unsigned long long *hist = (unsigned long long *)(w + 64);
+    unsigned long long *hist = (unsigned long long *)(w + QUEUE_BYTES);
     unsigned long long *cursor = hist + nk;
     unsigned long long *chunk_cnt = cursor + nk;
     unsigned long long *tile_cnt = chunk_cnt + (L.nchunks + 1);
@@ -2205,7 +2224,7 @@ int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t
         if (e != hipSuccess) rc = fail(BL_EHIP, "pairs_direct_kernel launch: %s", hipGetErrorString(e));
     } else {
         /* the counter and the histograms start at zero */
-        rc = launch_clear(ctr, (int)(8 + nk), s);
+        rc = launch_clear(ctr, (int)(QUEUE_BYTES / 8 + nk), s);
         const unsigned cblocks = (unsigned)((n + 1023) / 1024);  /* 1024 pairs per block */
         if (rc == BL_OK) {
             pairs_classify_kernel<<<cblocks, 1024, 0, s>>>(d_off_a, d_off_b, n, d_out, dtype, k, d_kvec,

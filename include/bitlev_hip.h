@@ -213,7 +213,7 @@ typedef struct {
 } bl_pairs_params;
 
 /* Bytes of device workspace bl_cdist needs for nq queries of at most
- * max_qlen bytes: the work-queue counter (256 bytes), one Peq table per query
+ * max_qlen bytes: the work-queue counters (4 KiB), one Peq table per query
  * (256 x ceil(max_qlen / 64) u64 words, the word count rounded up to 1, 2, 4
  * or 8), then two 16-byte profiles per query (its byte-class and 2-gram
  * profiles). Size workspaces from the return value, not from this list. 0
