@@ -413,6 +413,7 @@ def cpu_leg(W_, args, dominant_s):
         la, lb = np.diff(W_.hx[1]).astype(np.float64), np.diff(W_.hy[1]).astype(np.float64)
         extra["cells_reference_rule"] = float((la * lb).sum())
 
+    cpu_simd = cpu_simd_leg(W_, run_cpu, threads, rows1)
     parity = {f: {"checked": pairs_sample, "equal": equal[f],
                   **({"first_mismatch": first_bad[f]} if f in first_bad else {})} for f in forms}
     parity["against"] = "reference C core (csrc/myers.c)" if use_ref else "oracle"
@@ -425,12 +426,75 @@ def cpu_leg(W_, args, dominant_s):
         "single_thread": round(pairs_one / t_one / 1e6, 3),
         "host_cpus": cpus,
         "build": "-O3 -std=c11 (portable, no -march=native)",
+        "simd": cpu_simd,
     }
     for f in forms:
         if not equal[f]:
             _log(f"WARNING: {W_.name} {f}: GPU results differ from the CPU reference at "
                  f"{first_bad[f]}")
     return parity, cpu, extra
+
+
+def cpu_simd_leg(W_, run_ref, threads, rows1):
+    """The package's CPU engine (the x86 AVX2 kernels behind method="simd")
+    on the same inputs: one thread on the reference's one-thread prefix, all
+    cores (a thread pool over query rows or pair chunks; the calls release
+    the GIL) on a bounded sample, each checked against the reference core."""
+    from concurrent.futures import ThreadPoolExecutor
+    from bitlev import _edpy
+    from bitlev import workloads as W
+    if not _edpy.simd_available():
+        return None
+    k = W_.k
+
+    def simd(a, b, out):
+        if W_.kind == "cdist":
+            _edpy.cross_packed(a[0], a[1], b[0], b[1], out, k, True)
+        else:
+            _edpy.pairs_packed(a[0], a[1], b[0], b[1], out, True, k, True)
+
+    if W_.kind == "cdist":
+        nc = W_.wl["nc"]
+        one = W.head(W_.hx, rows1)
+        many_rows = min(W_.wl["nq"], max(rows1, 4 * threads if nc > 1_000_000 else 8 * threads))
+        many = W.head(W_.hx, many_rows)
+        out1 = np.empty((rows1, nc), np.int32)
+        outm = np.empty((many_rows, nc), np.int32)
+        t = time.perf_counter()
+        simd(one, W_.hy, out1)
+        t1 = time.perf_counter() - t
+        step = max(1, many_rows // threads)
+        spans = [(lo, min(many_rows, lo + step)) for lo in range(0, many_rows, step)]
+        t = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(lambda sp: simd(span(many, sp[0], sp[1]), W_.hy, outm[sp[0]:sp[1]]), spans))
+        tm = time.perf_counter() - t
+        equal = bool(np.array_equal(outm, run_ref(many, threads)))
+        n1, nm = rows1 * nc, many_rows * nc
+        txt = f"first {rows1} queries (1 thread), first {many_rows} (all cores) x all {nc:,} choices"
+    else:
+        n = W_.wl["n"]
+        n1 = min(n, rows1)
+        nm = min(n, max(n1, 1 << 21))
+        out1, outm = np.empty(n1, np.int32), np.empty(nm, np.int32)
+        a1, b1 = W.head(W_.hx, n1), W.head(W_.hy, n1)
+        t = time.perf_counter()
+        simd(a1, b1, out1)
+        t1 = time.perf_counter() - t
+        am, bm = W.head(W_.hx, nm), W.head(W_.hy, nm)
+        step = max(1, nm // (4 * threads))
+        spans = [(lo, min(nm, lo + step)) for lo in range(0, nm, step)]
+        t = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(lambda sp: simd(span(am, sp[0], sp[1]), span(bm, sp[0], sp[1]),
+                                        outm[sp[0]:sp[1]]), spans))
+        tm = time.perf_counter() - t
+        equal = bool(np.array_equal(outm, run_ref(am, bm, threads)))
+        txt = f"first {n1:,} pairs (1 thread), first {nm:,} (all cores)"
+    return {"value": round(nm / tm / 1e6, 3), "unit": "M pairs/s", "cores": threads,
+            "single_thread": round(n1 / t1 / 1e6, 3), "sample": txt,
+            "equal_to_reference": equal,
+            "kind": "x86 AVX2 kernels of the package's CPU engine (method='simd')"}
 
 
 # ============================================================ rooflines --

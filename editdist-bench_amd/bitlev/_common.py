@@ -1,6 +1,7 @@
 """Argument checks shared by the backend (same rules as bitlev/_common.py:9-33
 of the reference: max_dist None -> -1, negative -> ValueError, clamp to 2**30;
-dtype one of int32 / uint16 / uint8)."""
+dtype one of int32 / uint16 / uint8), and the saturating conversion of the
+CPU methods' int32 results."""
 from __future__ import annotations
 
 import numpy as np
@@ -27,3 +28,12 @@ def check_dtype(dtype) -> np.dtype:
 
 def as_list(x):
     return x if isinstance(x, list) else list(x)
+
+
+def saturate(out: np.ndarray, dt) -> np.ndarray:
+    """int32 results in the requested dtype, narrower types saturating at
+    their maximum (bitlev/_common.py:25-30)."""
+    dt = np.dtype(dt)
+    if dt == np.int32:
+        return out
+    return np.minimum(out, np.iinfo(dt).max).astype(dt)
