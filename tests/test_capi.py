@@ -152,7 +152,7 @@ def test_argument_checks():
     with pytest.raises(ValueError):
         check_dtype(np.float32)
     with pytest.raises(ValueError):
-        bitlev.pairwise(["a"], ["b"], method="scalar")
+        bitlev.pairwise(["a"], ["b"], method="neon")      # not a method of this package
 
 
 def test_api_errors_before_any_device_work():
