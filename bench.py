@@ -205,13 +205,13 @@ class Workload:
             self.X, self.Y = hip.Packed(*self.hx, stream=sp), hip.Packed(*self.hy, stream=sp)
             self.Y.stripes(sp)          # the resident search layout of the choice set
             self.out = hip.DeviceArray.empty((self.X.n, self.Y.n), self.dt)
-            self.plan = hip.cdist_plan(self.X, self.Y)
+            self.plan = hip.cdist_plan(self.X, self.Y, sp)
             self.pairs_total = self.X.n * self.Y.n
         else:
             self.hx, self.hy = W.pairs(wl["gen"], wl["n"], seed)
             self.X, self.Y = hip.Packed(*self.hx, stream=sp), hip.Packed(*self.hy, stream=sp)
             self.out = hip.DeviceArray.empty((self.X.n,), self.dt)
-            self.plan = hip.pairs_plan(self.X, self.Y)
+            self.plan = hip.pairs_plan(self.X, self.Y, sp)
             self.pairs_total = self.X.n
         stream.sync()
         self.setup_s = time.perf_counter() - t0

@@ -102,10 +102,16 @@ class DeviceBuffer:
             pass
 
     def upload(self, host: np.ndarray, stream=None, offset: int = 0):
+        """Copy host data in, ordered on `stream`. Without a stream the copy
+        has landed when this returns: the null stream is not ordered with the
+        non-blocking streams the package creates, so a kernel launched next on
+        one of them must not race an in-flight copy on it."""
         host = np.ascontiguousarray(host)
         if host.nbytes + offset > self.nbytes:
             raise ValueError("upload larger than the buffer")
         call("bl_memcpy_htod", self.ptr + offset, host.ctypes.data, host.nbytes, stream)
+        if stream is None and host.nbytes:
+            call("bl_stream_sync", None)
 
     @classmethod
     def from_host(cls, host: np.ndarray, stream=None) -> "DeviceBuffer":

@@ -160,33 +160,35 @@ class ChoiceStripes:
         return self.desc
 
 
-def _idx_buffers(groups):
-    return [DeviceBuffer.from_host(g) if len(g) else None for g in groups]
+def _idx_buffers(groups, stream=None):
+    return [DeviceBuffer.from_host(g, stream) if len(g) else None for g in groups]
 
 
 class Plan:
     """Class split of one call shape, with its index lists on the device.
     Built once per (A, B) pair set or per query set and reused by every call
-    on the same resident inputs, so a repeated call is kernel launches only."""
+    on the same resident inputs, so a repeated call is kernel launches only.
+    The index lists are uploaded on `stream` (the stream the calls run on);
+    without one the upload has finished when the plan is returned."""
 
-    def __init__(self, groups, rest, max_len):
+    def __init__(self, groups, rest, max_len, stream=None):
         self.groups = groups
-        self.bufs = _idx_buffers(groups)
+        self.bufs = _idx_buffers(groups, stream)
         self.rest = rest
-        self.rest_buf = DeviceBuffer.from_host(rest) if len(rest) else None
+        self.rest_buf = DeviceBuffer.from_host(rest, stream) if len(rest) else None
         self.max_len = max_len
 
 
-def pairs_plan(A: Packed, B: Packed) -> Plan:
+def pairs_plan(A: Packed, B: Packed, stream=None) -> Plan:
     groups, rest = pair_classes(A.h_off, B.h_off)
     mx = int(max(A.lengths[rest].max(), B.lengths[rest].max())) if len(rest) else 0
-    return Plan(groups, rest, mx)
+    return Plan(groups, rest, mx, stream)
 
 
-def cdist_plan(Q: Packed, C: Packed) -> Plan:
+def cdist_plan(Q: Packed, C: Packed, stream=None) -> Plan:
     groups, rest = query_classes(Q.h_off)
     mx = int(max(Q.lengths[rest].max(), C.max_len)) if len(rest) else 0
-    return Plan(groups, rest, mx)
+    return Plan(groups, rest, mx, stream)
 
 
 def pairs_into(A: Packed, B: Packed, out: DeviceArray, k: int = -1, trim: bool = True,
@@ -197,7 +199,7 @@ def pairs_into(A: Packed, B: Packed, out: DeviceArray, k: int = -1, trim: bool =
     if A.n != B.n:
         raise ValueError(f"length mismatch: {A.n} vs {B.n}")
     if plan is None:
-        plan = pairs_plan(A, B)
+        plan = pairs_plan(A, B, stream)
     if A.n == 0:
         return plan
     dt = DTYPE_CODE[out.dtype]
@@ -218,7 +220,7 @@ def cdist_into(Q: Packed, C: Packed, out: DeviceArray, k: int = -1, stream=None,
     reference's launches one for one (bitlev/gpu.py:121-130); stripes=True
     reads the choice set's resident stripes instead (bl_cdist_class_stripes)."""
     if plan is None:
-        plan = cdist_plan(Q, C)
+        plan = cdist_plan(Q, C, stream)
     if Q.n == 0 or C.n == 0:
         return plan
     dt = DTYPE_CODE[out.dtype]
@@ -329,7 +331,7 @@ def cdist_batched_into(Q: Packed, C: Packed, out: DeviceArray, k: int = -1, stre
         if kvec is not None:
             raise ValueError("per-query thresholds need every query <= 512 bytes")
         if plan is None:
-            plan = cdist_plan(Q, C)
+            plan = cdist_plan(Q, C, stream)
         call("bl_long", 0, Q.d_buf.ptr, Q.d_off.ptr, C.d_buf.ptr, C.d_off.ptr, plan.rest_buf.ptr,
              len(plan.rest), C.n, out.ptr, DTYPE_CODE[out.dtype], 0, int(k), plan.max_len, stream)
     return plan
@@ -414,6 +416,7 @@ class Graph:
 
     def __init__(self, fn, stream: Stream, warm: bool = True):
         self.stream = stream
+        self.fn = fn          # holds what the captured calls read and write (workspace, outputs)
         if warm:
             fn()
             stream.sync()
