@@ -95,6 +95,10 @@ def parse():
                     help="pairs workloads run on their own: end-to-end leg from Python lists of "
                          "this many pairs (packing, PCIe copies, kernels, results back), staged "
                          "pipeline vs one whole-batch pack; 0 skips it")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="time only the named form(s) of the workload: no plain-stripes form, cold "
+                         "call, match list or ingest leg (for counter runs whose per-step sums must "
+                         "hold the step's kernels only)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 3 times (a kernel of known traffic "
                          "for calibrating the HBM byte counters) and print the byte count")
@@ -750,6 +754,7 @@ def bench_workload(name, args, seed, ev, stream, full):
     plain-stripes form, the cold call, the match list and the ingest leg)."""
     W_ = Workload(name, args, seed, stream)
     forms = ["plain", "batched"] if args.form == "both" else [args.form]
+    full = full and not args.no_legs
     measured = run_forms(W_, forms, ev, args)
     if full and W_.kind == "cdist" and "plain" in forms:
         measured["plain-stripes"] = W_.measure("plain-stripes", ev, args.steps, args.warmup)
