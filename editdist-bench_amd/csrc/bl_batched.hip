@@ -98,21 +98,18 @@ __device__ __forceinline__ long long wave_take(unsigned long long *counter, long
  * (about 16 ns apart here), so one counter shared by 4,096 waves is a queue
  * in itself -- at the launch, at the end (every wave's last, failed
  * ticket) and for workloads of many small tiles. Wave w draws from queue
- * w % QUEUES only, and each queue owns an interleaved share of the tiles, so
- * every queue is balanced dynamically among its waves while each address
- * sees 1/QUEUES of the traffic. */
-constexpr int QUEUES = 16;
+ * w % Q only (Q = min(QUEUES, waves)), and each queue owns an interleaved
+ * share of the tiles, so every queue is balanced dynamically among its waves
+ * while each address sees 1/Q of the traffic. */
 constexpr int QUEUE_STRIDE = 32;                        /* u64 words: 256 bytes */
 constexpr int QUEUE_BYTES = QUEUES * QUEUE_STRIDE * 8;
 
-/* Tickets in runs of `grab` tiles; run r covers tiles r * grab .. + grab - 1.
- * With `first_static` every wave's first run is its own (run w), so a
- * launch's waves do not all queue at once (the 1,024 x 1e6 search at k = 2:
- * 0.34 -> 0.30 ms with one counter); tiles of uneven cost can prefer the
- * dynamic start. Later runs: queue j hands out runs S + j + Q i (S = the
- * static round, Q = min(QUEUES, waves), so that every queue has a wave), one
- * atomic per run. A wave takes its runs in increasing order. next / end are
- * wave-uniform. */
+/* Tickets in runs of `grab` tiles; run r covers tiles r * grab .. + grab - 1,
+ * scheduled by queue_run (bl_device.h). With `first_static` every wave's
+ * first run is its own (run w), so a launch's waves do not all queue at once
+ * (the 1,024 x 1e6 search at k = 2: 0.34 -> 0.30 ms with one counter); tiles
+ * of uneven cost can prefer the dynamic start. One atomic per later run. A
+ * wave takes its runs in increasing order. next / end are wave-uniform. */
 struct Tickets {
     long long next = 0, end = 0;
     bool started = false;
@@ -127,11 +124,8 @@ struct Tickets {
                 started = true;
                 run = wid;
             } else {
-                /* every queue in use needs a wave: a launch of fewer than
-                 * QUEUES waves uses as many queues as it has waves */
-                const long long nqs = waves < QUEUES ? waves : QUEUES;
-                const int j = (int)(wid % nqs);
-                run = (first_static ? waves : 0) + j + nqs * wave_take(queues + j * QUEUE_STRIDE, 1);
+                const int j = queue_of(wid, waves);
+                run = queue_run(wid, waves, first_static, wave_take(queues + j * QUEUE_STRIDE, 1));
             }
             next = run * grab;
             end = next + grab;

@@ -1423,4 +1423,19 @@ BL_HD int packed_pair_w32(const u8 *ba, const int64_t *oa, const u8 *bb, const i
                               k);
 }
 
+/* ------------------------------------------------------------ work queues */
+/* The persistent kernels' tile schedule. Tiles go out in runs; with
+ * first_static, wave w's first run is run w. After that wave w draws from
+ * queue queue_of(w, waves) only, and its draw that finds the queue's counter
+ * at `taken` gets run queue_run(w, waves, first_static, taken): queue j hands
+ * out S + j, S + j + Q, S + j + 2Q, ... (S the static round, Q the queues in
+ * use). Q = min(QUEUES, waves), so every queue in use has a wave to drain it. */
+constexpr int QUEUES = 16;
+BL_HD int64_t queue_count(int64_t waves) { return waves < QUEUES ? waves : QUEUES; }
+BL_HD int queue_of(int64_t wid, int64_t waves) { return (int)(wid % queue_count(waves)); }
+BL_HD int64_t queue_run(int64_t wid, int64_t waves, bool first_static, int64_t taken)
+{
+    return (first_static ? waves : 0) + queue_of(wid, waves) + queue_count(waves) * taken;
+}
+
 }  // namespace bl

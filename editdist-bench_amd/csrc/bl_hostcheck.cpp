@@ -454,6 +454,13 @@ int blh_store(int d, int dtype)
 
 /* Pieces the tests pin individually. */
 uint64_t blh_transpose8(uint64_t x) { return transpose8(x); }
+
+/* The work-queue schedule of the persistent kernels. */
+int blh_queue_of(int64_t wid, int64_t waves) { return queue_of(wid, waves); }
+int64_t blh_queue_run(int64_t wid, int64_t waves, int first_static, int64_t taken)
+{
+    return queue_run(wid, waves, first_static != 0, taken);
+}
 uint32_t blh_bop3(uint32_t a, uint32_t b, uint32_t c, int which)
 {
     switch (which) {
