@@ -633,8 +633,11 @@ def kernel_name(W_, form):
         nw = 1 if mq <= 64 else 2 if mq <= 128 else 4 if mq <= 256 else 8
         if form.startswith("plain"):
             return f"cdist_class_kernel<{nw}>"
-        if W_.params and W_.params.get("mode") == "count" and W_.k >= 0:
+        mode = W_.params.get("mode") if W_.params else None
+        if mode == "count" and W_.k >= 0:
             return f"cdist_count_kernel<{nw}>"
+        if mode == "exact" or (mode is None and W_.k < 0):
+            return f"cdist_exact_kernel<{nw}>"
         return f"cdist_batched_kernel<{nw}>"
     return ("pairs_class_kernel (one launch per length class)" if form == "plain"
             else "pairs pipeline (classify, scan, scatter, pairs_batched_kernel) or "

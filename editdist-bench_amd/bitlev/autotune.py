@@ -10,8 +10,8 @@ signature in a JSON file ($BITLEV_TUNE_CACHE, default
 ~/.cache/bitlev/autotune.json).
 
 cdist: lane mode (compacted lockstep passes, strided lanes, gap windows for
-k >= 0, or for k <= 3 the four-choice byte filter), persistent waves per CU,
-choices per tile. pairs: direct (one launch) or bucketed, and
+k >= 0, or for k <= 3 the four-choice byte filter; exact scans for exact
+searches), persistent waves per CU, choices per tile. pairs: direct (one launch) or bucketed, and
 for bucketed persistent waves per CU, workgroup size, text-length bucket
 width, bucketing chunk; whether to trim common affixes (which never changes a
 distance); then, on the two best bucketed shapes, the smallest tile (in
@@ -37,6 +37,9 @@ CDIST_SPACE = {"mode": ["strided", "queue"], "waves_per_cu": [8, 16, 32],
 CDIST_SWAR_SPACE = {"mode": ["swar"], "waves_per_cu": [8, 16, 32], "tile": [2048, 8192, 32768]}
 # gap windows (any threshold): the length test on 256-choice windows, the rest staged
 CDIST_GAP_SPACE = {"mode": ["gap"], "waves_per_cu": [8, 16, 32], "tile": [2048, 8192, 32768]}
+# exact scans (exact searches): one workgroup per (query, choice block), the
+# set in length order; only the block size matters
+CDIST_EXACT_SPACE = {"mode": ["exact"], "tile": [256, 1024, 4096, 16384]}
 # count windows (any threshold; choices with class profiles): a choice block
 # against a group of queries, the length and class-count bounds first
 CDIST_COUNT_SPACE = {"mode": ["count"], "waves_per_cu": [16, 32], "tile": [2048, 8192, 32768],
@@ -202,6 +205,8 @@ def tune_cdist(Q: hip.Packed, C: hip.Packed, out: hip.DeviceArray, k: int = -1, 
     cands = list(_grid(space or CDIST_SPACE))
     if space is None and 0 <= k <= 3:
         cands += list(_grid(CDIST_SWAR_SPACE))
+    if space is None and k < 0:
+        cands += list(_grid(CDIST_EXACT_SPACE))
     if space is None and k >= 0:
         cands += list(_grid(CDIST_GAP_SPACE))
         if C.stripes(stream).desc.counts:

@@ -35,6 +35,7 @@ from .device import DeviceArray, DeviceBuffer
 PAIR_CAPS = (16, 32, 64, 128, 256)   # pairs kernels: shorter string <= CAP
 CROSS_NWS = (1, 2, 4, 8)             # cdist kernels: query <= 64 * NW
 BATCHED_MAX_QUERY = 512              # longer cdist queries take the wavefront kernel
+SORT_BLOCK = 4096                    # length-ordered stripes: strings sorted within blocks of this many
 FORM = "batched"                     # default form of pairwise() / cdist()
 
 __all__ = ["available", "distance", "pairwise", "cdist", "Packed", "Plan", "pairs_plan",
@@ -108,14 +109,28 @@ class Packed:
         self.d_off = DeviceBuffer.from_host(self.h_off, stream)
 
         self._stripes = None
+        self._sorted = None
 
     @classmethod
     def from_strings(cls, strings, stream=None) -> "Packed":
         return cls(*pack(strings), stream=stream)
 
-    def stripes(self, stream=None) -> "ChoiceStripes":
+    def stripes(self, stream=None, order: str = "set") -> "ChoiceStripes":
         """The set as search-kernel stripes (built on the device on first use,
-        on `stream`, which is then synchronised; kept with the set)."""
+        on `stream`, which is then synchronised; kept with the set).
+        order="set": slot c holds string c, with the byte-class and 2-gram
+        profiles (every search mode); order="length": sorted by length within
+        blocks of SORT_BLOCK strings, no profiles (exact scans only)."""
+        if order == "length":
+            if self._sorted is None:
+                if self.n < 2 or bool(np.all(self.lengths[1:] >= self.lengths[:-1])):
+                    # already in length order (a serving set): the set's own stripes
+                    self._sorted = self.stripes(stream)
+                else:
+                    self._sorted = ChoiceStripes(self, stream, profiles=False, sort_block=SORT_BLOCK)
+            return self._sorted
+        if order != "set":
+            raise ValueError("order must be 'set' or 'length'")
         if self._stripes is None:
             self._stripes = ChoiceStripes(self, stream)
         return self._stripes
@@ -123,8 +138,8 @@ class Packed:
     @property
     def device_bytes(self) -> int:
         b = self.d_buf.nbytes + self.d_off.nbytes
-        if self._stripes is not None:
-            b += self._stripes.nbytes
+        for st in {id(x): x for x in (self._stripes, self._sorted) if x is not None}.values():
+            b += st.nbytes
         return b
 
 
@@ -135,10 +150,17 @@ class ChoiceStripes:
     2-gram profiles (the batched searches' lower bounds; profiles=False leaves
     them out)."""
 
-    def __init__(self, P: Packed, stream=None, profiles: bool = True):
+    def __init__(self, P: Packed, stream=None, profiles: bool = True, sort_block: int = 0):
         n = P.n
         gbase = np.empty((n + 63) // 64 + 1, np.int64)
-        total = call("bl_stripes_layout", P.h_off.ctypes.data, n, gbase.ctypes.data)
+        self.d_order = None
+        if sort_block:
+            order = np.empty(max(n, 1), np.int32)
+            total = call("bl_stripes_layout_sorted", P.h_off.ctypes.data, n, int(sort_block),
+                         order.ctypes.data, gbase.ctypes.data)
+            self.d_order = DeviceBuffer.from_host(order, stream)
+        else:
+            total = call("bl_stripes_layout", P.h_off.ctypes.data, n, gbase.ctypes.data)
         self.d_gbase = DeviceBuffer.from_host(gbase, stream)
         self.d_words = DeviceBuffer(max(int(total), 1) * 4)
         self.d_len = DeviceBuffer(max(n, 1) * 4)
@@ -146,14 +168,21 @@ class ChoiceStripes:
         self.d_counts2 = DeviceBuffer(max(n, 1) * 16) if profiles else None
         counts = self.d_counts.ptr if profiles else None
         counts2 = self.d_counts2.ptr if profiles else None
-        call("bl_stripes_build", P.d_buf.ptr, P.d_off.ptr, n, self.d_gbase.ptr, self.d_words.ptr,
-             self.d_len.ptr, counts, counts2, stream)
+        if sort_block:
+            call("bl_stripes_build_order", P.d_buf.ptr, P.d_off.ptr, n, self.d_order.ptr,
+                 self.d_gbase.ptr, self.d_words.ptr, self.d_len.ptr, counts, counts2, stream)
+        else:
+            call("bl_stripes_build", P.d_buf.ptr, P.d_off.ptr, n, self.d_gbase.ptr, self.d_words.ptr,
+                 self.d_len.ptr, counts, counts2, stream)
         # built once per set; finish here so that any stream may read them next
         call("bl_stream_sync", stream)
         self.nbytes = self.d_gbase.nbytes + self.d_words.nbytes + self.d_len.nbytes
         if profiles:
             self.nbytes += self.d_counts.nbytes + self.d_counts2.nbytes
-        self.desc = Stripes(self.d_words.ptr, self.d_gbase.ptr, self.d_len.ptr, n, counts, counts2)
+        if sort_block:
+            self.nbytes += self.d_order.nbytes
+        self.desc = Stripes(self.d_words.ptr, self.d_gbase.ptr, self.d_len.ptr, n, counts, counts2,
+                            self.d_order.ptr if sort_block else None)
 
     @property
     def ref(self):
@@ -270,7 +299,8 @@ def _cdist_params(params):
     p = CdistParams()
     p.waves_per_cu = int(params.get("waves_per_cu", 0))
     p.tile = int(params.get("tile", 0))
-    p.reserved[0] = {"strided": 1, "swar": 2, "gap": 3, "count": 4}.get(params.get("mode"), 0)
+    p.reserved[0] = {"strided": 1, "swar": 2, "gap": 3, "count": 4, "exact": 5,
+                     "queue": 6}.get(params.get("mode"), 0)
     p.reserved[2] = int(params.get("qgroup", 0))
     p.reserved[1] = int(params.get("grab", 0))
     p.reserved[3] = (0 if params.get("gram", True) else 1) | (0 if params.get("static", True) else 2)
@@ -324,7 +354,12 @@ def cdist_batched_into(Q: Packed, C: Packed, out: DeviceArray, k: int = -1, stre
         return plan
     mq = min(Q.max_len, BATCHED_MAX_QUERY)
     ws = (workspace or _workspace(stream)).get(lib().bl_cdist_workspace_size(Q.n, mq))
-    call("bl_cdist_tuned", Q.d_buf.ptr, Q.d_off.ptr, Q.n, C.stripes(stream).ref, mq, out.ptr,
+    # exact scans (the automatic mode of an exact search, or asked for) read
+    # the set in length order, so that the lanes of a wave finish together
+    mode = params.get("mode") if params else None
+    exact = mode == "exact" or (mode is None and k < 0 and kvec is None)
+    st = C.stripes(stream, "length" if exact else "set")
+    call("bl_cdist_tuned", Q.d_buf.ptr, Q.d_off.ptr, Q.n, st.ref, mq, out.ptr,
          DTYPE_CODE[out.dtype], int(k), kvec.ptr if kvec else None, ws.ptr, ws.nbytes,
          _cdist_params(params), stream)
     if Q.max_len > BATCHED_MAX_QUERY:

@@ -215,7 +215,10 @@ class Server:
     # ------------------------------------------------------------ scheduling
     @staticmethod
     def _key(r: Request):
-        return (r.kind, r.set_name, r.dtype.str, r.trim)
+        # exact searches fuse only with exact searches: they run another
+        # kernel, and a thresholded request would wait for a batch that costs
+        # far more than its own
+        return (r.kind, r.set_name, r.dtype.str, r.trim, r.kind == "cdist" and r.k < 0)
 
     def _footprint(self, reqs) -> int:
         out = sum(r.nbytes_out for r in reqs)
@@ -313,8 +316,10 @@ class _GpuExecutor:
     others wait. A search batch whose requests all have a max_dist runs the
     count kernel (the batch's queries as one group against blocks of the
     set, most choices settled by the length and class-count bounds), sized
-    so each slot's waves get about two blocks each; a batch with an exact
-    request uses the four-choice filter (the launch falls back per query)."""
+    so each slot's waves get about two blocks each; exact requests batch
+    apart from thresholded ones and run the exact scans (one workgroup per
+    query and choice block, the whole Peq table in LDS, 32-row units) over
+    the set, which is kept in length order."""
 
     def __init__(self, slots: int = 4):
         wpc = max(4, 32 // max(1, int(slots)))
@@ -324,8 +329,12 @@ class _GpuExecutor:
                              "trim": True}
 
     def search_params(self, nq: int, nc: int, kv: np.ndarray) -> dict:
-        if len(kv) == 0 or int(kv.min()) < 0:
+        if len(kv) == 0:
             return self.cdist_params
+        if int(kv.min()) < 0:
+            # exact scans: one workgroup per (query, choice block); the set is
+            # already in length order, so a wave's lanes finish together
+            return {"mode": "exact"}
         g = min(64, nq)
         waves = self.wpc * 256
         blocks_wanted = max(1, 2 * waves // ((nq + g - 1) // g))

@@ -852,6 +852,86 @@ __global__ __launch_bounds__(256) void cdist_batched_kernel(
     }
 }
 
+/* Exact searches (lane mode 5; any threshold, lane by lane): one workgroup
+ * per (query, block of `tile` slots), the grid walking the queries of a block
+ * before the next block so that the block's stripes serve every query from
+ * L2 (the query index rotated by the block, so every query visits every
+ * XCD). The query's whole Peq table is copied into LDS word-major (word w of
+ * byte c at T[256 w + c]: one 8-byte bank pair per byte value whatever the
+ * word count) from the table peq_build_kernel wrote; each wave takes 64 slots
+ * at a time and every lane scans its choice in the query's 32-row units
+ * (units_lane: one u32 block, u64 blocks, or u64 blocks and a u32 tail), so a
+ * 72-byte query costs 3 units, not 4. With `order` (stripes in length order)
+ * slot c holds choice order[c]: the lanes of a wave scan strings of about the
+ * same length and finish together, and each result goes to the caller's
+ * column order[c]. */
+template <int NWMAX, typename Sink>
+__global__ __launch_bounds__(256) void cdist_exact_kernel(
+    const int64_t *__restrict__ oq, int64_t nq, const u64 *__restrict__ gpeq,
+    const u32 *__restrict__ words, const int64_t *__restrict__ gbase,
+    const int32_t *__restrict__ clen, const int32_t *__restrict__ order, int64_t nc, Sink sink,
+    int k_all, const int32_t *__restrict__ kvec, int64_t tile)
+{
+    __shared__ u64 T[NWMAX * 256];
+    /* workgroups go round-robin to the XCDs (b % 8), so with the query index
+     * fastest query q would run on XCD q % 8 for every block and an XCD's
+     * share would follow its queries' lengths; rotating by the block spreads
+     * every query over all XCDs (64 queries of 16-128 bytes against 1e6
+     * choices: 5.66 -> 4.00 ms) */
+    const int64_t b = blockIdx.x;
+    const int64_t blk = b / nq;
+    const int64_t q = (b - blk * nq + blk) % nq;
+    const int64_t c0 = blk * tile;
+    const int64_t c1 = c0 + tile < nc ? c0 + tile : nc;
+    const int m = (int)(oq[q + 1] - oq[q]);
+    const int k = kvec ? clamp_k_dev(kvec[q]) : k_all;
+    const int64_t orow = q * nc;
+    if (m == 0 || m > 64 * NWMAX) {
+        /* empty query: n, or k + 1 (csrc/gpu_kernels.cu:53-54); too long for this
+         * launch: caller error, marked -1 */
+        for (int64_t c = c0 + threadIdx.x; c < c1; c += 256)
+            sink.put(orow + (order ? (int64_t)order[c] : c), m == 0 ? empty_pattern(clen[c], k) : -1, k);
+        return;
+    }
+    const u64 *qp = gpeq + (size_t)q * 256 * NWMAX;
+    for (int i = threadIdx.x; i < 256 * NWMAX; i += 256) T[(i % NWMAX) * 256 + i / NWMAX] = qp[i];
+    __syncthreads();
+    const int lane = lane_id();
+    const int64_t first = c0 + 64 * (threadIdx.x >> 6);
+    auto run = [&](auto uc) {
+        constexpr int U = decltype(uc)::value;
+        auto eq64 = [&](u32 ch, int w) { return T[w * 256 + ch]; };
+        auto eq32 = [&](u32 ch, int u) { return ((const u32 *)T)[2 * ((u >> 1) * 256 + ch) + (u & 1)]; };
+        for (int64_t s = first; s < c1; s += 256) {
+            const int64_t c = s + lane;
+            if (c < c1) {
+                const StripeText text{words + gbase[c >> 6] + (c & 63)};
+                const int d = units_lane<U>(m, text, clen[c], k, eq64, eq32);
+                sink.put(orow + (order ? (int64_t)order[c] : c), d, k);
+            }
+        }
+    };
+    using std::integral_constant;
+    switch ((m + 31) >> 5) {
+    case 1: run(integral_constant<int, 1>()); break;
+    case 2: run(integral_constant<int, 2>()); break;
+    case 3: if constexpr (NWMAX >= 2) run(integral_constant<int, 3>()); break;
+    case 4: if constexpr (NWMAX >= 2) run(integral_constant<int, 4>()); break;
+    case 5: if constexpr (NWMAX >= 4) run(integral_constant<int, 5>()); break;
+    case 6: if constexpr (NWMAX >= 4) run(integral_constant<int, 6>()); break;
+    case 7: if constexpr (NWMAX >= 4) run(integral_constant<int, 7>()); break;
+    case 8: if constexpr (NWMAX >= 4) run(integral_constant<int, 8>()); break;
+    case 9: if constexpr (NWMAX >= 8) run(integral_constant<int, 9>()); break;
+    case 10: if constexpr (NWMAX >= 8) run(integral_constant<int, 10>()); break;
+    case 11: if constexpr (NWMAX >= 8) run(integral_constant<int, 11>()); break;
+    case 12: if constexpr (NWMAX >= 8) run(integral_constant<int, 12>()); break;
+    case 13: if constexpr (NWMAX >= 8) run(integral_constant<int, 13>()); break;
+    case 14: if constexpr (NWMAX >= 8) run(integral_constant<int, 14>()); break;
+    case 15: if constexpr (NWMAX >= 8) run(integral_constant<int, 15>()); break;
+    default: if constexpr (NWMAX >= 8) run(integral_constant<int, 16>()); break;
+    }
+}
+
 /* Count mode, the windows of one tile: choices c0..c1 (c0 a multiple of 256)
  * against the queries q0 + j whose bit j is set in okmask, query j's length,
  * threshold and profile in lane j of lm, lk, lp. A lane holds four
@@ -1927,8 +2007,10 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
     /* tiles start on stripe groups (multiples of 64 choices; of 256 for the
      * four-choice filter's windows) */
     const int64_t tile = (prm && prm->tile > 0 ? (prm->tile + 255) / 256 * 256 : 8192);
-    int mode = prm && prm->reserved[0] >= 1 && prm->reserved[0] <= 4 ? prm->reserved[0] : 0;
-    if (mode == 4 && !choices->counts) mode = 0;   /* count mode needs the choices' profiles */
+    int mode = prm && prm->reserved[0] >= 1 && prm->reserved[0] <= 6 ? prm->reserved[0] : 0;
+    if (mode == 0) mode = k < 0 && !d_kvec ? 5 : 6;  /* automatic: exact scans for exact searches */
+    if (choices->order) mode = 5;                    /* a length-ordered set: exact scans only */
+    if (mode == 4 && !choices->counts) mode = 6;   /* count mode needs the choices' profiles */
     const int grab = prm && prm->reserved[1] > 0 ? prm->reserved[1] : 1;
     const int qgroup = mode == 4 ? (prm->reserved[2] > 0 ? (prm->reserved[2] < 64 ? prm->reserved[2] : 64) : 16)
                                  : 1;
@@ -1939,6 +2021,16 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
     /* count mode: each wave's first tile statically, unless switched off (bit 1) */
     const int first_static = prm && (prm->reserved[3] & 2) ? 0 : 1;
     const size_t lds = sizeof(WaveLds) * 4;
+    /* exact scans: one workgroup per (query, block of xtile slots); by default
+     * blocks sized for about 16 workgroups per CU over the batch, 256 to 4096
+     * slots, and never more than 2^24 workgroups */
+    int64_t xtile = prm && prm->tile > 0 ? (prm->tile + 63) / 64 * 64 : 0;
+    if (!xtile) {
+        xtile = nq * nc / ((int64_t)device_cus() * 16) / 256 * 256;
+        xtile = xtile < 256 ? 256 : xtile > 4096 ? 4096 : xtile;
+    }
+    while (nq * ((nc + xtile - 1) / xtile) > ((int64_t)1 << 24)) xtile *= 2;
+    const int64_t xblk = (nc + xtile - 1) / xtile;
     int64_t blocks = (int64_t)device_cus() * wpc / 4;
     const int64_t ntiles = ((nc + tile - 1) / tile) * ((nq + qgroup - 1) / qgroup);
     const int64_t need = (ntiles + 3) / 4;
@@ -1981,11 +2073,15 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
                 kern<<<capped(kern, lb), 256, lb, s>>>(
                     d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, ccnt2,
                     qcnt2, nc, sink, k, d_kvec, tile, grab, qgroup, ctr, first_static);
+            } else if (mode == 5) {
+                cdist_exact_kernel<NW, Sink><<<(unsigned)(nq * xblk), 256, 0, s>>>(
+                    d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, choices->order, nc,
+                    sink, k, d_kvec, xtile);
             } else {
                 auto kern = cdist_batched_kernel<NW, Sink>;
                 kern<<<capped(kern, lds), 256, lds, s>>>(
                     d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, nc,
-                    sink, k, d_kvec, tile, mode, grab, ctr);
+                    sink, k, d_kvec, tile, mode == 6 ? 0 : mode, grab, ctr);
             }
         };
         switch (nwmax) {

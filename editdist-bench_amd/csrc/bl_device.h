@@ -1047,6 +1047,37 @@ BL_HD int peq_lane(Lo32Ptr lo32, PeqPtr peq, int m, const Text &t, int n, int k)
     return scan_text<u64, NW, EAGER>(m, nw, 64 * nw, t, n, k, eq64);
 }
 
+/* The search lane in 32-row units: a query of m rows, 32 (U - 1) < m <= 32 U,
+ * given by its Peq table -- eq64(c, w) = rows 64w .. 64w + 63 of the row of
+ * byte c, eq32(c, u) = rows 32u .. 32u + 31 -- against a text of n bytes.
+ * U = 1 scans one u32 block, an even U U/2 u64 blocks, an odd U > 1 (U - 1)/2
+ * u64 blocks and a u32 tail, so a pattern costs its 32-row units and no more
+ * (peq_lane rounds up to 64-row words). k < 0: the exact distance; k >= 0:
+ * min(d, k + 1), the 32-row stage first as in peq_lane. Same results as
+ * peq_lane (csrc/gpu_myers.cuh:124-145, csrc/myers.c:58-77, :107-130). */
+template <int U, typename Text, typename Eq64, typename Eq32>
+BL_HD int units_lane(int m, const Text &t, int n, int k, Eq64 eq64, Eq32 eq32)
+{
+    static_assert(U >= 1 && U <= 16, "1 to 16 units of 32 rows");
+    if (k >= 0 && (n > m ? n - m : m - n) > k) return k + 1;
+    if (n == 0) return m;
+    auto lo = [&](u32 c, int) { return eq32(c, 0); };
+    if constexpr (U == 1) {
+        return scan_text<u32, 1, true>(m, 1, 32, t, n, k, lo);
+    } else {
+        if (k >= 0) {
+            const int r = scan_text<u32, 1, true>(m, 1, 32, t, n, k, lo);
+            if (r >= 0) return r;
+        }
+        constexpr int W = U / 2;
+        if constexpr (U % 2 == 0) {
+            return scan_text<u64, W, (W <= 4)>(m, W, 64 * W, t, n, k, eq64);
+        } else {
+            return scan_mixed<W, true>(m, t, n, k, eq64, [&](u32 c) { return eq32(c, U - 1); });
+        }
+    }
+}
+
 /* Empty cross query: d = n, or k + 1 when k >= 0 and n > k
  * (csrc/gpu_kernels.cu:53-54, csrc/edpy.c:111-115). */
 BL_HD int empty_pattern(int n, int k) { return (k >= 0 && n > k) ? k + 1 : n; }

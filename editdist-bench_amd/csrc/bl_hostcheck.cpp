@@ -231,6 +231,16 @@ static int blh_pairs_batched_core(const u8 *ba, const int64_t *oa, const u8 *bb,
     return 0;
 }
 
+/* The exact-search lane (units_lane) over a word-major Peq table T (word w of
+ * byte c at T[256 w + c]), as the exact kernel's LDS copy holds it. */
+template <int U>
+static int units_host(const u64 *T, int m, const StripeText &t, int n, int k)
+{
+    auto eq64 = [&](u32 ch, int w) { return T[w * 256 + ch]; };
+    auto eq32 = [&](u32 ch, int u) { return ((const u32 *)T)[2 * ((u >> 1) * 256 + ch) + (u & 1)]; };
+    return units_lane<U>(m, t, n, k, eq64, eq32);
+}
+
 extern "C" {
 
 /* One pair through the lane code the pairs kernels run (class by the shorter
@@ -289,6 +299,35 @@ int blh_peq(const u8 *q, int m, const u8 *t, int n, int k)
     const int64_t off[2] = {0, n};
     const HostStripes hs(t, off, 1);
     return peq_host(q, m, hs.text(0), t, n, k);
+}
+
+/* One (query, choice) through the exact-search lane in 32-row units; m <= 512. */
+int blh_units(const u8 *q, int m, const u8 *t, int n, int k)
+{
+    if (m < 1 || m > 512) return -1;
+    std::vector<u64> T((size_t)256 * 8, 0);
+    for (int i = 0; i < m; i++) T[(size_t)(i >> 6) * 256 + q[i]] |= (u64)1 << (i & 63);
+    const int64_t off[2] = {0, n};
+    const HostStripes hs(t, off, 1);
+    const StripeText st = hs.text(0);
+    switch ((m + 31) >> 5) {
+    case 1: return units_host<1>(T.data(), m, st, n, k);
+    case 2: return units_host<2>(T.data(), m, st, n, k);
+    case 3: return units_host<3>(T.data(), m, st, n, k);
+    case 4: return units_host<4>(T.data(), m, st, n, k);
+    case 5: return units_host<5>(T.data(), m, st, n, k);
+    case 6: return units_host<6>(T.data(), m, st, n, k);
+    case 7: return units_host<7>(T.data(), m, st, n, k);
+    case 8: return units_host<8>(T.data(), m, st, n, k);
+    case 9: return units_host<9>(T.data(), m, st, n, k);
+    case 10: return units_host<10>(T.data(), m, st, n, k);
+    case 11: return units_host<11>(T.data(), m, st, n, k);
+    case 12: return units_host<12>(T.data(), m, st, n, k);
+    case 13: return units_host<13>(T.data(), m, st, n, k);
+    case 14: return units_host<14>(T.data(), m, st, n, k);
+    case 15: return units_host<15>(T.data(), m, st, n, k);
+    default: return units_host<16>(T.data(), m, st, n, k);
+    }
 }
 
 /* The wavefront path for any lengths (is_pair: trim/swap as pairs do). */

@@ -245,6 +245,8 @@ int bl_cdist_class_stripes(int nw, const uint8_t *d_buf_q, const int64_t *d_off_
     if (!d_buf_q || !d_off_q || !d_qidx || !choices->words || !choices->gbase || !choices->len)
         return fail(BL_EINVAL, "bl_cdist_class_stripes: null input");
     if (nqc > 0x7fffffff) return fail(BL_EINVAL, "bl_cdist_class_stripes: too many queries");
+    if (choices->order)
+        return fail(BL_EINVAL, "bl_cdist_class_stripes: choices in length order (use bl_cdist)");
     return cdist_class_launch("bl_cdist_class_stripes", nw, d_buf_q, d_off_q, d_qidx, nqc,
                               StripeChoices{choices->words, choices->gbase, choices->len}, nc,
                               d_out, dtype, k, stream);

@@ -12,8 +12,19 @@
  * so a wave stepping its lanes' texts by one dword reads 256 contiguous bytes.
  * Every string of a group is zero-padded to the group's widest (at least one
  * dword), and len[c] keeps its length. Built once per resident choice set.
+ *
+ * A set may also be laid out in length order: within each block of `block`
+ * slots the choices are sorted by length (stable), slot c holding choice
+ * order[c]. The lanes of a wave then scan strings of about the same length
+ * and finish together (an exact search costs m x n cells per pair, so a wave
+ * of unsorted lengths 16..128 runs as long as its longest string), the
+ * groups pad less, and the results still land in the caller's columns
+ * (order[c]) within the block's range of each row.
  */
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
 
 #include "bl_device.h"
 #include "bl_internal.h"
@@ -22,10 +33,12 @@ using namespace bl;
 
 namespace {
 
-/* One wave per group of 64 choices. */
+/* One wave per group of 64 slots; slot c holds choice order[c] (c when order
+ * is null). */
 __global__ __launch_bounds__(256) void stripes_build_kernel(const u8 *__restrict__ buf,
                                                             const int64_t *__restrict__ off,
                                                             int64_t n,
+                                                            const int32_t *__restrict__ order,
                                                             const int64_t *__restrict__ gbase,
                                                             u32 *__restrict__ words,
                                                             int32_t *__restrict__ len,
@@ -41,8 +54,9 @@ __global__ __launch_bounds__(256) void stripes_build_kernel(const u8 *__restrict
     int ln = 0;
     const u8 *s = buf;
     if (c < n) {
-        const int64_t o = off[c];
-        ln = (int)(off[c + 1] - o);
+        const int64_t src = order ? (int64_t)order[c] : c;
+        const int64_t o = off[src];
+        ln = (int)(off[src + 1] - o);
         s = buf + o;
         len[c] = ln;
         if (counts) counts[c] = class_profile(s, ln);
@@ -108,23 +122,79 @@ int64_t bl_stripes_layout(const int64_t *h_off, int64_t n, int64_t *h_gbase)
     return total;
 }
 
+int64_t bl_stripes_layout_sorted(const int64_t *h_off, int64_t n, int64_t block, int32_t *h_order,
+                                 int64_t *h_gbase)
+{
+    if (n < 0) return fail(BL_EINVAL, "bl_stripes_layout_sorted: negative count");
+    if (!h_off || !h_order || !h_gbase) return fail(BL_EINVAL, "bl_stripes_layout_sorted: null pointer");
+    if (block < 64 || block % 64 != 0)
+        return fail(BL_EINVAL, "bl_stripes_layout_sorted: block %lld is not a positive multiple of 64",
+                    (long long)block);
+    if (n > 0x7fffffff) return fail(BL_EINVAL, "bl_stripes_layout_sorted: too many strings");
+    for (int64_t c = 0; c < n; c++)
+        if (h_off[c + 1] < h_off[c]) return fail(BL_EINVAL, "bl_stripes_layout_sorted: offsets decrease");
+    auto len = [&](int64_t c) { return h_off[c + 1] - h_off[c]; };
+    std::vector<int64_t> cnt;
+    for (int64_t b0 = 0; b0 < n; b0 += block) {
+        const int64_t e = b0 + block < n ? b0 + block : n;
+        int64_t lo = len(b0), hi = lo;
+        for (int64_t c = b0; c < e; c++) {
+            const int64_t l = len(c);
+            lo = l < lo ? l : lo;
+            hi = l > hi ? l : hi;
+        }
+        if (hi - lo < 65536) {
+            /* counting sort, stable: slots taken in string order within each length */
+            cnt.assign((size_t)(hi - lo + 2), 0);
+            for (int64_t c = b0; c < e; c++) cnt[(size_t)(len(c) - lo + 1)]++;
+            for (size_t i = 1; i < cnt.size(); i++) cnt[i] += cnt[i - 1];
+            for (int64_t c = b0; c < e; c++) h_order[b0 + cnt[(size_t)(len(c) - lo)]++] = (int32_t)c;
+        } else {
+            for (int64_t c = b0; c < e; c++) h_order[c] = (int32_t)c;
+            std::stable_sort(h_order + b0, h_order + e,
+                             [&](int32_t x, int32_t y) { return len(x) < len(y); });
+        }
+    }
+    /* the group bases of the strings in slot order */
+    std::vector<int64_t> soff((size_t)n + 1);
+    soff[0] = 0;
+    for (int64_t c = 0; c < n; c++) soff[(size_t)c + 1] = soff[(size_t)c] + len(h_order[c]);
+    return stripes_layout(soff.data(), n, h_gbase);
+}
+
+static int stripes_build(const char *who, const uint8_t *d_buf, const int64_t *d_off, int64_t n,
+                         const int32_t *d_order, const int64_t *d_gbase, uint32_t *d_words,
+                         int32_t *d_len, uint32_t *d_counts, uint32_t *d_counts2, void *stream)
+{
+    if (n < 0) return fail(BL_EINVAL, "%s: negative count", who);
+    if (n == 0) return BL_OK;
+    if (!d_buf || !d_off || !d_gbase || !d_words || !d_len) return fail(BL_EINVAL, "%s: null pointer", who);
+    const int64_t groups = (n + 63) / 64;
+    const int64_t blocks = (groups + 3) / 4;
+    if (blocks > 0x7fffffff) return fail(BL_EINVAL, "%s: too many strings", who);
+    if (((uintptr_t)d_counts & 15) != 0 || ((uintptr_t)d_counts2 & 15) != 0)
+        return fail(BL_EINVAL, "%s: profiles not 16-byte aligned", who);
+    stripes_build_kernel<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(
+        d_buf, d_off, n, d_order, d_gbase, d_words, d_len, (Profile *)d_counts, (Profile *)d_counts2);
+    BL_LAUNCHED("stripes_build_kernel");
+    return BL_OK;
+}
+
 int bl_stripes_build(const uint8_t *d_buf, const int64_t *d_off, int64_t n, const int64_t *d_gbase,
                      uint32_t *d_words, int32_t *d_len, uint32_t *d_counts, uint32_t *d_counts2,
                      void *stream)
 {
-    if (n < 0) return fail(BL_EINVAL, "bl_stripes_build: negative count");
-    if (n == 0) return BL_OK;
-    if (!d_buf || !d_off || !d_gbase || !d_words || !d_len)
-        return fail(BL_EINVAL, "bl_stripes_build: null pointer");
-    const int64_t groups = (n + 63) / 64;
-    const int64_t blocks = (groups + 3) / 4;
-    if (blocks > 0x7fffffff) return fail(BL_EINVAL, "bl_stripes_build: too many strings");
-    if (((uintptr_t)d_counts & 15) != 0 || ((uintptr_t)d_counts2 & 15) != 0)
-        return fail(BL_EINVAL, "bl_stripes_build: profiles not 16-byte aligned");
-    stripes_build_kernel<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(
-        d_buf, d_off, n, d_gbase, d_words, d_len, (Profile *)d_counts, (Profile *)d_counts2);
-    BL_LAUNCHED("stripes_build_kernel");
-    return BL_OK;
+    return stripes_build("bl_stripes_build", d_buf, d_off, n, nullptr, d_gbase, d_words, d_len, d_counts,
+                         d_counts2, stream);
+}
+
+int bl_stripes_build_order(const uint8_t *d_buf, const int64_t *d_off, int64_t n, const int32_t *d_order,
+                           const int64_t *d_gbase, uint32_t *d_words, int32_t *d_len, uint32_t *d_counts,
+                           uint32_t *d_counts2, void *stream)
+{
+    if (n > 0 && !d_order) return fail(BL_EINVAL, "bl_stripes_build_order: null order");
+    return stripes_build("bl_stripes_build_order", d_buf, d_off, n, d_order, d_gbase, d_words, d_len,
+                         d_counts, d_counts2, stream);
 }
 
 }  /* extern "C" */

@@ -35,7 +35,7 @@
     X(bl_event_record) X(bl_stream_wait_event) X(bl_event_sync) X(bl_event_query)             \
     X(bl_event_elapsed_ms) X(bl_graph_begin) X(bl_graph_end) X(bl_graph_launch)               \
     X(bl_graph_destroy) X(bl_pairs_class) X(bl_stripes_layout) X(bl_stripes_build)            \
-    X(bl_gather_cols) X(bl_cdist_class) X(bl_cdist_class_stripes) X(bl_long)                  \
+    X(bl_stripes_layout_sorted) X(bl_stripes_build_order) X(bl_gather_cols) X(bl_cdist_class) X(bl_cdist_class_stripes) X(bl_long)                  \
     X(bl_cdist_workspace_size) X(bl_cdist) X(bl_cdist_tuned) X(bl_cdist_matches)              \
     X(bl_pairs_workspace_size) X(bl_pairs) X(bl_pairs_tuned) X(bl_cdist_noalloc)
 
@@ -253,6 +253,25 @@ FN(bl_stripes_build)
     PARSE("O&O&LO&O&O&O&O&O&", P(buf), P(off), &n, P(gbase), P(words), P(len), P(counts),
           P(counts2), P(s));
     RUN(H.bl_stripes_build(buf, off, n, gbase, words, len, counts, counts2, s));
+}
+
+FN(bl_stripes_layout_sorted)
+{
+    void *off, *order, *gbase;
+    long long n, block;
+    PARSE("O&LLO&O&", P(off), &n, &block, P(order), P(gbase));
+    long long r;
+    Py_BEGIN_ALLOW_THREADS r = H.bl_stripes_layout_sorted(off, n, block, order, gbase);
+    Py_END_ALLOW_THREADS return PyLong_FromLongLong(r);
+}
+
+FN(bl_stripes_build_order)
+{
+    void *buf, *off, *order, *gbase, *words, *len, *counts, *counts2, *s;
+    long long n;
+    PARSE("O&O&LO&O&O&O&O&O&O&", P(buf), P(off), &n, P(order), P(gbase), P(words), P(len),
+          P(counts), P(counts2), P(s));
+    RUN(H.bl_stripes_build_order(buf, off, n, order, gbase, words, len, counts, counts2, s));
 }
 
 FN(bl_gather_cols)

@@ -116,6 +116,14 @@ typedef struct {
                                 (L1 + |t_q - t_c|) / 4 over the two (t: the
                                 2-gram totals; L1 / 4 past 256 bytes), applied to
                                 what the first leaves open */
+    const int32_t *order;    /* NULL: slot c holds choice c. Otherwise the set is
+                                in length order (bl_stripes_layout_sorted): slot c
+                                holds choice order[c], and words, len and the
+                                profiles are indexed by slot; results still go to
+                                the caller's column order[c]. Only the exact-search
+                                kernel (lane mode 5) reads such a set: bl_cdist*
+                                run it whatever the mode asked for, and
+                                bl_cdist_class_stripes rejects it */
 } bl_stripes;
 
 /* Host: group bases of the stripes of n strings with host offsets
@@ -129,6 +137,25 @@ int64_t bl_stripes_layout(const int64_t *h_off, int64_t n, int64_t *h_gbase);
 int bl_stripes_build(const uint8_t *d_buf, const int64_t *d_off, int64_t n,
                      const int64_t *d_gbase, uint32_t *d_words, int32_t *d_len,
                      uint32_t *d_counts, uint32_t *d_counts2, void *stream);
+
+/* Host: the length-ordered layout of n strings with host offsets
+ * h_off[0..n]: within each block of `block` slots (a positive multiple of
+ * 64; the last block may be short) the strings are sorted by length, stably,
+ * into h_order[0..n) (slot c holds string h_order[c], a string of the same
+ * block), and h_gbase[0..ceil(n/64)] receives the group bases of the strings
+ * in that order. Returns the number of dwords of the words array, or a
+ * negative status. No counterpart in the reference: a wave of an exact search
+ * runs as long as its longest string, so lanes of equal lengths waste
+ * nothing. */
+int64_t bl_stripes_layout_sorted(const int64_t *h_off, int64_t n, int64_t block,
+                                 int32_t *h_order, int64_t *h_gbase);
+
+/* Device: bl_stripes_build for the slot order d_order (from
+ * bl_stripes_layout_sorted, uploaded): slot c gets string d_order[c]. */
+int bl_stripes_build_order(const uint8_t *d_buf, const int64_t *d_off, int64_t n,
+                           const int32_t *d_order, const int64_t *d_gbase, uint32_t *d_words,
+                           int32_t *d_len, uint32_t *d_counts, uint32_t *d_counts2,
+                           void *stream);
 
 /* dst[r][i] = src[r][inv[i]] for r < rows, i < cols (dtype BL_INT32 /
  * BL_UINT16 / BL_UINT8; src and dst distinct). No counterpart in the
@@ -177,16 +204,21 @@ int bl_long(int is_pair, const uint8_t *d_buf_a, const int64_t *d_off_a,
 typedef struct {
     int waves_per_cu;      /* persistent waves per compute unit (default 16) */
     int64_t tile;          /* choices per (query, block) tile (default 8192) */
-    int reserved[4];       /* [0]: lane mode -- 0 compacted passes (default), 1 strided
+    int reserved[4];       /* [0]: lane mode -- 0 automatic (5 for an exact search
+                              with no per-query thresholds, else 6), 1 strided
                               lanes, 2 four-choice byte filter for k <= 3 (queries of
                               8 to 256 bytes; other queries run compacted), 3 gap
                               windows for k >= 0 (the length test on 256 choices at
                               a time, the in-band ones scanned 64 at a time), 4 count
                               windows for k >= 0 (needs the choices' class profiles,
-                              else mode 0): a tile is a choice block against a group
+                              else mode 6): a tile is a choice block against a group
                               of queries, the length and class-count bounds settle
                               most choices, the rest are rescanned with per-lane
-                              queries;
+                              queries; 5 exact scans: one workgroup per (query,
+                              block of `tile` choices; default sized from the batch),
+                              the query's whole Peq table in LDS, each lane scanning
+                              its choice in 32-row units (any threshold; the only
+                              mode for choices in length order); 6 compacted passes;
                               [1]: tiles per work-queue ticket (default 1);
                               [2]: count mode: queries per tile (default 16, at most 64);
                               [3]: count mode options: bit 0 switches off the

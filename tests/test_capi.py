@@ -227,6 +227,44 @@ def test_stripes_layout_on_host():
     assert _capi.call("bl_stripes_layout", off.ctypes.data, 0, g.ctypes.data) == 0
 
 
+@pytest.mark.parametrize("block", [64, 128, 4096])
+def test_stripes_layout_sorted_on_host(block):
+    """bl_stripes_layout_sorted: within each block of slots the strings are
+    sorted by length, ties in string order (a stable sort, checked against
+    numpy's), no string leaves its block, and the group bases are those of
+    bl_stripes_layout over the strings in slot order. Lengths with a range
+    above the counting sort's (>= 65536) take the comparison sort."""
+    rng = np.random.default_rng(block)
+    for lens in (rng.integers(0, 130, 10_000), rng.integers(0, 3, 777),
+                 np.concatenate([rng.integers(0, 50, 300), [70_000], rng.integers(0, 9, 200)]),
+                 np.array([5], np.int64)):
+        lens = lens.astype(np.int64)
+        n = len(lens)
+        off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        order = np.empty(n, np.int32)
+        g = np.empty((n + 63) // 64 + 1, np.int64)
+        total = _capi.call("bl_stripes_layout_sorted", off.ctypes.data, n, block, order.ctypes.data,
+                           g.ctypes.data)
+        for b0 in range(0, n, block):
+            sl = slice(b0, min(b0 + block, n))
+            want = b0 + np.argsort(lens[sl], kind="stable")
+            assert order[sl].tolist() == want.tolist()
+        soff = np.concatenate([[0], np.cumsum(lens[order])]).astype(np.int64)
+        g2 = np.empty_like(g)
+        assert total == _capi.call("bl_stripes_layout", soff.ctypes.data, n, g2.ctypes.data)
+        assert g.tolist() == g2.tolist()
+    g = np.empty(4, np.int64)
+    o = np.empty(4, np.int32)
+    off = np.array([0, 1, 2, 3], np.int64)
+    for bad_block in (0, 63, 100):
+        with pytest.raises(ValueError):
+            _capi.call("bl_stripes_layout_sorted", off.ctypes.data, 3, bad_block, o.ctypes.data,
+                       g.ctypes.data)
+    bad = np.array([0, 5, 3], np.int64)
+    with pytest.raises(ValueError):
+        _capi.call("bl_stripes_layout_sorted", bad.ctypes.data, 2, 64, o.ctypes.data, g.ctypes.data)
+
+
 def test_match_list_needs_a_threshold():
     with pytest.raises(ValueError):
         hip.cdist_matches(["abc"], ["abd"], None)

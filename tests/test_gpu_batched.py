@@ -80,12 +80,12 @@ def test_band_tiles_on_uneven_pairs(band):
         np.testing.assert_array_equal(out.get(), np.where(kv < 0, exact, np.minimum(exact, kv + 1)))
 
 
-@pytest.mark.parametrize("gen,k", [(4, 8), (2, 2), (2, 0)])
+@pytest.mark.parametrize("gen,k", [(4, 8), (2, 2), (2, 0), (5, -1)])
 def test_every_cdist_launch_shape_gives_the_same_results(gen, k):
     pq, pc = W.cdist(gen, 12, 5000, seed=3)
     Q, C = hip.Packed(*pq), hip.Packed(*pc)
     want = O.cross(pq, pc, k=k, threads=8)
-    space = {"mode": ["strided", "queue", "swar", "gap", "count"], "waves_per_cu": [4, 32],
+    space = {"mode": ["strided", "queue", "swar", "gap", "count", "exact"], "waves_per_cu": [4, 32],
              "tile": [64, 1000, 8192]}
     for p in autotune._grid(space):
         out = hip.DeviceArray.empty((Q.n, C.n), np.uint8)
@@ -268,7 +268,7 @@ def test_filter_mode_with_every_choice_open():
             np.testing.assert_array_equal(out.get(), want, err_msg=f"{kq} {p}")
 
 
-@pytest.mark.parametrize("mode", ["swar", "queue", "strided", "gap", "count"])
+@pytest.mark.parametrize("mode", ["swar", "queue", "strided", "gap", "count", "exact"])
 def test_match_list_equals_the_dense_result(mode):
     """bl_cdist_matches returns exactly the cells the dense cdist puts at or
     below k (here with per-query thresholds, one of them negative: that query
@@ -286,6 +286,73 @@ def test_match_list_equals_the_dense_result(mode):
         assert got == want, (mode, cap, len(got), len(want))
     q, c, d = hip.cdist_matches([b"kitten", b"flaw"], [b"sitting", b"lawn", b"kitten"], 2)
     assert (q.tolist(), c.tolist(), d.tolist()) == ([0, 1], [2, 1], [0, 2])
+
+
+def _exact_call(Q, C, out, k, stripes, params=None, kvec=None):
+    """bl_cdist_tuned straight through the C ABI on the given stripes."""
+    mq = min(Q.max_len, hip.BATCHED_MAX_QUERY)
+    ws = hip.DeviceBuffer(call("bl_cdist_workspace_size", Q.n, mq))
+    call("bl_cdist_tuned", Q.d_buf.ptr, Q.d_off.ptr, Q.n, stripes.ref, mq, out.ptr,
+         hip.DTYPE_CODE[out.dtype], int(k), kvec.ptr if kvec else None, ws.ptr, ws.nbytes,
+         hip._cdist_params(params), None)
+    call("bl_stream_sync", None)
+
+
+@pytest.mark.parametrize("nc", [1, 63, 65, 3000])
+def test_exact_scans_every_unit_count(nc):
+    """Exact-scan mode (5): queries of 1-512 bytes (every count of 32-row
+    units, u64 blocks with and without a u32 tail) against choices of 0-300
+    bytes, on the set's own stripes and on stripes in length order (results in
+    the caller's columns), in every dtype, exact and with per-query thresholds
+    (some exact); the automatic mode of an exact search is this one, and the
+    plain stripes kernel refuses a set in length order."""
+    rng = np.random.default_rng(nc)
+    ms = [1, 2, 31, 32, 33, 64, 65, 96, 97, 128, 129, 160, 161, 200, 256, 257, 300, 383, 400, 511, 512]
+    Qs = [bytes(rng.integers(0, 4, m).astype(np.uint8) + 97) for m in ms] + [b""]
+    Cs = []
+    for i in range(nc):
+        q = Qs[i % (len(Qs) - 1)]
+        if i % 4 == 0:
+            b = bytearray(q[: 300])
+            for _ in range(int(rng.integers(0, 8))):
+                b[int(rng.integers(0, len(b)))] = 97 + int(rng.integers(0, 4))
+            Cs.append(bytes(b))
+        else:
+            Cs.append(bytes(rng.integers(0, 4, int(rng.integers(0, 301))).astype(np.uint8) + 97))
+    pq, pc = O.pack(Qs), O.pack(Cs)
+    Q, C = hip.Packed(*pq), hip.Packed(*pc)
+    exact = O.cross(pq, pc, threads=8)
+    kq = rng.choice([-1, 0, 3, 17, 300], size=Q.n).astype(np.int32)
+    kv = hip.DeviceBuffer.from_host(kq)
+    capped = np.where(kq[:, None] < 0, exact, np.minimum(exact, kq[:, None] + 1))
+    for order in ("set", "length"):
+        st = C.stripes(None, order)
+        for tile in (64, 448, 4096):
+            for dt in (np.int32, np.uint16, np.uint8):
+                out = hip.DeviceArray.empty((Q.n, C.n), dt)
+                _exact_call(Q, C, out, -1, st, {"mode": "exact", "tile": tile})
+                np.testing.assert_array_equal(out.get(), O.saturate(exact, dt),
+                                              err_msg=f"{order} tile={tile} {dt}")
+            out = hip.DeviceArray.empty((Q.n, C.n), np.int32)
+            _exact_call(Q, C, out, -1, st, {"mode": "exact", "tile": tile}, kvec=kv)
+            np.testing.assert_array_equal(out.get(), capped, err_msg=f"{order} tile={tile} kvec")
+        out = hip.DeviceArray.empty((Q.n, C.n), np.int32)
+        _exact_call(Q, C, out, -1, st)                  # automatic mode
+        np.testing.assert_array_equal(out.get(), exact, err_msg=f"{order} default")
+    # the public path: an exact search runs the exact scans on the length-ordered set
+    out = hip.DeviceArray.empty((Q.n, C.n), np.int32)
+    hip.cdist_batched_into(Q, C, out, -1, None)
+    np.testing.assert_array_equal(out.get(), exact)
+    got = hip.cdist([bytes(q) for q in Qs], [bytes(c) for c in Cs], dtype=np.uint16)
+    np.testing.assert_array_equal(got, O.saturate(exact, np.uint16))
+    st = C.stripes(None, "length")
+    # a set already in length order (one string here) keeps its own stripes
+    assert bool(st.desc.order) == (nc > 1)
+    if st.desc.order:
+        qidx = hip.DeviceBuffer.from_host(np.arange(2, dtype=np.int32))
+        with pytest.raises(ValueError):
+            call("bl_cdist_class_stripes", 1, Q.d_buf.ptr, Q.d_off.ptr, qidx.ptr, 2, st.ref, out.ptr,
+                 hip.DTYPE_CODE[out.dtype], -1, None)
 
 
 def test_per_item_thresholds():

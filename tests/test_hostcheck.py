@@ -243,6 +243,37 @@ def test_32_row_unit_lanes(hc):
                     assert got == want[i], (i, len(a), len(b), k, trim, w)
 
 
+def test_exact_search_unit_lanes(hc):
+    """The exact-search kernel's lane (units_lane: the query's Peq table
+    word-major as in its LDS copy, the choice read from stripes) equals the
+    oracle's cross result for query lengths at and around every 32-row unit
+    edge up to 512 (u32 block, u64 blocks, u64 blocks + u32 tail), texts from
+    empty to longer than the query, near copies and unrelated strings, exact
+    and thresholded. An off-by-one in the unit count, the tail's dword or the
+    word-major index fails here."""
+    hc.blh_units.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                             ctypes.c_int]
+    rng = np.random.default_rng(71)
+    Q, C = [], []
+    for m in (1, 2, 31, 32, 33, 63, 64, 65, 95, 96, 97, 127, 128, 129, 159, 160, 161, 191, 192,
+              193, 255, 256, 257, 300, 383, 384, 385, 449, 480, 511, 512):
+        for alpha in (4, 26, 256):
+            base = 97 if alpha <= 26 else 0
+            q = bytes(rng.integers(0, alpha, m).astype(np.uint8) + base)
+            Q.append(q)
+            C.append(_near(rng, q, int(rng.integers(0, 6)), min(alpha, 26)))
+            C.append(bytes(rng.integers(0, alpha, int(rng.integers(0, m + 20))).astype(np.uint8)
+                           + base))
+    C.append(b"")
+    pq, pc = O.pack(Q), O.pack(C)
+    for k in (-1, 0, 2, 7, 40):
+        want = O.cross(pq, pc, k=k)
+        for i, q in enumerate(Q):
+            for j in range(i % 4, len(C), 4):          # a quarter of the grid, every text shape
+                got = hc.blh_units(q, len(q), C[j], len(C[j]), k)
+                assert got == want[i, j], (len(q), len(C[j]), k, got, want[i, j])
+
+
 def _dp_cell(a: bytes, b: bytes, i: int, j: int) -> int:
     """Wagner-Fischer D[i][j] of pattern a against text b."""
     prev = list(range(i + 1))
