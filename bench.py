@@ -90,7 +90,7 @@ def parse():
     ap.add_argument("--serving-requests", type=int, default=600)
     ap.add_argument("--serving-load", type=float, default=0.75,
                     help="offered load as a fraction of the measured batched-offline throughput")
-    ap.add_argument("--serving-slots", type=int, default=4)
+    ap.add_argument("--serving-slots", type=int, default=8)
     ap.add_argument("--ingest-pairs", type=int, default=1 << 21,
                     help="pairs workloads run on their own: end-to-end leg from Python lists of "
                          "this many pairs (packing, PCIe copies, kernels, results back), staged "
@@ -945,6 +945,21 @@ def serving_leg(args, seed):
     hs_on, got_on, t_on = run(arrivals)
     batches = srv.stats["batches"] - batches0
     lat = np.array([h.latency for h in hs_on]) * 1e3
+    kinds = np.array([r["kind"] if r["kind"] == "pairwise" or r["k"] is not None else "cdist-exact"
+                      for r in reqs])
+    by_kind = {}
+    wait = np.array([h.t_admit - h.t_submit for h in hs_on]) * 1e3
+    for kd in ("pairwise", "cdist", "cdist-exact"):
+        sel = kinds == kd
+        if sel.any():
+            by_kind[kd] = {"requests": int(sel.sum()),
+                           "p50_ms": round(float(np.percentile(lat[sel], 50)), 3),
+                           "p99_ms": round(float(np.percentile(lat[sel], 99)), 3),
+                           "wait_p50_ms": round(float(np.percentile(wait[sel], 50)), 3),
+                           "wait_p99_ms": round(float(np.percentile(wait[sel], 99)), 3),
+                           "run_p50_ms": round(float(np.percentile((lat - wait)[sel], 50)), 3),
+                           "run_p99_ms": round(float(np.percentile((lat - wait)[sel], 99)), 3),
+                           "pairs": int(sum(sizes[i] for i in np.nonzero(sel)[0]))}
     mix_equal = all(np.array_equal(got_on[i], got_off[i]) for i in keep)
     oracle_equal, oracle_checked = _serving_oracle(reqs, sizes, got_on, sets)
     srv.close()
@@ -958,6 +973,7 @@ def serving_leg(args, seed):
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "mean_ms": round(float(lat.mean()), 3),
         "max_ms": round(float(lat.max()), 3),
+        "by_kind": by_kind,
         "requests": len(reqs),
         "pairs": int(total),
         "batches": int(batches),

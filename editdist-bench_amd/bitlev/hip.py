@@ -115,6 +115,36 @@ class Packed:
     def from_strings(cls, strings, stream=None) -> "Packed":
         return cls(*pack(strings), stream=stream)
 
+    @classmethod
+    def from_parts(cls, parts, stream=None) -> "Packed":
+        """One resident set of several packed host sets (buf, off) in turn,
+        each part's bytes copied straight to its place in the device buffer
+        (no host concatenation: a fresh 50 MB host array costs more in page
+        faults than its upload). The parts must stay alive and unchanged
+        until `stream` has run the copies."""
+        self = cls.__new__(cls)
+        offs, base = [np.zeros(1, np.int64)], 0
+        spans = []
+        for buf, off in parts:
+            off = np.asarray(off, np.int64)
+            n = int(off[-1] - off[0])
+            spans.append((buf, int(off[0]), n, base))
+            offs.append(off[1:] - off[0] + base)
+            base += n
+        self.h_off = np.concatenate(offs)
+        self.n = len(self.h_off) - 1
+        self.lengths = np.diff(self.h_off)
+        self.max_len = int(self.lengths.max()) if self.n else 0
+        self.nbytes = max(base, 1)
+        self.d_buf = DeviceBuffer(self.nbytes)
+        for buf, o, n, at in spans:
+            if n:
+                self.d_buf.upload(np.asarray(buf, np.uint8)[o:o + n], stream, at)
+        self.d_off = DeviceBuffer.from_host(self.h_off, stream)
+        self._stripes = None
+        self._sorted = None
+        return self
+
     def stripes(self, stream=None, order: str = "set") -> "ChoiceStripes":
         """The set as search-kernel stripes (built on the device on first use,
         on `stream`, which is then synchronised; kept with the set).

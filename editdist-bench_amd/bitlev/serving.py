@@ -1,6 +1,6 @@
 """Continuous-batching request loop on one GPU.
 
-    srv = Server(slots=4, budget_bytes=32 << 30)
+    srv = Server(slots=8, budget_bytes=32 << 30)
     srv.add_set("names", strings)                    # packed once, resident in HBM
     r1 = srv.submit_cdist(queries, "names", max_dist=2, dtype="uint8")
     r2 = srv.submit_pairwise(a, b, max_dist=None, dtype="int32")
@@ -11,8 +11,12 @@ Requests queue up; a scheduler thread admits them into free slots (one HIP
 stream each) as slots and memory free up, so work of several requests runs
 on the device at once. Pending requests of the same kind, resident set and
 output dtype are fused into one launch of the batched kernels (a per-pair /
-per-query threshold vector lets requests with different max_dist share it);
-each request gets its own view of the fused output. Admission is by HBM
+per-query threshold vector lets requests with different max_dist share it;
+exact searches fuse only with exact searches); each request gets its own
+view of the fused output, and the fused inputs are uploaded part by part,
+never concatenated on the host. Eight slots by default: on the serving mix
+at 75 % load, p99 fell from 22 ms with four to 14.5 ms with eight (offline
+throughput 41 -> 37 G pairs/s, less fusion). Admission is by HBM
 footprint: a batch is admitted only when its inputs, outputs and workspace fit
 in the budget next to everything still held (live results included); a
 request that can never fit is rejected with MemoryError.
@@ -108,7 +112,7 @@ class Server:
     workspace; resident sets are counted too); max_batch_pairs: cap on the
     distances fused into one launch."""
 
-    def __init__(self, slots: int = 4, budget_bytes: int = 32 << 30,
+    def __init__(self, slots: int = 8, budget_bytes: int = 32 << 30,
                  max_batch_pairs: int = 256 << 20, executor=None):
         if slots < 1:
             raise ValueError("slots must be >= 1")
@@ -321,7 +325,7 @@ class _GpuExecutor:
     query and choice block, the whole Peq table in LDS, 32-row units) over
     the set, which is kept in length order."""
 
-    def __init__(self, slots: int = 4):
+    def __init__(self, slots: int = 8):
         wpc = max(4, 32 // max(1, int(slots)))
         self.wpc = wpc
         self.cdist_params = {"mode": "swar", "waves_per_cu": wpc, "tile": 8192}
@@ -381,10 +385,9 @@ class _GpuExecutor:
     def run(self, server, slot, batch):
         dt = batch[0].dtype
         if batch[0].kind == "pairwise":
-            pa = _concat([r.payload[0] for r in batch])
-            pb = _concat([r.payload[1] for r in batch])
             kv = np.concatenate([np.full(r.pairs, r.k, np.int32) for r in batch])
-            A, B = hip.Packed(*pa, stream=slot.stream), hip.Packed(*pb, stream=slot.stream)
+            A = hip.Packed.from_parts([r.payload[0] for r in batch], slot.stream)
+            B = hip.Packed.from_parts([r.payload[1] for r in batch], slot.stream)
             kvec = DeviceBuffer.from_host(kv, slot.stream)
             out = DeviceArray.empty((A.n,), dt)
             hip.pairs_batched_into(A, B, out, -1, batch[0].trim, slot.stream, slot.workspace,
@@ -395,10 +398,9 @@ class _GpuExecutor:
                      for r, o in zip(batch, offs)]
         else:
             C = server.sets[batch[0].set_name]
-            pq = _concat([r.payload for r in batch])
             nqs = [len(r.payload[1]) - 1 for r in batch]
             kv = np.concatenate([np.full(q, r.k, np.int32) for r, q in zip(batch, nqs)])
-            Q = hip.Packed(*pq, stream=slot.stream)
+            Q = hip.Packed.from_parts([r.payload for r in batch], slot.stream)
             kvec = DeviceBuffer.from_host(kv, slot.stream)
             out = DeviceArray.empty((Q.n, C.n), dt)
             inv = getattr(C, "col_order", None)
@@ -452,15 +454,3 @@ def _gather_strings(buf, off, order, chunk=1 << 16):
         pos = np.arange(noff[a], noff[a] + tot, dtype=np.int64)
         nbuf[pos] = src[starts + pos]
     return nbuf, noff
-
-
-def _concat(parts):
-    """Concatenate packed host sets (buf, off)."""
-    bufs, offs, base = [], [np.zeros(1, np.int64)], 0
-    for buf, off in parts:
-        n = int(off[-1] - off[0])
-        bufs.append(np.asarray(buf[off[0]:off[-1]], np.uint8))
-        offs.append(np.asarray(off[1:], np.int64) - off[0] + base)
-        base += n
-    buf = np.concatenate(bufs) if base else np.zeros(1, np.uint8)
-    return buf, np.concatenate(offs)

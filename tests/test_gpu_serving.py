@@ -75,3 +75,29 @@ def test_each_result_is_the_same_alone_and_in_the_mix():
 def hip_list(packed):
     buf, off = packed
     return [bytes(buf[off[i]:off[i + 1]]) for i in range(len(off) - 1)]
+
+
+def test_packed_from_parts_equals_the_concatenated_set():
+    """Packed.from_parts (each part's bytes copied to its place on the device,
+    no host concatenation) holds exactly the set of the parts in turn: parts
+    whose offsets do not start at zero (views into a larger set), empty parts
+    and empty strings; a search over it equals the oracle on the joined set."""
+    rng = np.random.default_rng(21)
+    S = [bytes(rng.integers(97, 101, int(rng.integers(0, 40))).astype(np.uint8)) for _ in range(300)]
+    whole = O.pack(S)
+    buf, off = whole
+    cuts = [0, 7, 7, 120, 299, 300]
+    parts = [(buf, off[a:b + 1]) for a, b in zip(cuts[:-1], cuts[1:])]
+    stream = hip.Stream()
+    P = hip.Packed.from_parts(parts, stream.ptr)
+    stream.sync()
+    assert P.n == len(S) and P.h_off.tolist() == off.tolist()
+    got = np.empty(max(int(off[-1]), 1), np.uint8)
+    from bitlev._capi import call
+    call("bl_memcpy_dtoh", got.ctypes.data, P.d_buf.ptr, int(off[-1]), None)
+    call("bl_stream_sync", None)
+    assert bytes(got[: off[-1]]) == bytes(buf[: off[-1]])
+    Q = hip.Packed(*O.pack(S[:5]))
+    out = hip.DeviceArray.empty((Q.n, P.n), np.int32)
+    hip.cdist_batched_into(Q, P, out, 3)
+    np.testing.assert_array_equal(out.get(), O.cross(O.pack(S[:5]), whole, k=3))
