@@ -145,12 +145,14 @@ class Packed:
         self._sorted = None
         return self
 
-    def stripes(self, stream=None, order: str = "set") -> "ChoiceStripes":
+    def stripes(self, stream=None, order: str = "set", grams3: bool = False) -> "ChoiceStripes":
         """The set as search-kernel stripes (built on the device on first use,
         on `stream`, which is then synchronised; kept with the set).
         order="set": slot c holds string c, with the byte-class and 2-gram
-        profiles (every search mode); order="length": sorted by length within
-        blocks of SORT_BLOCK strings, no profiles (exact scans only)."""
+        profiles (every search mode; grams3 adds the 3-gram profiles, and
+        rebuilds a layout made without them); order="length": sorted by
+        length within blocks of SORT_BLOCK strings, no profiles (exact scans
+        only)."""
         if order == "length":
             if self._sorted is None:
                 if self.n < 2 or bool(np.all(self.lengths[1:] >= self.lengths[:-1])):
@@ -161,8 +163,8 @@ class Packed:
             return self._sorted
         if order != "set":
             raise ValueError("order must be 'set' or 'length'")
-        if self._stripes is None:
-            self._stripes = ChoiceStripes(self, stream)
+        if self._stripes is None or (grams3 and self._stripes.d_counts3 is None):
+            self._stripes = ChoiceStripes(self, stream, grams3=grams3)
         return self._stripes
 
     @property
@@ -177,10 +179,16 @@ class ChoiceStripes:
     """A packed set interleaved by dword in groups of 64 strings (bl_stripes):
     the layout the cdist kernels read, so that a wave's lanes load 256
     contiguous bytes per step, plus each string's 16-byte byte-class and
-    2-gram profiles (the batched searches' lower bounds; profiles=False leaves
-    them out)."""
+    2-gram profiles (the batched searches' lower bounds; profiles=False
+    leaves them out) and, with grams3, the 64-byte 3-gram profiles (a third
+    bound for small alphabets; off by default: on the 256 x 4.2M DNA search
+    at k = 8 it settles what the 2-gram bound leaves but its per-pair profile
+    loads cost as much as the scans they save, 1.83 -> 1.81 ms, and the
+    set's build takes 14 ms instead of a few). sort_block > 0: the strings in
+    length order within blocks of that many (exact scans)."""
 
-    def __init__(self, P: Packed, stream=None, profiles: bool = True, sort_block: int = 0):
+    def __init__(self, P: Packed, stream=None, profiles: bool = True, sort_block: int = 0,
+                 grams3: bool = False):
         n = P.n
         gbase = np.empty((n + 63) // 64 + 1, np.int64)
         self.d_order = None
@@ -196,23 +204,24 @@ class ChoiceStripes:
         self.d_len = DeviceBuffer(max(n, 1) * 4)
         self.d_counts = DeviceBuffer(max(n, 1) * 16) if profiles else None
         self.d_counts2 = DeviceBuffer(max(n, 1) * 16) if profiles else None
+        self.d_counts3 = DeviceBuffer(max(n, 1) * 64) if profiles and grams3 else None
         counts = self.d_counts.ptr if profiles else None
         counts2 = self.d_counts2.ptr if profiles else None
-        if sort_block:
-            call("bl_stripes_build_order", P.d_buf.ptr, P.d_off.ptr, n, self.d_order.ptr,
-                 self.d_gbase.ptr, self.d_words.ptr, self.d_len.ptr, counts, counts2, stream)
-        else:
-            call("bl_stripes_build", P.d_buf.ptr, P.d_off.ptr, n, self.d_gbase.ptr, self.d_words.ptr,
-                 self.d_len.ptr, counts, counts2, stream)
+        counts3 = self.d_counts3.ptr if self.d_counts3 is not None else None
+        call("bl_stripes_build_order", P.d_buf.ptr, P.d_off.ptr, n,
+             self.d_order.ptr if sort_block else None, self.d_gbase.ptr, self.d_words.ptr,
+             self.d_len.ptr, counts, counts2, counts3, stream)
         # built once per set; finish here so that any stream may read them next
         call("bl_stream_sync", stream)
         self.nbytes = self.d_gbase.nbytes + self.d_words.nbytes + self.d_len.nbytes
         if profiles:
             self.nbytes += self.d_counts.nbytes + self.d_counts2.nbytes
+        if self.d_counts3 is not None:
+            self.nbytes += self.d_counts3.nbytes
         if sort_block:
             self.nbytes += self.d_order.nbytes
         self.desc = Stripes(self.d_words.ptr, self.d_gbase.ptr, self.d_len.ptr, n, counts, counts2,
-                            self.d_order.ptr if sort_block else None)
+                            self.d_order.ptr if sort_block else None, counts3)
 
     @property
     def ref(self):

@@ -68,6 +68,24 @@ __device__ __forceinline__ int64_t uni64(int64_t v)
 }
 
 /* Make this wave's LDS writes visible to its own later LDS reads by other lanes. */
+__device__ __forceinline__ int wave_min(int v)
+{
+    for (int o = 32; o; o >>= 1) {
+        const int x = __shfl_xor(v, o);
+        v = x < v ? x : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int wave_max(int v)
+{
+    for (int o = 32; o; o >>= 1) {
+        const int x = __shfl_xor(v, o);
+        v = x > v ? x : v;
+    }
+    return v;
+}
+
 __device__ __forceinline__ void wave_lds_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -218,11 +236,13 @@ struct WaveLds {
 __global__ __launch_bounds__(256) void peq_build_kernel(const u8 *__restrict__ bq,
                                                         const int64_t *__restrict__ oq, int nwmax,
                                                         u64 *__restrict__ peq, Profile *__restrict__ qcnt,
-                                                        Profile *__restrict__ qcnt2)
+                                                        Profile *__restrict__ qcnt2,
+                                                        Profile3 *__restrict__ qcnt3)
 {
     __shared__ u8 qs[512];
     __shared__ u32 cls[16];
     __shared__ u32 cls2[16];
+    __shared__ u32 cls3[64];
     const int64_t q = blockIdx.x;
     const int64_t q0 = oq[q];
     int m = (int)(oq[q + 1] - q0);
@@ -232,6 +252,7 @@ __global__ __launch_bounds__(256) void peq_build_kernel(const u8 *__restrict__ b
         cls[threadIdx.x] = 0;
         cls2[threadIdx.x] = 0;
     }
+    if (threadIdx.x < 64) cls3[threadIdx.x] = 0;
     __syncthreads();
     const u32 c = threadIdx.x;
     u64 *row = peq + ((size_t)q * 256 + c) * nwmax;
@@ -250,7 +271,10 @@ __global__ __launch_bounds__(256) void peq_build_kernel(const u8 *__restrict__ b
     if (qcnt2)                               /* the 2-gram profile, from the bytes */
         for (int i = threadIdx.x; i + 1 < m; i += 256)
             atomicAdd(&cls2[4u * class2(qs[i]) + class2(qs[i + 1])], 1u);
+    if (qcnt3)                               /* and the 3-gram profile */
+        for (int i = threadIdx.x; i + 2 < m; i += 256) atomicAdd(&cls3[class3(qs[i], qs[i + 1], qs[i + 2])], 1u);
     __syncthreads();
+    if (qcnt3 && threadIdx.x == 0) qcnt3[q] = pack_profile3([&](int x) { return cls3[x]; });
     if (threadIdx.x < 4) {
         u32 v = 0, v2 = 0;
         for (int i = 0; i < 4; i++) {
@@ -958,8 +982,14 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
      * query lengths within a byte, aligned rows -- then full windows whose
      * choices fit a byte run a loop with nothing to decide per query */
     const u64 group_mask = gq >= 64 ? ~0ull : ((1ull << gq) - 1ull);
+    /* lengths enter the byte sums as (length - lbase): lbase = the group's
+     * shortest query less 128 (at least 0), so strings of 200-256 bytes fit
+     * a byte too; a query or a window outside [lbase, lbase + 255] takes the
+     * general loop */
+    const int lbase = uni(max(0, wave_min(lane < gq && lm > 0 ? lm : 0x7fffffff) - 128));
+    const int lmb = lm - lbase;
     const bool fast_tile = okmask == group_mask && uniform_k &&
-                           wave_ballot(lane < gq && lm > 255) == 0 && (!DENSE || rows_aligned);
+                           wave_ballot(lane < gq && (u32)lmb > 255u) == 0 && (!DENSE || rows_aligned);
     const u32 k2_uni = 2u * (u32)k_uni;
     const u32 fill_uni = out_k1<OutT>(k_uni) * (sizeof(OutT) == 1 ? 0x01010101u
                                                 : sizeof(OutT) == 2 ? 0x00010001u : 1u);
@@ -1115,12 +1145,12 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
         }
         bool big = false;
 #pragma unroll
-        for (int s = 0; s < 4; s++) big = big || n[s] > 255;
+        for (int s = 0; s < 4; s++) big = big || (valid[s] && (u32)(n[s] - lbase) > 255u);
         const bool small = wave_ballot(big) == 0;
         int64_t orow = q0 * nc;
         if (FULL && fast_tile && small) {
             for (int j = 0; j < gq; j++, orow += nc) {
-                const int mq = __builtin_amdgcn_readlane(lm, j);
+                const int mqb = __builtin_amdgcn_readlane(lmb, j);
                 const Profile pq{(u32)__builtin_amdgcn_readlane((int)lp.x, j),
                                  (u32)__builtin_amdgcn_readlane((int)lp.y, j),
                                  (u32)__builtin_amdgcn_readlane((int)lp.z, j),
@@ -1131,7 +1161,7 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
                 for (int s = 0; s < 4; s++) {
                     t[s] = sad_u8(pq.x, cp[s].x, 0u);
                     t[s] = sad_u8(pq.y, cp[s].y, t[s]);
-                    t[s] = sad_u8((u32)n[s], (u32)mq, t[s]);
+                    t[s] = sad_u8((u32)(n[s] - lbase), (u32)mqb, t[s]);
                     half = half || t[s] <= k2_uni;
                 }
                 if (DENSE) {
@@ -1169,7 +1199,7 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
             const u32 k2 = 2u * (u32)kq;                 /* kq <= 2^30 */
             /* the open choices as wave masks, straight from the compares */
             u64 om[4];
-            if (small && mq <= 255) {
+            if (small && (u32)(mq - lbase) <= 255u) {
                 /* classes 0-7 and the gap first: a partial L1 is still a lower
                  * bound, and it settles most choices; classes 8-15 only when a
                  * lane of the wave is left open */
@@ -1179,7 +1209,7 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
                 for (int s = 0; s < 4; s++) {
                     t[s] = sad_u8(pq.x, cp[s].x, 0u);
                     t[s] = sad_u8(pq.y, cp[s].y, t[s]);
-                    t[s] = sad_u8((u32)n[s], (u32)mq, t[s]);
+                    t[s] = sad_u8((u32)(n[s] - lbase), (u32)(mq - lbase), t[s]);
                     half = half || (valid[s] && t[s] <= k2);
                 }
                 if (wave_ballot(half) == 0) {
@@ -1201,7 +1231,7 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
                     t = sad_u8(pq.y, cp[s].y, t);
                     t = sad_u8(pq.z, cp[s].z, t);
                     t = sad_u8(pq.w, cp[s].w, t);
-                    t += ((mq <= 255) & (n[s] <= 255)) ? gap : 0u;
+                    t += gap;                        /* (saturation only shrinks L1) */
                     om[s] = wave_ballot(valid[s] && gap <= (u32)kq && t <= k2);
                 }
             }
@@ -1261,7 +1291,7 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
         }
         bool big = false;
 #pragma unroll
-        for (int s = 0; s < 8; s++) big = big || n[s] > 255;
+        for (int s = 0; s < 8; s++) big = big || (u32)(n[s] - lbase) > 255u;
         if (wave_ballot(big) != 0) {
             window(w, std::true_type());
             window(w + 256, std::true_type());
@@ -1269,7 +1299,7 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
         }
         int64_t orow = q0 * nc;
         for (int j = 0; j < gq; j++, orow += nc) {
-            const int mq = __builtin_amdgcn_readlane(lm, j);
+            const int mqb = __builtin_amdgcn_readlane(lmb, j);
             const Profile pq{(u32)__builtin_amdgcn_readlane((int)lp.x, j),
                              (u32)__builtin_amdgcn_readlane((int)lp.y, j),
                              (u32)__builtin_amdgcn_readlane((int)lp.z, j),
@@ -1280,7 +1310,7 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
             for (int s = 0; s < 8; s++) {
                 t[s] = sad_u8(pq.x, cp[s].x, 0u);
                 t[s] = sad_u8(pq.y, cp[s].y, t[s]);
-                t[s] = sad_u8((u32)n[s], (u32)mq, t[s]);
+                t[s] = sad_u8((u32)(n[s] - lbase), (u32)mqb, t[s]);
                 half = half || t[s] <= k2_uni;
             }
             if (DENSE) {
@@ -1338,7 +1368,8 @@ __attribute__((amdgpu_waves_per_eu(NWMAX == 4 ? 4 : 1))) void cdist_count_kernel
     const u32 *__restrict__ words, const int64_t *__restrict__ gbase,
     const int32_t *__restrict__ clen, const Profile *__restrict__ ccnt,
     const Profile *__restrict__ qcnt, const Profile *__restrict__ ccnt2,
-    const Profile *__restrict__ qcnt2, int64_t nc, Sink sink,
+    const Profile *__restrict__ qcnt2, const Profile3 *__restrict__ ccnt3,
+    const Profile3 *__restrict__ qcnt3, int64_t nc, Sink sink,
     int k_all, const int32_t *__restrict__ kvec, int64_t tile, int grab, int qgroup,
     unsigned long long *counter, int first_static)
 {
@@ -1425,14 +1456,16 @@ __attribute__((amdgpu_waves_per_eu(NWMAX == 4 ? 4 : 1))) void cdist_count_kernel
                 const int64_t c = idx - q * nc;
                 const int mq = (int)(oq[q + 1] - oq[q]);
                 const int kq = kvec ? clamp_k_dev(kvec[q]) : k_all;
-                keep = !count2_over(qcnt2[q], ccnt2[c], mq, clen[c], kq);
+                const int n = clen[c];
+                keep = !count2_over(qcnt2[q], ccnt2[c], mq, n, kq);
+                if (keep && ccnt3) keep = !count3_over(qcnt3[q], ccnt3[c], mq, n, kq);
             }
         }
         const u64 km = wave_ballot(keep);
-        if (GRAM) {
-            /* after the L2 prefilter the open pairs are mostly near matches,
-             * which the 32-row stage cannot settle: straight to the whole
-             * pattern (one 64-row word) */
+        if (GRAM || ccnt3) {
+            /* after the L2 prefilter (or the 3-gram bound) the open pairs are
+             * mostly near matches, which the 32-row stage cannot settle:
+             * straight to the whole pattern */
             if (keep) L.esc[ne + rank_below(km)] = idx;
             ne = uni(ne + __popcll(km));
             ns = uni(base);
@@ -1726,24 +1759,6 @@ __device__ void pairs_tile(const u8 *__restrict__ ba, const int64_t *__restrict_
     store_out(out, i, packed_pair_w32<W32>(ba, oa, bb, ob, i, trim, pair_k(k_all, kvec, i)), dtype);
 }
 
-__device__ __forceinline__ int wave_min(int v)
-{
-    for (int o = 32; o; o >>= 1) {
-        const int x = __shfl_xor(v, o);
-        v = x < v ? x : v;
-    }
-    return v;
-}
-
-__device__ __forceinline__ int wave_max(int v)
-{
-    for (int o = 32; o; o >>= 1) {
-        const int x = __shfl_xor(v, o);
-        v = x > v ? x : v;
-    }
-    return v;
-}
-
 /* Pair indices a wave's band tiles could not decide, scanned in full 64 at a
  * time by the same wave (its LDS stack), so those scans overlap the other
  * waves' band tiles. */
@@ -1976,7 +1991,8 @@ static int cdist_nwmax(int max_qlen)
 }
 
 /* Workspace: the work queues (QUEUE_BYTES), the queries' Peq tables,
- * then one 16-byte class profile and one 2-gram profile per query. */
+ * then one 16-byte class profile and one 2-gram profile per query, then one
+ * 64-byte 3-gram profile per query. */
 static size_t cdist_peq_end(int64_t nq, int max_qlen)
 {
     return QUEUE_BYTES + (size_t)nq * 256 * cdist_nwmax(max_qlen) * sizeof(u64);
@@ -1985,7 +2001,7 @@ static size_t cdist_peq_end(int64_t nq, int max_qlen)
 size_t bl_cdist_workspace_size(int64_t nq, int max_qlen)
 {
     if (nq < 0 || max_qlen < 0) return 0;
-    return cdist_peq_end(nq, max_qlen) + 2 * (size_t)nq * sizeof(Profile);
+    return cdist_peq_end(nq, max_qlen) + 2 * (size_t)nq * sizeof(Profile) + (size_t)nq * sizeof(Profile3);
 }
 
 }  /* extern "C" */
@@ -2045,10 +2061,13 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
     Profile *qcnt = ccnt ? (Profile *)(ws + cdist_peq_end(nq, max_qlen)) : nullptr;
     const Profile *ccnt2 = ccnt ? (const Profile *)choices->counts2 : nullptr;
     Profile *qcnt2 = ccnt2 ? (Profile *)(ws + cdist_peq_end(nq, max_qlen)) + nq : nullptr;
+    /* the 3-gram profiles after the 2-gram ones (64 bytes a query) */
+    const Profile3 *ccnt3 = ccnt2 ? (const Profile3 *)choices->counts3 : nullptr;
+    Profile3 *qcnt3 = ccnt3 ? (Profile3 *)((Profile *)(ws + cdist_peq_end(nq, max_qlen)) + 2 * nq) : nullptr;
     int rc = launch_clear(ctr, QUEUE_BYTES / 8, s);
     if (rc == BL_OK && extra_clear) rc = launch_clear(extra_clear, 1, s);
     if (rc == BL_OK) {
-        peq_build_kernel<<<(unsigned)nq, 256, 0, s>>>(d_buf_q, d_off_q, nwmax, gpeq, qcnt, qcnt2);
+        peq_build_kernel<<<(unsigned)nq, 256, 0, s>>>(d_buf_q, d_off_q, nwmax, gpeq, qcnt, qcnt2, qcnt3);
         auto capped = [&](auto kernel, size_t lds_bytes) {
             const int64_t cap = resident_blocks(kernel, 256, lds_bytes);
             return (unsigned)(blocks < cap ? blocks : cap);
@@ -2060,19 +2079,19 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
                 const size_t lb = (sizeof(CountLds<0>) + sizeof(GramLds)) * 4;
                 kern<<<capped(kern, lb), 256, lb, s>>>(
                     d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, ccnt2,
-                    qcnt2, nc, sink, k, d_kvec, tile, grab, qgroup, ctr, first_static);
+                    qcnt2, ccnt3, qcnt3, nc, sink, k, d_kvec, tile, grab, qgroup, ctr, first_static);
             } else if (mode == 4 && qgroup <= CD_GL) {
                 auto kern = cdist_count_kernel<NW, CD_GL, Sink>;
                 const size_t lb = sizeof(CountLds<CD_GL>) * 4;
                 kern<<<capped(kern, lb), 256, lb, s>>>(
                     d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, ccnt2,
-                    qcnt2, nc, sink, k, d_kvec, tile, grab, qgroup, ctr, first_static);
+                    qcnt2, ccnt3, qcnt3, nc, sink, k, d_kvec, tile, grab, qgroup, ctr, first_static);
             } else if (mode == 4) {
                 auto kern = cdist_count_kernel<NW, 0, Sink>;
                 const size_t lb = sizeof(CountLds<0>) * 4;
                 kern<<<capped(kern, lb), 256, lb, s>>>(
                     d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, ccnt2,
-                    qcnt2, nc, sink, k, d_kvec, tile, grab, qgroup, ctr, first_static);
+                    qcnt2, ccnt3, qcnt3, nc, sink, k, d_kvec, tile, grab, qgroup, ctr, first_static);
             } else if (mode == 5) {
                 cdist_exact_kernel<NW, Sink><<<(unsigned)(nq * xblk), 256, 0, s>>>(
                     d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, choices->order, nc,

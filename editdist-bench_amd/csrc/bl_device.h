@@ -1170,10 +1170,10 @@ BL_HD u32 swar_over4(u32 Pv, u32 Mv, int m, const int (&n)[4], int k)
  * max(a_c - b_c, 0), N = sum of max(b_c - a_c, 0). An insertion or deletion
  * changes one count by one, a substitution lowers one count and raises
  * another, so every edit lowers P by at most one and N by at most one; hence
- * d(a, b) >= max(P, N) = (L1 + |len a - len b|) / 2 with L1 = P + N. With
- * saturated counts, L1' <= L1 (clamping moves no two counts apart), so
- * d >= L1' / 2 always, and when both strings are at most 255 bytes no count
- * saturates and d >= (L1' + |len a - len b|) / 2. The bound is exact in the
+ * d(a, b) >= max(P, N) = (L1 + |len a - len b|) / 2 with L1 = P + N (P - N
+ * = len a - len b). With saturated counts L1' <= L1 (clamping moves no two
+ * counts apart) while the length gap comes from the lengths themselves, so
+ * d >= (L1' + |len a - len b|) / 2 at any length. The bound is exact in the
  * sense the results need: a pair it puts above k has d > k, and its stored
  * result min(d, k + 1) = k + 1 is what the scan would give. */
 struct alignas(16) Profile {
@@ -1202,8 +1202,7 @@ BL_HD bool count_over(const Profile &qc, const Profile &cc, int m, int n, int k)
     l1 = sad_u8(qc.z, cc.z, l1);
     l1 = sad_u8(qc.w, cc.w, l1);
     const u32 gap = (u32)(n > m ? n - m : m - n);
-    const u32 twice = l1 + (m <= 255 && n <= 255 ? gap : 0u);
-    return twice > 2u * (u32)k;              /* k <= 2^30: no overflow */
+    return (u64)l1 + gap > 2ull * (u64)k;
 }
 
 /* A second bound from 2-grams. Byte b falls into class c2(b) = (b >> 1) & 3
@@ -1213,8 +1212,8 @@ BL_HD bool count_over(const Profile &qc, const Profile &cc, int m, int n, int k)
  * counts against b's: a substitution removes two 2-grams and adds two, an
  * insertion removes one and adds two, a deletion removes two and adds one,
  * so each edit lowers P by at most two and N by at most two, and d >=
- * max(P, N) / 2 = (L1 + |t_a - t_b|) / 4 with t the 2-gram totals. Counts
- * saturate only past 256 bytes; then d >= L1' / 4 alone. */
+ * max(P, N) / 2 = (L1 + |t_a - t_b|) / 4 with t the 2-gram totals; as for
+ * the class counts, saturation only shrinks L1, so this holds at any length. */
 BL_HD u32 class2(u32 b) { return (b >> 1) & 3u; }
 
 BL_HD bool count2_over(const Profile &qc, const Profile &cc, int m, int n, int k)
@@ -1225,9 +1224,74 @@ BL_HD bool count2_over(const Profile &qc, const Profile &cc, int m, int n, int k
     l1 = sad_u8(qc.w, cc.w, l1);
     const int ta = m > 0 ? m - 1 : 0, tb = n > 0 ? n - 1 : 0;
     const u32 gap = (u32)(ta > tb ? ta - tb : tb - ta);
-    const u32 four = l1 + (m <= 256 && n <= 256 ? gap : 0u);
-    return (u64)four > 4ull * (u64)k;       /* 4k overflows 32 bits at k = 2^30 */
+    return (u64)l1 + gap > 4ull * (u64)k;   /* 4k overflows 32 bits at k = 2^30 */
 }
+
+/* A fourth bound from 3-grams, for small alphabets (DNA), where byte classes
+ * and 2-grams leave many unrelated pairs open. A 3-gram (x, y, z) falls into
+ * class 16 c2(x) + 4 c2(y) + c2(z) of 64; the profile holds the saturated
+ * counts over the string's max(n - 2, 0) 3-grams, byte i of dword j (of 16)
+ * counting class 4j + i. A substitution removes three 3-grams and adds three,
+ * an insertion removes two and adds three, a deletion removes three and adds
+ * two, so each edit lowers the excess P and the deficit N of a's counts by
+ * at most three each: d >= max(P, N) / 3 = (L1 + |t_a - t_b|) / 6, t the
+ * 3-gram totals (at any length: saturation only shrinks L1).
+ * Two random DNA strings of 230 bytes differ by about 137 in L1 over the 64
+ * classes (bound about 23), so at k = 8 almost only true near matches stay
+ * open. */
+struct alignas(16) Profile3 {
+    Profile p[4];
+};
+
+BL_HD bool count3_over(const Profile3 &qc, const Profile3 &cc, int m, int n, int k)
+{
+    u32 l1 = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        l1 = sad_u8(qc.p[i].x, cc.p[i].x, l1);
+        l1 = sad_u8(qc.p[i].y, cc.p[i].y, l1);
+        l1 = sad_u8(qc.p[i].z, cc.p[i].z, l1);
+        l1 = sad_u8(qc.p[i].w, cc.p[i].w, l1);
+    }
+    const int ta = m > 1 ? m - 2 : 0, tb = n > 1 ? n - 2 : 0;
+    const u32 gap = (u32)(ta > tb ? ta - tb : tb - ta);
+    return (u64)l1 + gap > 6ull * (u64)k;
+}
+
+BL_HD u32 class3(u32 x, u32 y, u32 z) { return 16u * class2(x) + 4u * class2(y) + class2(z); }
+
+/* Pack 64 counts, count(x) for class x, into a Profile3, saturated at 255. */
+template <typename Get>
+BL_HD Profile3 pack_profile3(Get count)
+{
+    Profile3 r;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        u32 w[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            u32 v = 0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const u32 c = (u32)count(16 * q + 4 * j + i);
+                v |= (c > 255u ? 255u : c) << (8 * i);
+            }
+            w[j] = v;
+        }
+        r.p[q] = Profile{w[0], w[1], w[2], w[3]};
+    }
+    return r;
+}
+
+#ifndef __HIP_DEVICE_COMPILE__
+/* The 3-gram profile of s[0..n) (host; the kernels count into LDS). */
+static inline Profile3 class3_profile(const u8 *s, int n)
+{
+    u32 cnt[64] = {0};
+    for (int i = 0; i + 2 < n; i++) cnt[class3(s[i], s[i + 1], s[i + 2])]++;
+    return pack_profile3([&](int x) { return cnt[x]; });
+}
+#endif
 
 /* The profile of s[0..n): 16 class counters of 16 bits in 8 dwords while
  * counting, then saturated to bytes. */

@@ -451,6 +451,19 @@ int blh_count2_over(const u8 *a, int m, const u8 *b, int n, int k)
     return count2_over(class2_profile(a, m), class2_profile(b, n), m, n, k) ? 1 : 0;
 }
 
+void blh_class3_profile(const u8 *s, int n, uint32_t *out)
+{
+    const Profile3 p = class3_profile(s, n);
+    for (int i = 0; i < 4; i++) {
+        out[4 * i] = p.p[i].x; out[4 * i + 1] = p.p[i].y; out[4 * i + 2] = p.p[i].z; out[4 * i + 3] = p.p[i].w;
+    }
+}
+
+int blh_count3_over(const u8 *a, int m, const u8 *b, int n, int k)
+{
+    return count3_over(class3_profile(a, m), class3_profile(b, n), m, n, k) ? 1 : 0;
+}
+
 uint32_t blh_swar_over(const u8 *q, int m, const u8 *t, const int *n, int k)
 {
     std::vector<u8> lo8(256, 0);

@@ -34,7 +34,9 @@ using namespace bl;
 namespace {
 
 /* One wave per group of 64 slots; slot c holds choice order[c] (c when order
- * is null). */
+ * is null). The 3-gram counts (counts3) go through a per-lane LDS histogram
+ * of 64 u16 counters, class-major (counter x of thread t at h3[x][t]), fed
+ * from the dwords as they are written. */
 __global__ __launch_bounds__(256) void stripes_build_kernel(const u8 *__restrict__ buf,
                                                             const int64_t *__restrict__ off,
                                                             int64_t n,
@@ -43,10 +45,16 @@ __global__ __launch_bounds__(256) void stripes_build_kernel(const u8 *__restrict
                                                             u32 *__restrict__ words,
                                                             int32_t *__restrict__ len,
                                                             Profile *__restrict__ counts,
-                                                            Profile *__restrict__ counts2)
+                                                            Profile *__restrict__ counts2,
+                                                            Profile3 *__restrict__ counts3)
 {
+    extern __shared__ uint16_t h3s[];        /* 64 x 256 counters, only with counts3 */
+    uint16_t(*h3)[256] = (uint16_t(*)[256])h3s;
     const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = (int)(threadIdx.x & 63);
+    const int t = (int)threadIdx.x;
+    if (counts3)
+        for (int x = 0; x < 64; x++) h3[x][t] = 0;
     if (g >= (n + 63) >> 6) return;
     const int64_t c = g * 64 + lane;
     const int64_t gb = gbase[g];
@@ -62,7 +70,26 @@ __global__ __launch_bounds__(256) void stripes_build_kernel(const u8 *__restrict
         if (counts) counts[c] = class_profile(s, ln);
         if (counts2) counts2[c] = class2_profile(s, ln);
     }
-    for (int64_t j = 0; j < width; j++) words[gb + 64 * j + lane] = stripe_word(s, ln, j);
+    if (!counts3) {
+        for (int64_t j = 0; j < width; j++) words[gb + 64 * j + lane] = stripe_word(s, ln, j);
+        return;
+    }
+    u32 b0 = 0, b1 = 0;                   /* the two bytes before the current one */
+    for (int64_t j = 0; j < width; j++) {
+        const u32 w = stripe_word(s, ln, j);
+        words[gb + 64 * j + lane] = w;
+        for (int i = 0; i < 4; i++) {
+            const int64_t at = 4 * j + i;
+            const u32 b2 = (w >> (8 * i)) & 0xFFu;
+            if (at >= 2 && at < ln) {
+                uint16_t &h = h3[class3(b0, b1, b2)][t];
+                h = (uint16_t)(h + (h < 0xFFFFu ? 1u : 0u));
+            }
+            b0 = b1;
+            b1 = b2;
+        }
+    }
+    if (c < n) counts3[c] = pack_profile3([&](int x) { return h3[x][t]; });
 }
 
 /* dst[r][i] = src[r][inv[i]]: one thread per column, rows in turn, so every
@@ -164,7 +191,8 @@ int64_t bl_stripes_layout_sorted(const int64_t *h_off, int64_t n, int64_t block,
 
 static int stripes_build(const char *who, const uint8_t *d_buf, const int64_t *d_off, int64_t n,
                          const int32_t *d_order, const int64_t *d_gbase, uint32_t *d_words,
-                         int32_t *d_len, uint32_t *d_counts, uint32_t *d_counts2, void *stream)
+                         int32_t *d_len, uint32_t *d_counts, uint32_t *d_counts2, uint32_t *d_counts3,
+                         void *stream)
 {
     if (n < 0) return fail(BL_EINVAL, "%s: negative count", who);
     if (n == 0) return BL_OK;
@@ -172,10 +200,17 @@ static int stripes_build(const char *who, const uint8_t *d_buf, const int64_t *d
     const int64_t groups = (n + 63) / 64;
     const int64_t blocks = (groups + 3) / 4;
     if (blocks > 0x7fffffff) return fail(BL_EINVAL, "%s: too many strings", who);
-    if (((uintptr_t)d_counts & 15) != 0 || ((uintptr_t)d_counts2 & 15) != 0)
+    if (((uintptr_t)d_counts & 15) != 0 || ((uintptr_t)d_counts2 & 15) != 0 ||
+        ((uintptr_t)d_counts3 & 15) != 0)
         return fail(BL_EINVAL, "%s: profiles not 16-byte aligned", who);
-    stripes_build_kernel<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(
-        d_buf, d_off, n, d_order, d_gbase, d_words, d_len, (Profile *)d_counts, (Profile *)d_counts2);
+    /* 32 KB of LDS per block whether or not the 3-gram histogram needs it:
+     * every lane reads its own string with scattered dword loads, and more
+     * resident waves only thrash L1 and L2 (4.2M strings of 200-256 bytes:
+     * 35 ms without the cap, 13.6 ms with it) */
+    const size_t lds = 64 * 256 * sizeof(uint16_t);
+    stripes_build_kernel<<<(unsigned)blocks, 256, lds, (hipStream_t)stream>>>(
+        d_buf, d_off, n, d_order, d_gbase, d_words, d_len, (Profile *)d_counts, (Profile *)d_counts2,
+        (Profile3 *)d_counts3);
     BL_LAUNCHED("stripes_build_kernel");
     return BL_OK;
 }
@@ -185,16 +220,15 @@ int bl_stripes_build(const uint8_t *d_buf, const int64_t *d_off, int64_t n, cons
                      void *stream)
 {
     return stripes_build("bl_stripes_build", d_buf, d_off, n, nullptr, d_gbase, d_words, d_len, d_counts,
-                         d_counts2, stream);
+                         d_counts2, nullptr, stream);
 }
 
 int bl_stripes_build_order(const uint8_t *d_buf, const int64_t *d_off, int64_t n, const int32_t *d_order,
                            const int64_t *d_gbase, uint32_t *d_words, int32_t *d_len, uint32_t *d_counts,
-                           uint32_t *d_counts2, void *stream)
+                           uint32_t *d_counts2, uint32_t *d_counts3, void *stream)
 {
-    if (n > 0 && !d_order) return fail(BL_EINVAL, "bl_stripes_build_order: null order");
     return stripes_build("bl_stripes_build_order", d_buf, d_off, n, d_order, d_gbase, d_words, d_len,
-                         d_counts, d_counts2, stream);
+                         d_counts, d_counts2, d_counts3, stream);
 }
 
 }  /* extern "C" */

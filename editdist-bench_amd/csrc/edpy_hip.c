@@ -267,11 +267,11 @@ FN(bl_stripes_layout_sorted)
 
 FN(bl_stripes_build_order)
 {
-    void *buf, *off, *order, *gbase, *words, *len, *counts, *counts2, *s;
+    void *buf, *off, *order, *gbase, *words, *len, *counts, *counts2, *counts3, *s;
     long long n;
-    PARSE("O&O&LO&O&O&O&O&O&O&", P(buf), P(off), &n, P(order), P(gbase), P(words), P(len),
-          P(counts), P(counts2), P(s));
-    RUN(H.bl_stripes_build_order(buf, off, n, order, gbase, words, len, counts, counts2, s));
+    PARSE("O&O&LO&O&O&O&O&O&O&O&", P(buf), P(off), &n, P(order), P(gbase), P(words), P(len),
+          P(counts), P(counts2), P(counts3), P(s));
+    RUN(H.bl_stripes_build_order(buf, off, n, order, gbase, words, len, counts, counts2, counts3, s));
 }
 
 FN(bl_gather_cols)

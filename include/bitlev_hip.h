@@ -124,6 +124,14 @@ typedef struct {
                                 kernel (lane mode 5) reads such a set: bl_cdist*
                                 run it whatever the mode asked for, and
                                 bl_cdist_class_stripes rejects it */
+    const uint32_t *counts3; /* 16 * n dwords of 3-gram profiles, or NULL: byte i
+                                of dword 16c + j counts the 3-grams (x, y, z) of
+                                choice c with 16 ((x >> 1) & 3) + 4 ((y >> 1) & 3)
+                                + ((z >> 1) & 3) == 4j + i, saturated at 255; a
+                                third bound, d >= (L1 + |t_q - t_c|) / 6 (L1 / 6
+                                past 257 bytes), applied by the count windows to
+                                what the 2-gram bound leaves open (16-byte
+                                aligned) */
 } bl_stripes;
 
 /* Host: group bases of the stripes of n strings with host offsets
@@ -151,11 +159,13 @@ int64_t bl_stripes_layout_sorted(const int64_t *h_off, int64_t n, int64_t block,
                                  int32_t *h_order, int64_t *h_gbase);
 
 /* Device: bl_stripes_build for the slot order d_order (from
- * bl_stripes_layout_sorted, uploaded): slot c gets string d_order[c]. */
+ * bl_stripes_layout_sorted, uploaded; NULL: slot c gets string c, with the
+ * group bases of bl_stripes_layout): slot c gets string d_order[c]. Also
+ * fills the 3-gram profiles unless d_counts3 is NULL. */
 int bl_stripes_build_order(const uint8_t *d_buf, const int64_t *d_off, int64_t n,
                            const int32_t *d_order, const int64_t *d_gbase, uint32_t *d_words,
                            int32_t *d_len, uint32_t *d_counts, uint32_t *d_counts2,
-                           void *stream);
+                           uint32_t *d_counts3, void *stream);
 
 /* dst[r][i] = src[r][inv[i]] for r < rows, i < cols (dtype BL_INT32 /
  * BL_UINT16 / BL_UINT8; src and dst distinct). No counterpart in the
@@ -248,8 +258,8 @@ typedef struct {
  * max_qlen bytes: the work-queue counters (4 KiB), one Peq table per query
  * (256 x ceil(max_qlen / 64) u64 words, the word count rounded up to 1, 2, 4
  * or 8), then two 16-byte profiles per query (its byte-class and 2-gram
- * profiles). Size workspaces from the return value, not from this list. 0
- * for a negative argument. */
+ * profiles) and one 64-byte 3-gram profile. Size workspaces from the return
+ * value, not from this list. 0 for a negative argument. */
 size_t bl_cdist_workspace_size(int64_t nq, int max_qlen);
 
 /* All queries (each at most max_qlen <= 512 bytes) against all nc choices:

@@ -214,13 +214,15 @@ def test_class_count_bound_in_searches():
     pq, pc = O.pack(Qs), O.pack(Cs)
     Q, C = hip.Packed(*pq), hip.Packed(*pc)
     exact = O.cross(pq, pc, threads=8)
-    S = C.stripes()
-    assert S.desc.counts and S.desc.counts2
-    saved, saved2 = S.desc.counts, S.desc.counts2
+    S = C.stripes(grams3=True)
+    assert S.desc.counts and S.desc.counts2 and S.desc.counts3
+    saved, saved2, saved3 = S.desc.counts, S.desc.counts2, S.desc.counts3
     try:
-        for with_profiles, with_2grams in ((True, True), (True, False), (False, False)):
+        for with_profiles, with_2grams, with_3grams in ((True, True, True), (True, True, False),
+                                                        (True, False, False), (False, False, False)):
             S.desc.counts = saved if with_profiles else None
             S.desc.counts2 = saved2 if with_2grams else None
+            S.desc.counts3 = saved3 if with_3grams else None
             for k in (0, 1, 2, 3, 5, 8, 300):
                 want = np.minimum(exact, k + 1)
                 for mode, extra in (("swar", {}), ("gap", {}), ("compacted", {}), ("count", {}),
@@ -237,7 +239,7 @@ def test_class_count_bound_in_searches():
                     assert got == sorted(zip(wq.tolist(), wc.tolist(), exact[wq, wc].tolist())), \
                         (k, mode, with_profiles)
     finally:
-        S.desc.counts, S.desc.counts2 = saved, saved2
+        S.desc.counts, S.desc.counts2, S.desc.counts3 = saved, saved2, saved3
 
 
 def test_filter_mode_with_every_choice_open():

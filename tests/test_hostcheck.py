@@ -464,8 +464,8 @@ def test_class_profiles_and_count_bound(hc):
     """The search kernels' class-count bound: profiles are the saturated counts
     of b & 15; whenever the bound puts a pair above k its exact distance (the
     oracle) is above k; the bound is reached on pairs that make it tight (so
-    an off-by-one in the test or the gap term fails here); strings above 255
-    bytes fall back to L1 / 2."""
+    an off-by-one in the test or the gap term fails here); above 255 bytes the
+    counts saturate, which only shrinks L1, and the gap term still holds."""
     hc.blh_class_profile.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)]
     hc.blh_count_over.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
                                   ctypes.c_int]
@@ -523,11 +523,11 @@ def test_class_profiles_and_count_bound(hc):
     for a, b in tight:
         dist = int(O.pairs(O.pack([a]), O.pack([b]))[0])
         assert over(a, b, dist - 1) and not over(a, b, dist), (a, b, dist)
-    # above 255 bytes: saturated counts, L1 / 2 only
+    # above 255 bytes: saturated counts
     a, b = b"a" * 300, b"b" * 300            # L1' = 510: bound 255 <= d = 300
     assert over(a, b, 254) and not over(a, b, 255)
-    a, b = b"a" * 300, b"a" * 290            # counts saturate alike: no pruning (the gap test decides)
-    assert not over(a, b, 0)
+    a, b = b"a" * 300, b"a" * 290            # counts saturate alike: L1' = 0, gap 10 -> bound 5 (d = 10)
+    assert over(a, b, 4) and not over(a, b, 5)
     # same counts in every class, far apart: the bound cannot prune (it is a bound, not a filter)
     assert not over(b"ab" * 20, b"ba" * 20, 0)
 
@@ -539,8 +539,8 @@ def test_two_gram_profiles_and_bound(hc):
     and DNA pairs); it is reached on pairs that make it tight (one
     substitution in the middle of a string moves two 2-grams out and two in;
     a run of one letter against a run of another), so a dropped gap term, a
-    factor of 2 for 4 or an off-by-one fails; past 256 bytes the gap term is
-    left out."""
+    factor of 2 for 4 or an off-by-one fails; past 256 bytes (saturated
+    counts) the gap term still holds."""
     hc.blh_class2_profile.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)]
     hc.blh_count2_over.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
                                    ctypes.c_int]
@@ -612,9 +612,89 @@ def test_two_gram_profiles_and_bound(hc):
     # the gap term decides: 9 vs 29 AA 2-grams, L1 = 20, gap 20 -> bound 10 (d = 20)
     a, b = b"A" * 10, b"A" * 30
     assert over(a, b, 9) and not over(a, b, 10)
-    # past 256 bytes the gap term is left out
+    # past 256 bytes both counts saturate (L1' = 0); the gap of 20 2-grams gives 5 (d = 20)
     a, b = b"A" * 300, b"A" * 280
-    assert not over(a, b, 0)
+    assert over(a, b, 4) and not over(a, b, 5)
+
+
+def test_three_gram_profiles_and_bound(hc):
+    """The 3-gram bound: profiles are the saturated counts of the 64 classes
+    16 c2(x) + 4 c2(y) + c2(z) over the string's 3-grams; every pair it prunes
+    has an oracle distance above k (random, edited, shuffled and DNA pairs);
+    it is reached on pairs that make it tight (one substitution in the middle
+    moves three 3-grams out and three in: L1 = 6, bound 1 = d), a run against
+    a run, and a length gap that decides, so a factor 3 or 4 for 6, a dropped
+    gap term or an off-by-one fails; past 257 bytes (saturated counts) the
+    gap term still holds."""
+    hc.blh_class3_profile.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)]
+    hc.blh_count3_over.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                   ctypes.c_int]
+    rng = np.random.default_rng(79)
+    for n in (0, 1, 2, 3, 17, 257, 258, 900):
+        s = bytes(rng.integers(0, 256, n).astype(np.uint8)) if n < 900 else b"A" * 900
+        out = (ctypes.c_uint32 * 16)()
+        hc.blh_class3_profile(s, n, out)
+        a = np.frombuffer(s, np.uint8).astype(np.int64)
+        c = (a >> 1) & 3
+        cls = 16 * c[:-2] + 4 * c[1:-1] + c[2:] if n > 2 else np.zeros(0, np.int64)
+        want = np.minimum(np.bincount(cls, minlength=64), 255)
+        assert np.frombuffer(bytes(out), np.uint8).tolist() == want.tolist(), n
+
+    def over(a, b, k):
+        return bool(hc.blh_count3_over(a, len(a), b, len(b), k))
+
+    A, B = [], []
+    for _ in range(4000):
+        alpha = rng.choice([b"ACGT", b"ab", b"abcdefghijklmnopqrstuvwxyz", bytes(range(256))])
+        m = int(rng.integers(0, 300))
+        a = bytes(rng.choice(np.frombuffer(alpha, np.uint8), m)) if m else b""
+        r = rng.random()
+        if r < 0.4:
+            b = bytearray(a)
+            for _ in range(int(rng.integers(0, 14))):
+                op = rng.integers(0, 3)
+                pos = int(rng.integers(0, len(b) + 1))
+                ch = int(rng.choice(np.frombuffer(alpha, np.uint8)))
+                if op == 0 and pos < len(b):
+                    b[pos] = ch
+                elif op == 1:
+                    b.insert(pos, ch)
+                elif pos < len(b):
+                    del b[pos]
+            b = bytes(b)
+        elif r < 0.6:
+            x = np.frombuffer(a, np.uint8).copy()
+            rng.shuffle(x)
+            b = x.tobytes()
+        else:
+            b = bytes(rng.choice(np.frombuffer(alpha, np.uint8), int(rng.integers(0, 300))))
+        A.append(a)
+        B.append(b)
+    d = O.pairs(O.pack(A), O.pack(B), threads=8)
+    pruned = 0
+    for a, b, dist in zip(A, B, d):
+        for k in (0, 1, 2, 3, 5, 8, 13, 30):
+            if over(a, b, k):
+                assert dist > k, (a, b, k, dist)
+                pruned += 1
+    assert pruned > 3000
+    # tight: one substitution in the middle (GTA TAC ACG out, GTT TTC TCG in)
+    a, b = b"ACGTACGTAC", b"ACGTTCGTAC"
+    assert int(O.pairs(O.pack([a]), O.pack([b]))[0]) == 1
+    pa, pb = (ctypes.c_uint32 * 16)(), (ctypes.c_uint32 * 16)()
+    hc.blh_class3_profile(a, len(a), pa); hc.blh_class3_profile(b, len(b), pb)
+    l1 = int(np.abs(np.frombuffer(bytes(pa), np.uint8).astype(int)
+                    - np.frombuffer(bytes(pb), np.uint8).astype(int)).sum())
+    assert l1 == 6 and over(a, b, 0) and not over(a, b, 1)
+    # runs: d = 10; 3-gram L1 = 16, gap 0 -> bound 16 / 6
+    a, b = b"A" * 10, b"C" * 10
+    assert over(a, b, 2) and not over(a, b, 3)
+    # the gap term decides: 8 vs 28 AAA 3-grams, L1 = 20, gap 20 -> bound 40 / 6 (d = 20)
+    a, b = b"A" * 10, b"A" * 30
+    assert over(a, b, 6) and not over(a, b, 7)
+    # past 257 bytes both counts saturate (L1' = 0); the gap of 20 3-grams gives 20 / 6 (d = 20)
+    a, b = b"A" * 300, b"A" * 280
+    assert over(a, b, 3) and not over(a, b, 4)
 
 
 def test_count_l2_bound(hc):
