@@ -67,7 +67,6 @@ __device__ __forceinline__ int64_t uni64(int64_t v)
     return (int64_t)(((u64)hi << 32) | lo);
 }
 
-/* Make this wave's LDS writes visible to its own later LDS reads by other lanes. */
 __device__ __forceinline__ int wave_min(int v)
 {
     for (int o = 32; o; o >>= 1) {
@@ -86,6 +85,7 @@ __device__ __forceinline__ int wave_max(int v)
     return v;
 }
 
+/* Make this wave's LDS writes visible to its own later LDS reads by other lanes. */
 __device__ __forceinline__ void wave_lds_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -109,6 +109,21 @@ __device__ __forceinline__ long long wave_take(unsigned long long *counter, long
     const u32 lo = __builtin_amdgcn_readfirstlane((u32)t);
     const u32 hi = __builtin_amdgcn_readfirstlane((u32)(t >> 32));
     return (long long)(((u64)hi << 32) | lo);
+}
+
+/* Reserve n slots of an append counter for the active lanes, one atomic per
+ * wave: only the first active lane adds, and readfirstlane hands its old
+ * value to every lane. (wave_take has every lane add, the others zero, for
+ * work-queue loops whose exit depends on the ticket; a same-address atomic
+ * from 64 lanes is 64 operations in L2. Here the result only places stores,
+ * and the branch has nothing after it that depends on which lane took it.) */
+__device__ __forceinline__ unsigned long long wave_reserve(unsigned long long *counter, u32 n)
+{
+    unsigned long long t = 0;
+    if (rank_below(__builtin_amdgcn_read_exec()) == 0) t = atomicAdd(counter, (unsigned long long)n);
+    const u32 lo = __builtin_amdgcn_readfirstlane((u32)t);
+    const u32 hi = __builtin_amdgcn_readfirstlane((u32)(t >> 32));
+    return ((u64)hi << 32) | lo;
 }
 
 /* Work queues: QUEUES counters on separate 256-byte lines (QUEUE_BYTES of
@@ -187,7 +202,7 @@ struct MatchSink {
         const bool hit = d >= 0 && d <= k;
         const u64 m = wave_ballot(hit);
         if (m == 0) return;
-        const unsigned long long base = (unsigned long long)wave_take(count, (long long)__popcll(m));
+        const unsigned long long base = wave_reserve(count, (u32)__popcll(m));
         if (hit) {
             const unsigned long long pos = base + (unsigned long long)rank_below(m);
             if (pos < (unsigned long long)cap) {
