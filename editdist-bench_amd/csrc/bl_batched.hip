@@ -1377,8 +1377,9 @@ __device__ __forceinline__ void count_windows(int64_t *open_stack, const int32_t
  * queries' class profiles. */
 template <int NWMAX, int GL, typename Sink, bool GRAM = false>
 __global__ __launch_bounds__(256)
-/* the four-word instance sits at the 128-VGPR line: keep it at 4 waves/SIMD */
-__attribute__((amdgpu_waves_per_eu(NWMAX == 4 ? 4 : 1))) void cdist_count_kernel(
+/* the four-word instance sits at the 128-VGPR line: keep it at 4 waves/SIMD;
+ * the others within 168 VGPRs, 3 waves/SIMD */
+__attribute__((amdgpu_waves_per_eu(NWMAX == 4 ? 4 : 3))) void cdist_count_kernel(
     const int64_t *__restrict__ oq, int64_t nq, const u64 *__restrict__ gpeq,
     const u32 *__restrict__ words, const int64_t *__restrict__ gbase,
     const int32_t *__restrict__ clen, const Profile *__restrict__ ccnt,
@@ -1386,7 +1387,8 @@ __attribute__((amdgpu_waves_per_eu(NWMAX == 4 ? 4 : 1))) void cdist_count_kernel
     const Profile *__restrict__ qcnt2, const Profile3 *__restrict__ ccnt3,
     const Profile3 *__restrict__ qcnt3, int64_t nc, Sink sink,
     int k_all, const int32_t *__restrict__ kvec, int64_t tile, int grab, int qgroup,
-    unsigned long long *counter, int first_static)
+    unsigned long long *counter, int first_static, int64_t *__restrict__ elist,
+    unsigned long long *elist_count, int64_t ecap)
 {
     extern __shared__ u64 lds[];
     CountLds<GL> &L = ((CountLds<GL> *)lds)[threadIdx.x >> 6];
@@ -1399,10 +1401,21 @@ __attribute__((amdgpu_waves_per_eu(NWMAX == 4 ? 4 : 1))) void cdist_count_kernel
     int64_t q0 = 0;
     /* pairs the 32-row stage leaves undecided (near matches, which need the
      * whole pattern) go through the wider stages 64 at a time, so one hard
-     * pair does not hold 63 easy lanes for its whole scan */
-    auto escalate = [&](int cnt) {
+     * pair does not hold 63 easy lanes for its whole scan. With an
+     * escalation list in the workspace (cap > 0) they are appended there
+     * instead (one atomic per batch) for cdist_list_kernel, which runs a
+     * diagonal band first; what does not fit the list is scanned here. */
+    auto escalate = [&](int cnt) __attribute__((noinline)) {
         const int base = ne - cnt;
-        if (lane < cnt) {
+        bool here = lane < cnt;
+        if (here && ecap > 0) {
+            const u64 pos = wave_reserve(elist_count, (u32)cnt) + (u64)lane;
+            if (pos < (u64)ecap) {
+                elist[pos] = L.esc[base + lane];
+                here = false;
+            }
+        }
+        if (here) {
             const int64_t idx = L.esc[base + lane];
             const int64_t q = idx / nc;
             const int64_t c = idx - q * nc;
@@ -1591,6 +1604,63 @@ __attribute__((amdgpu_waves_per_eu(NWMAX == 4 ? 4 : 1))) void cdist_count_kernel
     while (ns > 0) drain(ns < 64 ? ns : 64);
     while (nsc > 0) scan_drain(nsc < 64 ? nsc : 64);
     while (ne > 0) escalate(ne < 64 ? ne : 64);
+}
+
+/* The escalation list of a count-mode launch: every (query, choice) entry
+ * (output index q * nc + c) the bounds and the 32-row stage left open --
+ * near matches, mostly -- in batches of 64 per wave. A match within k lies on
+ * at most k + 1 diagonals (Ukkonen), so a diagonal band decides most of them:
+ * one window offset for the batch from its lanes' length differences, 32
+ * rows when that window reaches every lane's k, else 64 (band_scan: bit
+ * planes of the query bytes, the choice read from its stripe); a lane the
+ * window cannot reach runs the whole pattern (peq_lane). A kernel of its own
+ * so that the band's planes do not cost the count kernel its occupancy.
+ * (The 256 x 4.2M DNA search at k = 8 spent 1.0 of its 1.8 ms in the whole
+ * scans of its 420K near matches: four 64-row words a column, Eq from global
+ * memory.) */
+template <int NWMAX, typename Sink>
+__global__ __launch_bounds__(256) void cdist_list_kernel(
+    const u8 *__restrict__ bq, const int64_t *__restrict__ oq, const u64 *__restrict__ gpeq,
+    const u32 *__restrict__ words, const int64_t *__restrict__ gbase, const int32_t *__restrict__ clen,
+    int64_t nc, Sink sink, int k_all, const int32_t *__restrict__ kvec, const int64_t *__restrict__ elist,
+    const unsigned long long *__restrict__ elist_count, int64_t ecap)
+{
+    const int lane = lane_id();
+    const unsigned long long cnt_all = *elist_count;
+    const int64_t total = cnt_all < (unsigned long long)ecap ? (int64_t)cnt_all : ecap;
+    const int64_t waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; 64 * b < total; b += waves) {
+        const int64_t e = 64 * b + lane;
+        const bool live = e < total;
+        int64_t idx = 0, q = 0, c = 0;
+        int mq = 0, kq = -1, n = 0, dl = 0;
+        if (live) {
+            idx = elist[e];
+            q = idx / nc;
+            c = idx - q * nc;
+            mq = (int)(oq[q + 1] - oq[q]);
+            kq = kvec ? clamp_k_dev(kvec[q]) : k_all;
+            n = clen[c];
+            dl = mq - n;
+        }
+        const bool banded = live && kq >= 0 && n > 0 && mq > 0;
+        const int dmin = wave_min(banded ? dl : 0x7fffffff), dmax = wave_max(banded ? dl : -0x7fffffff);
+        const int d32 = band_offset<32>(dmin, dmax), d64 = band_offset<64>(dmin, dmax);
+        const bool w32 = wave_ballot(banded && kq > band_reach<32>(dl, d32)) == 0;
+        const StripeText text{words + gbase[c >> 6] + (c & 63)};
+        int d = -1;
+        if (banded) {
+            if (w32)
+                d = band_scan<u32>(bq + oq[q], mq, text, n, d32, kq, true);
+            else if (kq <= band_reach<64>(dl, d64))
+                d = band_scan<u64>(bq + oq[q], mq, text, n, d64, kq, true);
+        }
+        if (live && d < 0) {
+            const u64 *qp = gpeq + (size_t)q * 256 * NWMAX;
+            d = peq_lane<NWMAX, 1, false>((const u32 *)nullptr, qp, mq, text, n, kq);
+        }
+        if (live) sink.put(idx, d, kq);
+    }
 }
 
 /* ======================================================================= pairs */
@@ -2013,10 +2083,23 @@ static size_t cdist_peq_end(int64_t nq, int max_qlen)
     return QUEUE_BYTES + (size_t)nq * 256 * cdist_nwmax(max_qlen) * sizeof(u64);
 }
 
+/* The part of the workspace every launch needs: queues, Peq tables, the
+ * queries' profiles. Then, 256-byte aligned, the escalation list of count
+ * mode: its counter on a line of its own, then as many 8-byte entries as the
+ * rest of the workspace holds (none: near matches are scanned in the count
+ * kernel itself). */
+static size_t cdist_ws_core(int64_t nq, int max_qlen)
+{
+    return cdist_peq_end(nq, max_qlen) + 2 * (size_t)nq * sizeof(Profile) + (size_t)nq * sizeof(Profile3);
+}
+
+static size_t cdist_list_start(int64_t nq, int max_qlen) { return (cdist_ws_core(nq, max_qlen) + 255) / 256 * 256; }
+
 size_t bl_cdist_workspace_size(int64_t nq, int max_qlen)
 {
     if (nq < 0 || max_qlen < 0) return 0;
-    return cdist_peq_end(nq, max_qlen) + 2 * (size_t)nq * sizeof(Profile) + (size_t)nq * sizeof(Profile3);
+    const int64_t entries = nq * 2048 < ((int64_t)1 << 20) ? nq * 2048 : ((int64_t)1 << 20);
+    return cdist_list_start(nq, max_qlen) + 256 + (size_t)entries * 8;
 }
 
 }  /* extern "C" */
@@ -2029,10 +2112,13 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
                         const Sink &sink, unsigned long long *extra_clear)
 {
     const int64_t nc = choices->n;
-    const size_t need_ws = bl_cdist_workspace_size(nq, max_qlen);
+    const size_t need_ws = cdist_ws_core(nq, max_qlen);
     if (d_work && work_bytes < need_ws)
         return fail(BL_EINVAL, "%s: workspace of %zu bytes below the %zu needed", who, work_bytes,
                     need_ws);
+    const size_t ws_bytes = d_work ? work_bytes : bl_cdist_workspace_size(nq, max_qlen);
+    const size_t lstart = cdist_list_start(nq, max_qlen);
+    const int64_t ecap = ws_bytes >= lstart + 256 + 8 ? (int64_t)((ws_bytes - lstart - 256) / 8) : 0;
     hipStream_t s = (hipStream_t)stream;
     const int wpc = prm && prm->waves_per_cu > 0 ? prm->waves_per_cu : 16;
     /* tiles start on stripe groups (multiples of 64 choices; of 256 for the
@@ -2068,7 +2154,9 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
     if (blocks > need) blocks = need;
     if (blocks < 1) blocks = 1;
     unsigned char *ws = (unsigned char *)d_work;
-    if (!ws) BL_HIP(hipMallocAsync((void **)&ws, need_ws, s));
+    if (!ws) BL_HIP(hipMallocAsync((void **)&ws, ws_bytes, s));
+    unsigned long long *elist_count = ecap > 0 ? (unsigned long long *)(ws + lstart) : nullptr;
+    int64_t *elist = ecap > 0 ? (int64_t *)(ws + lstart + 256) : nullptr;
     unsigned long long *ctr = (unsigned long long *)ws;
     u64 *gpeq = (u64 *)(ws + QUEUE_BYTES);
     /* query class profiles after the Peq tables, when the choices carry theirs */
@@ -2081,6 +2169,7 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
     Profile3 *qcnt3 = ccnt3 ? (Profile3 *)((Profile *)(ws + cdist_peq_end(nq, max_qlen)) + 2 * nq) : nullptr;
     int rc = launch_clear(ctr, QUEUE_BYTES / 8, s);
     if (rc == BL_OK && extra_clear) rc = launch_clear(extra_clear, 1, s);
+    if (rc == BL_OK && mode == 4 && elist_count) rc = launch_clear(elist_count, 1, s);
     if (rc == BL_OK) {
         peq_build_kernel<<<(unsigned)nq, 256, 0, s>>>(d_buf_q, d_off_q, nwmax, gpeq, qcnt, qcnt2, qcnt3);
         auto capped = [&](auto kernel, size_t lds_bytes) {
@@ -2094,19 +2183,19 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
                 const size_t lb = (sizeof(CountLds<0>) + sizeof(GramLds)) * 4;
                 kern<<<capped(kern, lb), 256, lb, s>>>(
                     d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, ccnt2,
-                    qcnt2, ccnt3, qcnt3, nc, sink, k, d_kvec, tile, grab, qgroup, ctr, first_static);
+                    qcnt2, ccnt3, qcnt3, nc, sink, k, d_kvec, tile, grab, qgroup, ctr, first_static, elist, elist_count, ecap);
             } else if (mode == 4 && qgroup <= CD_GL) {
                 auto kern = cdist_count_kernel<NW, CD_GL, Sink>;
                 const size_t lb = sizeof(CountLds<CD_GL>) * 4;
                 kern<<<capped(kern, lb), 256, lb, s>>>(
                     d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, ccnt2,
-                    qcnt2, ccnt3, qcnt3, nc, sink, k, d_kvec, tile, grab, qgroup, ctr, first_static);
+                    qcnt2, ccnt3, qcnt3, nc, sink, k, d_kvec, tile, grab, qgroup, ctr, first_static, elist, elist_count, ecap);
             } else if (mode == 4) {
                 auto kern = cdist_count_kernel<NW, 0, Sink>;
                 const size_t lb = sizeof(CountLds<0>) * 4;
                 kern<<<capped(kern, lb), 256, lb, s>>>(
                     d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, ccnt, qcnt, ccnt2,
-                    qcnt2, ccnt3, qcnt3, nc, sink, k, d_kvec, tile, grab, qgroup, ctr, first_static);
+                    qcnt2, ccnt3, qcnt3, nc, sink, k, d_kvec, tile, grab, qgroup, ctr, first_static, elist, elist_count, ecap);
             } else if (mode == 5) {
                 cdist_exact_kernel<NW, Sink><<<(unsigned)(nq * xblk), 256, 0, s>>>(
                     d_off_q, nq, gpeq, choices->words, choices->gbase, choices->len, choices->order, nc,
@@ -2123,6 +2212,21 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
         case 2: go(std::integral_constant<int, 2>()); break;
         case 4: go(std::integral_constant<int, 4>()); break;
         default: go(std::integral_constant<int, 8>()); break;
+        }
+        /* count mode: the near matches the count kernel listed */
+        auto lgo = [&](auto nwc) {
+            constexpr int NW = decltype(nwc)::value;
+            cdist_list_kernel<NW, Sink><<<(unsigned)(device_cus() * 8), 256, 0, s>>>(
+                d_buf_q, d_off_q, gpeq, choices->words, choices->gbase, choices->len, nc, sink, k, d_kvec,
+                elist, elist_count, ecap);
+        };
+        if (mode == 4 && ecap > 0) {
+            switch (nwmax) {
+            case 1: lgo(std::integral_constant<int, 1>()); break;
+            case 2: lgo(std::integral_constant<int, 2>()); break;
+            case 4: lgo(std::integral_constant<int, 4>()); break;
+            default: lgo(std::integral_constant<int, 8>()); break;
+            }
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) rc = fail(BL_EHIP, "%s launch: %s", who, hipGetErrorString(e));
@@ -2230,7 +2334,7 @@ int bl_cdist_noalloc(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
     if (dtype < BL_INT32 || dtype > BL_UINT8)
         return fail(BL_EINVAL, "bl_cdist_noalloc: bad dtype %d", dtype);
     if (!d_out) return fail(BL_EINVAL, "bl_cdist_noalloc: null pointer");
-    const size_t need = bl_cdist_workspace_size(nq, max_qlen);
+    const size_t need = cdist_ws_core(nq, max_qlen);
     if (need > (size_t)BL_NOALLOC_SCRATCH_BYTES)
         return fail(BL_EINVAL,
                     "bl_cdist_noalloc: %lld queries of up to %d bytes need %zu bytes of scratch, above "

@@ -287,6 +287,50 @@ int blh_band(const u8 *a, int la, const u8 *b, int lb, int trim, int k, int d0, 
     return band_pair<u64>(a, la, b, lb, k, d0);
 }
 
+/* One batch of the count kernel's escalation list as cdist_list_kernel runs
+ * it: the queries' bytes as patterns (no trim, no swap), the choices read
+ * from stripes, one window offset for the batch from the lanes' m - n, 32
+ * rows when that window reaches every lane's k, else 64 per lane where it
+ * reaches, the whole pattern otherwise. n pairs (q[i], c[i]) in packed sets;
+ * out[i] = min(d, k + 1); *banded = how many a band decided. */
+int blh_list_batch(const u8 *bq, const int64_t *oq, const int32_t *qi, const u8 *bc, const int64_t *oc,
+                   int64_t nc, const int32_t *ci, int n, int k, int32_t *out, int *banded)
+{
+    if (n < 1 || n > 64 || k < 0) return -1;
+    const HostStripes hs(bc, oc, nc);
+    int dmin = 0x7fffffff, dmax = -0x7fffffff;
+    std::vector<int> m(n), len(n), dl(n);
+    std::vector<bool> ok(n);
+    for (int i = 0; i < n; i++) {
+        m[i] = (int)(oq[qi[i] + 1] - oq[qi[i]]);
+        len[i] = (int)(oc[ci[i] + 1] - oc[ci[i]]);
+        dl[i] = m[i] - len[i];
+        ok[i] = len[i] > 0 && m[i] > 0;
+        if (ok[i]) {
+            dmin = dl[i] < dmin ? dl[i] : dmin;
+            dmax = dl[i] > dmax ? dl[i] : dmax;
+        }
+    }
+    const int d32 = band_offset<32>(dmin, dmax), d64 = band_offset<64>(dmin, dmax);
+    bool w32 = true;
+    for (int i = 0; i < n; i++)
+        if (ok[i] && k > band_reach<32>(dl[i], d32)) w32 = false;
+    *banded = 0;
+    for (int i = 0; i < n; i++) {
+        const u8 *qb = bq + oq[qi[i]];
+        const StripeText text = hs.text(ci[i]);
+        int d = -1;
+        if (ok[i]) {
+            if (w32) d = band_scan<u32>(qb, m[i], text, len[i], d32, k, true);
+            else if (k <= band_reach<64>(dl[i], d64)) d = band_scan<u64>(qb, m[i], text, len[i], d64, k, true);
+        }
+        if (d >= 0) (*banded)++;
+        else d = peq_host(qb, m[i], text, bc + oc[ci[i]], len[i], k);
+        out[i] = d;
+    }
+    return 0;
+}
+
 /* The window offset a tile uses for pairs with m - n in [dl_min, dl_max]. */
 int blh_band_offset(int dl_min, int dl_max, int win)
 {

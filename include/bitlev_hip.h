@@ -258,8 +258,12 @@ typedef struct {
  * max_qlen bytes: the work-queue counters (4 KiB), one Peq table per query
  * (256 x ceil(max_qlen / 64) u64 words, the word count rounded up to 1, 2, 4
  * or 8), then two 16-byte profiles per query (its byte-class and 2-gram
- * profiles) and one 64-byte 3-gram profile. Size workspaces from the return
- * value, not from this list. 0 for a negative argument. */
+ * profiles) and one 64-byte 3-gram profile; then the escalation list of the
+ * count windows (lane mode 4): a counter and min(2048 nq, 2^20) 8-byte
+ * entries. A workspace may be smaller, down to the part before the list
+ * (the near matches the list would hold are then scanned in the count
+ * kernel: same results, slower), or larger (a longer list). Size workspaces
+ * from the return value, not from this list. 0 for a negative argument. */
 size_t bl_cdist_workspace_size(int64_t nq, int max_qlen);
 
 /* All queries (each at most max_qlen <= 512 bytes) against all nc choices:
@@ -290,12 +294,13 @@ int bl_cdist_tuned(const uint8_t *d_buf_q, const int64_t *d_off_q, int64_t nq,
 #define BL_NOALLOC_SCRATCH_BYTES (8 << 20)
 
 /* bl_cdist_tuned without any allocation: the workspace (work-queue counter,
- * Peq tables, query profiles) is a static device array of
- * BL_NOALLOC_SCRATCH_BYTES per GPU instead of a caller buffer or a
- * stream-ordered allocation; the call makes no hipMalloc, hipFree or other
- * allocating call. Returns BL_EINVAL when the queries need more than that
- * (bl_cdist_workspace_size(nq, max_qlen) > BL_NOALLOC_SCRATCH_BYTES: about
- * 4,000 queries of up to 64 bytes, 500 of up to 512). Every call on one GPU
+ * Peq tables, query profiles, and an escalation list in what remains) is a
+ * static device array of BL_NOALLOC_SCRATCH_BYTES per GPU instead of a
+ * caller buffer or a stream-ordered allocation; the call makes no hipMalloc,
+ * hipFree or other allocating call. Returns BL_EINVAL when the queries need
+ * more than that (the part of bl_cdist_workspace_size(nq, max_qlen) before
+ * the list above BL_NOALLOC_SCRATCH_BYTES: about 3,900 queries of up to 64
+ * bytes, 500 of up to 512). Every call on one GPU
  * shares that GPU's array, so calls that overlap on one GPU (on different
  * streams or from different threads) race; calls ordered on one stream do
  * not. No counterpart in the reference: for callers that cannot allocate

@@ -54,7 +54,8 @@ def test_extension_binds_every_entry_point():
     with pytest.raises(TypeError):
         _edpy.bl_free("not an address")
     assert _edpy.bl_cdist_workspace_size(-1, 0) == 0
-    assert _edpy.bl_cdist_workspace_size(4, 64) == 4096 + 4 * 2048 + 4 * 32 + 4 * 64
+    core = 4096 + 4 * 2048 + 4 * 32 + 4 * 64          # queues, Peq tables, profiles
+    assert _edpy.bl_cdist_workspace_size(4, 64) == (core + 255) // 256 * 256 + 256 + 4 * 2048 * 8
 
 
 def test_class_entry_points_reject_bad_arguments():

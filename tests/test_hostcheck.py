@@ -460,6 +460,51 @@ def test_trim_matches_definition(hc, a, b):
     assert list(res) == [p, len(a) - p - s, p, len(b) - p - s]
 
 
+def test_escalation_list_batches(hc):
+    """cdist_list_kernel's batches on the host: near matches and unrelated
+    pairs of queries up to 256 bytes against choices from stripes, batches of
+    1-64 with mixed length differences, thresholds 0-40; the band (32 rows
+    when it reaches every lane, else 64 where it reaches) and the whole-pattern
+    fallback give exactly min(d, k + 1) of the oracle, and the band decides
+    most near matches."""
+    hc.blh_list_batch.argtypes = [u8p, i64p, i32p, u8p, i64p, ctypes.c_int64, i32p, ctypes.c_int,
+                                  ctypes.c_int, i32p, ctypes.POINTER(ctypes.c_int)]
+    rng = np.random.default_rng(91)
+    Qs = [bytes(rng.integers(0, 4, int(rng.integers(1, 257))).astype(np.uint8) + 65) for _ in range(40)]
+    Cs = []
+    for i in range(600):
+        q = Qs[i % len(Qs)]
+        if i % 3:
+            b = bytearray(q)                      # edits within the queries' alphabet
+            for _ in range(int(rng.integers(0, 12))):
+                op, pos, ch = int(rng.integers(0, 3)), int(rng.integers(0, len(b) + 1)), 65 + int(rng.integers(0, 4))
+                if op == 0 and pos < len(b):
+                    b[pos] = ch
+                elif op == 1:
+                    b.insert(pos, ch)
+                elif pos < len(b):
+                    del b[pos]
+            Cs.append(bytes(b))
+        else:
+            Cs.append(bytes(rng.integers(0, 4, int(rng.integers(0, 260))).astype(np.uint8) + 65))
+    (bq, oq), (bc, oc) = O.pack(Qs), O.pack(Cs)
+    banded_total = 0
+    for k in (0, 1, 3, 8, 14, 15, 30, 40):
+        want = O.cross((bq, oq), (bc, oc), k=k)
+        for trial in range(12):
+            n = int(rng.integers(1, 65))
+            qi = rng.integers(0, len(Qs), n).astype(np.int32)
+            ci = np.array([(int(x) * 3 + int(rng.integers(1, 3))) % len(Cs) if trial % 2 else
+                           int(rng.integers(0, len(Cs))) for x in qi], np.int32)
+            out = np.full(n, -7, np.int32)
+            banded = ctypes.c_int()
+            assert hc.blh_list_batch(_p(bq, u8p), _p(oq, i64p), _p(qi, i32p), _p(bc, u8p), _p(oc, i64p),
+                                     len(Cs), _p(ci, i32p), n, k, _p(out, i32p), ctypes.byref(banded)) == 0
+            np.testing.assert_array_equal(out, want[qi, ci], err_msg=f"k={k} trial={trial}")
+            banded_total += banded.value
+    assert banded_total > 500
+
+
 def test_class_profiles_and_count_bound(hc):
     """The search kernels' class-count bound: profiles are the saturated counts
     of b & 15; whenever the bound puts a pair above k its exact distance (the
