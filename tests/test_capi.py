@@ -287,3 +287,16 @@ def test_parallel_pack_matches_serial(n):
     if n > 10:
         with pytest.raises(TypeError):
             ingest.pack(S[:-1] + [None])
+
+
+def test_struct_bindings_follow_the_header():
+    """The ctypes mirrors of the C ABI's structures list the header's fields
+    in the header's order (a missing trailing field would hand the library
+    whatever follows the Python object)."""
+    text = open(os.path.join(ROOT, "include", "bitlev_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    for cname, py in (("bl_stripes", _capi.Stripes), ("bl_cdist_params", _capi.CdistParams),
+                      ("bl_pairs_params", _capi.PairsParams)):
+        body = re.search(r"typedef struct \{([^{}]*)\}\s*" + cname + ";", text).group(1)
+        fields = re.findall(r"\b([a-z_0-9]+)\s*(?:\[\d+\])?\s*;", body)
+        assert fields == [f[0] for f in py._fields_], (cname, fields)
