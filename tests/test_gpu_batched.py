@@ -610,3 +610,49 @@ def test_count_kernel_matrix_core_tiles(nc):
                 got = sorted(zip(qi.get().tolist(), ci.get().tolist(), di.get().tolist()))
                 wq, wc = np.nonzero(exact <= k)
                 assert got == sorted(zip(wq.tolist(), wc.tolist(), exact[wq, wc].tolist())), (k, p)
+
+
+@pytest.mark.parametrize("gen,k", [(4, 8), (2, 2)])
+def test_count_mode_escalation_list_sizes(gen, k):
+    """Count mode lists its near matches for the band kernel in the rest of
+    the workspace; with no room for a list (a workspace that ends before it),
+    a list of 10 or 100 entries (overflow: the rest scanned in the count
+    kernel) or the full default list, dense and match-list results equal the
+    oracle. Near copies make the list long (k = 8 DNA, k = 2 letters)."""
+    pq, pc = W.cdist(gen, 24, 6000, seed=11)
+    Q, C = hip.Packed(*pq), hip.Packed(*pc)
+    st = C.stripes()
+    want = O.cross(pq, pc, k=k, threads=8)
+    mq = Q.max_len
+    full = call("bl_cdist_workspace_size", Q.n, mq)
+    entries = min(Q.n * 2048, 1 << 20)
+    core = full - 256 - 8 * entries
+    assert core > 0
+    p = hip._cdist_params({"mode": "count", "tile": 1024, "qgroup": 16})
+    for size in (core, core + 256 + 8 * 10, core + 256 + 8 * 100, full):
+        ws = hip.DeviceBuffer(size)
+        out = hip.DeviceArray.empty((Q.n, C.n), np.uint8)
+        call("bl_cdist_tuned", Q.d_buf.ptr, Q.d_off.ptr, Q.n, st.ref, mq, out.ptr, hip.DTYPE_CODE[out.dtype], k,
+             None, ws.ptr, ws.nbytes, p, None)
+        call("bl_stream_sync", None)
+        np.testing.assert_array_equal(out.get(), O.saturate(want, np.uint8), err_msg=f"workspace {size}")
+        cap = max(Q.n * C.n, 1)
+        dq, dc, dd = (hip.DeviceArray.empty((cap,), t) for t in (np.int32, np.int64, np.int32))
+        cnt = hip.DeviceBuffer(8)
+        call("bl_cdist_matches", Q.d_buf.ptr, Q.d_off.ptr, Q.n, st.ref, mq, k, None, dq.ptr, dc.ptr, dd.ptr,
+             cap, cnt.ptr, ws.ptr, ws.nbytes, p, None)
+        n = np.zeros(1, np.uint64)
+        call("bl_memcpy_dtoh", n.ctypes.data, cnt.ptr, 8, None)
+        call("bl_stream_sync", None)
+        n = int(n[0])
+        got = sorted(zip(dq.get()[:n].tolist(), dc.get()[:n].tolist(), dd.get()[:n].tolist()))
+        wq, wc = np.nonzero(want <= k)
+        assert got == sorted(zip(wq.tolist(), wc.tolist(), want[wq, wc].tolist())), size
+    nw = 1 if mq <= 64 else 2 if mq <= 128 else 4 if mq <= 256 else 8
+    needed = 4096 + Q.n * 256 * nw * 8 + Q.n * (16 + 16 + 64)   # queues, Peq tables, profiles
+    assert needed <= core < needed + 256
+    with pytest.raises(ValueError):                      # below the part before the list
+        ws = hip.DeviceBuffer(needed - 8)
+        out = hip.DeviceArray.empty((Q.n, C.n), np.uint8)
+        call("bl_cdist_tuned", Q.d_buf.ptr, Q.d_off.ptr, Q.n, st.ref, mq, out.ptr, 2, k, None, ws.ptr, ws.nbytes,
+             p, None)
