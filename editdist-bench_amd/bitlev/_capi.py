@@ -30,7 +30,7 @@ _vp, _i64, _int = _c.c_void_p, _c.c_int64, _c.c_int
 class Stripes(ctypes.Structure):
     """bl_stripes (include/bitlev_hip.h)."""
     _fields_ = [("words", _vp), ("gbase", _vp), ("len", _vp), ("n", _i64), ("counts", _vp),
-                ("counts2", _vp), ("order", _vp), ("counts3", _vp)]
+                ("counts2", _vp), ("order", _vp), ("counts3", _vp), ("buf", _vp), ("off", _vp)]
 
 
 class CdistParams(ctypes.Structure):

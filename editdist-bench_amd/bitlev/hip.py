@@ -221,7 +221,8 @@ class ChoiceStripes:
         if sort_block:
             self.nbytes += self.d_order.nbytes
         self.desc = Stripes(self.d_words.ptr, self.d_gbase.ptr, self.d_len.ptr, n, counts, counts2,
-                            self.d_order.ptr if sort_block else None, counts3)
+                            self.d_order.ptr if sort_block else None, counts3, P.d_buf.ptr, P.d_off.ptr)
+        self._packed = P            # the buffers desc.buf / desc.off point into
 
     @property
     def ref(self):

@@ -1623,7 +1623,8 @@ __global__ __launch_bounds__(256) void cdist_list_kernel(
     const u8 *__restrict__ bq, const int64_t *__restrict__ oq, const u64 *__restrict__ gpeq,
     const u32 *__restrict__ words, const int64_t *__restrict__ gbase, const int32_t *__restrict__ clen,
     int64_t nc, Sink sink, int k_all, const int32_t *__restrict__ kvec, const int64_t *__restrict__ elist,
-    const unsigned long long *__restrict__ elist_count, int64_t ecap)
+    const unsigned long long *__restrict__ elist_count, int64_t ecap, const u8 *__restrict__ cbuf,
+    const int64_t *__restrict__ coff)
 {
     const int lane = lane_id();
     const unsigned long long cnt_all = *elist_count;
@@ -1647,19 +1648,27 @@ __global__ __launch_bounds__(256) void cdist_list_kernel(
         const int dmin = wave_min(banded ? dl : 0x7fffffff), dmax = wave_max(banded ? dl : -0x7fffffff);
         const int d32 = band_offset<32>(dmin, dmax), d64 = band_offset<64>(dmin, dmax);
         const bool w32 = wave_ballot(banded && kq > band_reach<32>(dl, d32)) == 0;
-        const StripeText text{words + gbase[c >> 6] + (c & 63)};
-        int d = -1;
-        if (banded) {
-            if (w32)
-                d = band_scan<u32>(bq + oq[q], mq, text, n, d32, kq, true);
-            else if (kq <= band_reach<64>(dl, d64))
-                d = band_scan<u64>(bq + oq[q], mq, text, n, d64, kq, true);
-        }
-        if (live && d < 0) {
-            const u64 *qp = gpeq + (size_t)q * 256 * NWMAX;
-            d = peq_lane<NWMAX, 1, false>((const u32 *)nullptr, qp, mq, text, n, kq);
-        }
-        if (live) sink.put(idx, d, kq);
+        /* the text from the packed bytes when the set has them (16 aligned
+         * bytes a load, the string in 2-3 cache lines), else from its stripe
+         * (one line per dword: 58 lines for a 230-byte choice) */
+        auto run = [&](const auto &text) {
+            int d = -1;
+            if (banded) {
+                if (w32)
+                    d = band_scan<u32>(bq + oq[q], mq, text, n, d32, kq, true);
+                else if (kq <= band_reach<64>(dl, d64))
+                    d = band_scan<u64>(bq + oq[q], mq, text, n, d64, kq, true);
+            }
+            if (live && d < 0) {
+                const u64 *qp = gpeq + (size_t)q * 256 * NWMAX;
+                d = peq_lane<NWMAX, 1, false>((const u32 *)nullptr, qp, mq, text, n, kq);
+            }
+            if (live) sink.put(idx, d, kq);
+        };
+        if (cbuf)
+            run(BlockText{cbuf + coff[c]});
+        else
+            run(StripeText{words + gbase[c >> 6] + (c & 63)});
     }
 }
 
@@ -2218,7 +2227,8 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
             constexpr int NW = decltype(nwc)::value;
             cdist_list_kernel<NW, Sink><<<(unsigned)(device_cus() * 8), 256, 0, s>>>(
                 d_buf_q, d_off_q, gpeq, choices->words, choices->gbase, choices->len, nc, sink, k, d_kvec,
-                elist, elist_count, ecap);
+                elist, elist_count, ecap, choices->order ? nullptr : choices->buf,
+                choices->order ? nullptr : choices->off);
         };
         if (mode == 4 && ecap > 0) {
             switch (nwmax) {

@@ -132,6 +132,12 @@ typedef struct {
                                 past 257 bytes), applied by the count windows to
                                 what the 2-gram bound leaves open (16-byte
                                 aligned) */
+    const uint8_t *buf;      /* the packed set the stripes were built from (buf,
+                                off: the choices in the caller's order), or NULL:
+                                with it the near matches of count mode read their
+                                text from the packed bytes (a string in 2-3 cache
+                                lines) instead of the stripes (a line per dword) */
+    const int64_t *off;
 } bl_stripes;
 
 /* Host: group bases of the stripes of n strings with host offsets
