@@ -190,16 +190,22 @@ class ChoiceStripes:
     def __init__(self, P: Packed, stream=None, profiles: bool = True, sort_block: int = 0,
                  grams3: bool = False):
         n = P.n
-        gbase = np.empty((n + 63) // 64 + 1, np.int64)
+        groups = (n + 63) // 64
         self.d_order = None
         if sort_block:
+            gbase = np.empty(groups + 1, np.int64)
             order = np.empty(max(n, 1), np.int32)
             total = call("bl_stripes_layout_sorted", P.h_off.ctypes.data, n, int(sort_block),
                          order.ctypes.data, gbase.ctypes.data)
             self.d_order = DeviceBuffer.from_host(order, stream)
+            self.d_gbase = DeviceBuffer.from_host(gbase, stream)
         else:
-            total = call("bl_stripes_layout", P.h_off.ctypes.data, n, gbase.ctypes.data)
-        self.d_gbase = DeviceBuffer.from_host(gbase, stream)
+            # the group bases from the resident offsets (synchronises the stream)
+            self.d_gbase = DeviceBuffer(8 * (groups + 1))
+            work = DeviceBuffer(call("bl_stripes_layout_workspace_size", n))
+            total = call("bl_stripes_layout_device", P.d_off.ptr, n, self.d_gbase.ptr, work.ptr,
+                         work.nbytes, stream)
+            del work
         self.d_words = DeviceBuffer(max(int(total), 1) * 4)
         self.d_len = DeviceBuffer(max(n, 1) * 4)
         self.d_counts = DeviceBuffer(max(n, 1) * 16) if profiles else None

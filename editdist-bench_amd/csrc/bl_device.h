@@ -1293,8 +1293,6 @@ static inline Profile3 class3_profile(const u8 *s, int n)
 }
 #endif
 
-/* The profile of s[0..n): 16 class counters of 16 bits in 8 dwords while
- * counting, then saturated to bytes. */
 /* A third bound over the byte-class counts, in the form the matrix cores
  * evaluate. An insertion or a deletion moves the count vector by a unit
  * vector, a substitution by the difference of two (L2 norm sqrt 2), so after
@@ -1326,54 +1324,42 @@ BL_HD bool count_l2_over(const Profile &qc, const Profile &cc, int k)
     return d2 > 2 * (int64_t)k * (int64_t)k;    /* k <= 2^30: fits 64 bits */
 }
 
-BL_HD Profile class_profile(const u8 *s, int n)
+/* Pack 16 counts, count(x) for class x, into a Profile, saturated at 255. */
+template <typename Get>
+BL_HD Profile pack_profile(Get count)
 {
-    u32 c16[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int i = 0; i < n; i++) {
-        const u32 cls = s[i] & 15u;
-#pragma unroll
-        for (int t = 0; t < 8; t++)
-            c16[t] += (cls >> 1) == (u32)t ? 1u << (16 * (cls & 1u)) : 0u;
-    }
     u32 w[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         u32 v = 0;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int cls = 4 * j + i;
-            const u32 cnt = (c16[cls >> 1] >> (16 * (cls & 1))) & 0xFFFFu;
-            v |= (cnt > 255u ? 255u : cnt) << (8 * i);
+            const u32 c = (u32)count(4 * j + i);
+            v |= (c > 255u ? 255u : c) << (8 * i);
         }
         w[j] = v;
     }
     return Profile{w[0], w[1], w[2], w[3]};
 }
 
-/* The 2-gram profile of s[0..n) (classes as for count2_over). */
-BL_HD Profile class2_profile(const u8 *s, int n)
+#ifndef __HIP_DEVICE_COMPILE__
+/* The byte-class profile of s[0..n) (class b & 15; host, the kernels count
+ * into LDS). */
+static inline Profile class_profile(const u8 *s, int n)
 {
-    u32 c16[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int i = 0; i + 1 < n; i++) {
-        const u32 cls = 4u * class2(s[i]) + class2(s[i + 1]);
-#pragma unroll
-        for (int t = 0; t < 8; t++)
-            c16[t] += (cls >> 1) == (u32)t ? 1u << (16 * (cls & 1u)) : 0u;
-    }
-    u32 w[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        u32 v = 0;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int cls = 4 * j + i;
-            const u32 cnt = (c16[cls >> 1] >> (16 * (cls & 1))) & 0xFFFFu;
-            v |= (cnt > 255u ? 255u : cnt) << (8 * i);
-        }
-        w[j] = v;
-    }
-    return Profile{w[0], w[1], w[2], w[3]};
+    u32 cnt[16] = {0};
+    for (int i = 0; i < n; i++) cnt[s[i] & 15u]++;
+    return pack_profile([&](int x) { return cnt[x]; });
 }
+
+/* The 2-gram profile of s[0..n) (classes as for count2_over). */
+static inline Profile class2_profile(const u8 *s, int n)
+{
+    u32 cnt[16] = {0};
+    for (int i = 0; i + 1 < n; i++) cnt[4u * class2(s[i]) + class2(s[i + 1])]++;
+    return pack_profile([&](int x) { return cnt[x]; });
+}
+#endif
 
 /* ------------------------------------------------------------ long strings */
 /* Pairs whose shorter string exceeds 256 bytes, and cross queries longer than

@@ -33,30 +33,67 @@ using namespace bl;
 
 namespace {
 
-/* One wave per group of 64 slots; slot c holds choice order[c] (c when order
- * is null). The 3-gram counts (counts3) go through a per-lane LDS histogram
- * of 64 u16 counters, class-major (counter x of thread t at h3[x][t]), fed
- * from the dwords as they are written. */
-__global__ __launch_bounds__(256) void stripes_build_kernel(const u8 *__restrict__ buf,
-                                                            const int64_t *__restrict__ off,
-                                                            int64_t n,
-                                                            const int32_t *__restrict__ order,
-                                                            const int64_t *__restrict__ gbase,
-                                                            u32 *__restrict__ words,
-                                                            int32_t *__restrict__ len,
-                                                            Profile *__restrict__ counts,
-                                                            Profile *__restrict__ counts2,
-                                                            Profile3 *__restrict__ counts3)
+/* One wave (a block of 64) per group of 64 slots; slot c holds choice
+ * order[c] (c when order is null). When the group's strings lie together in
+ * the packed buffer (no order) and span at most STAGE bytes, the wave first
+ * copies that span into LDS with aligned 16-byte loads (a wave load reads
+ * 1 KB of consecutive bytes); every lane then builds its stripe words from
+ * LDS. Otherwise each lane reads its own string's dwords from global memory.
+ * Either way the profiles are counted from the bytes the words are built
+ * from, with LDS adds (no return value, so nothing waits on them) into
+ * per-lane u32 histograms, counter x of lane l at h[x][l] (bank l whatever
+ * x): byte classes, 2-gram classes and, with counts3, 3-gram classes.
+ * (Reading every string three times with scattered loads -- the words, then
+ * byte by byte for each profile -- took 13.9 ms for 4.2M strings of 200-256
+ * bytes.) */
+typedef u32 v4u __attribute__((ext_vector_type(4)));
+constexpr int STAGE = 16384;                 /* bytes of a group staged in LDS */
+constexpr int H1 = 0, H2 = 16, H3 = 32;      /* histogram rows: classes, 2-grams, 3-grams */
+
+static size_t stripes_build_lds(bool counts, bool counts3)
 {
-    extern __shared__ uint16_t h3s[];        /* 64 x 256 counters, only with counts3 */
-    uint16_t(*h3)[256] = (uint16_t(*)[256])h3s;
-    const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int lane = (int)(threadIdx.x & 63);
-    const int t = (int)threadIdx.x;
-    if (counts3)
-        for (int x = 0; x < 64; x++) h3[x][t] = 0;
-    if (g >= (n + 63) >> 6) return;
-    const int64_t c = g * 64 + lane;
+    const int rows = counts3 ? 96 : (counts ? 32 : 0);
+    return (size_t)STAGE + 32 + (size_t)rows * 64 * sizeof(u32);
+}
+
+__global__ __launch_bounds__(64) void stripes_build_kernel(const u8 *__restrict__ buf,
+                                                           const int64_t *__restrict__ off,
+                                                           int64_t n,
+                                                           const int32_t *__restrict__ order,
+                                                           const int64_t *__restrict__ gbase,
+                                                           u32 *__restrict__ words,
+                                                           int32_t *__restrict__ len,
+                                                           Profile *__restrict__ counts,
+                                                           Profile *__restrict__ counts2,
+                                                           Profile3 *__restrict__ counts3)
+{
+    extern __shared__ __align__(16) u32 lds[];
+    u8 *stage = (u8 *)lds;                                      /* STAGE + 32 bytes */
+    u32(*h)[64] = (u32(*)[64])(stage + STAGE + 32);              /* 32 or 96 rows of 64 lanes */
+    const int64_t g = blockIdx.x;
+    const int lane = (int)threadIdx.x;
+    const int64_t c0 = g * 64, c1 = c0 + 64 < n ? c0 + 64 : n;
+    const int64_t c = c0 + lane;
+    const int rows = counts3 ? 96 : (counts || counts2 ? 32 : 0);
+    for (int x = 0; x < rows; x++) h[x][lane] = 0;
+    /* stage the group's bytes when they lie together: the aligned 16-byte
+     * blocks from the one holding byte off[c0] up to the one holding byte
+     * off[c1] - 1 (an aligned block never crosses a page, and each holds a
+     * byte of the buffer) */
+    bool staged = false;
+    const u8 *pa = buf;
+    if (!order) {
+        const u8 *p0 = buf + off[c0], *p1 = buf + off[c1];
+        pa = (const u8 *)((uintptr_t)p0 & ~(uintptr_t)15);
+        staged = p1 > p0 && p1 - pa <= STAGE;
+        if (staged) {
+            const int64_t span = p1 - pa;
+#pragma unroll 16
+            for (int64_t a = 16 * lane; a < span; a += 64 * 16)
+                *(v4u *)(stage + a) = *(const v4u *)(pa + a);
+        }
+    }
+    __syncthreads();
     const int64_t gb = gbase[g];
     const int64_t width = (gbase[g + 1] - gb) >> 6;
     int ln = 0;
@@ -67,29 +104,184 @@ __global__ __launch_bounds__(256) void stripes_build_kernel(const u8 *__restrict
         ln = (int)(off[src + 1] - o);
         s = buf + o;
         len[c] = ln;
-        if (counts) counts[c] = class_profile(s, ln);
-        if (counts2) counts2[c] = class2_profile(s, ln);
     }
-    if (!counts3) {
-        for (int64_t j = 0; j < width; j++) words[gb + 64 * j + lane] = stripe_word(s, ln, j);
-        return;
-    }
-    u32 b0 = 0, b1 = 0;                   /* the two bytes before the current one */
-    for (int64_t j = 0; j < width; j++) {
-        const u32 w = stripe_word(s, ln, j);
-        words[gb + 64 * j + lane] = w;
-        for (int i = 0; i < 4; i++) {
-            const int64_t at = 4 * j + i;
-            const u32 b2 = (w >> (8 * i)) & 0xFFu;
-            if (at >= 2 && at < ln) {
-                uint16_t &h = h3[class3(b0, b1, b2)][t];
-                h = (uint16_t)(h + (h < 0xFFFFu ? 1u : 0u));
+    const u8 *ls = stage + (s - pa);          /* (staged) the string in LDS */
+    u32 b0 = 0, b1 = 0;                       /* the two bytes before the current one */
+    /* four dwords a step: their LDS reads issue together, one wait a step */
+    for (int64_t j0 = 0; j0 < width; j0 += 4) {
+        u32 w[4] = {0, 0, 0, 0};
+        if (4 * j0 < ln) {
+            if (staged) {
+                /* five aligned LDS dwords from the one holding byte 4 j0 (the
+                 * stage starts 16-byte aligned; the reads end at most 19
+                 * bytes past the staged span, inside the stage's 32 spare
+                 * bytes, and what passes the string is masked off) */
+                const u8 *p = ls + 4 * j0;
+                const u32 sh = (u32)((uintptr_t)p & 3);
+                const u32 *al = (const u32 *)(p - sh);
+                u32 d[5];
+#pragma unroll
+                for (int k = 0; k < 5; k++) d[k] = al[k];
+#pragma unroll
+                for (int k = 0; k < 4; k++) w[k] = align_bytes(d[k + 1], d[k], sh);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; k++) w[k] = j0 + k < width ? stripe_word(s, ln, j0 + k) : 0u;
             }
-            b0 = b1;
-            b1 = b2;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int64_t r = ln - 4 * (j0 + k);
+                w[k] = r <= 0 ? 0u : (r < 4 ? w[k] & ((1u << (8 * r)) - 1u) : w[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (j0 + k < width) words[gb + 64 * (j0 + k) + lane] = w[k];
+        if (rows && 4 * j0 < ln) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int64_t at = 4 * (j0 + k) + i;
+                    const u32 b2 = (w[k] >> (8 * i)) & 0xFFu;
+                    if (at < ln) {
+                        atomicAdd(&h[H1 + (b2 & 15u)][lane], 1u);
+                        if (at >= 1) atomicAdd(&h[H2 + 4u * class2(b1) + class2(b2)][lane], 1u);
+                        if (counts3 && at >= 2) atomicAdd(&h[H3 + class3(b0, b1, b2)][lane], 1u);
+                    }
+                    b0 = b1;
+                    b1 = b2;
+                }
+            }
         }
     }
-    if (c < n) counts3[c] = pack_profile3([&](int x) { return h3[x][t]; });
+    if (c < n && rows) {
+        auto pack16 = [&](int row0) {
+            u32 v[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                u32 x = 0;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const u32 cnt = h[row0 + 4 * q + i][lane];
+                    x |= (cnt > 255u ? 255u : cnt) << (8 * i);
+                }
+                v[q] = x;
+            }
+            return Profile{v[0], v[1], v[2], v[3]};
+        };
+        if (counts) counts[c] = pack16(H1);
+        if (counts2) counts2[c] = pack16(H2);
+        if (counts3) counts3[c] = pack_profile3([&](int x) { return h[H3 + x][lane]; });
+    }
+}
+
+/* The stripes layout on the device, for offsets already in HBM (a host pass
+ * over 4.2M offsets takes about 4 ms, longer than the search it prepares).
+ * Three launches over blocks of LAYOUT_GROUPS groups: the widths and each
+ * block's own inclusive prefix of them (part[b]: the block's total, or -1
+ * for a decreasing offset), one block scanning the block totals in place
+ * into exclusive prefixes (and part[nb] = the grand total, or -1), and the
+ * prefixes added. gbase[0] = 0, gbase[g + 1] = sum of widths 0..g, a width
+ * 64 x the group's widest string in dwords, at least 1: the host layout's
+ * arithmetic. */
+constexpr int LAYOUT_GROUPS = 256;
+
+template <typename T>
+__device__ T wave_scan_incl(T v, int lane)
+{
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const T u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(256) void layout_widths_kernel(const int64_t *__restrict__ off, int64_t n,
+                                                            int64_t groups, int64_t *__restrict__ gbase,
+                                                            int64_t *__restrict__ part)
+{
+    __shared__ int64_t wsum[4];
+    __shared__ int wbad[4];
+    const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
+    const int64_t g0 = (int64_t)blockIdx.x * LAYOUT_GROUPS + 64 * wv;
+    int64_t widest = 0;                 /* lane i: group g0 + i */
+    bool bad = false;
+    for (int i = 0; i < 64 && g0 + i < groups; i++) {
+        const int64_t c = 64 * (g0 + i) + lane;
+        const int64_t l = c < n ? off[c + 1] - off[c] : 0;
+        bad |= l < 0;
+        int64_t mx = l;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            const int64_t o = __shfl_xor(mx, d, 64);
+            mx = o > mx ? o : mx;
+        }
+        if (lane == i) widest = mx;
+    }
+    const int64_t g = g0 + lane;
+    const int64_t w = g < groups ? 64 * (widest > 0 ? (widest + 3) / 4 : 1) : 0;
+    const int64_t incl = wave_scan_incl(w, lane);
+    if (lane == 63) wsum[wv] = incl;
+    const bool any_bad = __any(bad);
+    if (lane == 0) wbad[wv] = any_bad ? 1 : 0;
+    __syncthreads();
+    int64_t before = 0;
+    for (int x = 0; x < wv; x++) before += wsum[x];
+    if (g < groups) gbase[g + 1] = before + incl;
+    if (threadIdx.x == 0) {
+        const bool b = wbad[0] | wbad[1] | wbad[2] | wbad[3];
+        part[blockIdx.x] = b ? -1 : wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    }
+}
+
+/* One block of 1024: thread t scans entries [t per, (t + 1) per) of part in
+ * turn; a block scan of the thread sums gives each run its start. */
+__global__ __launch_bounds__(1024) void layout_parts_kernel(int64_t *__restrict__ part, int64_t nb,
+                                                            int64_t *__restrict__ gbase)
+{
+    __shared__ int64_t wsum[16];
+    __shared__ int wbad[16];
+    const int t = (int)threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int64_t per = (nb + 1023) / 1024;
+    const int64_t a = t * per, e = a + per < nb ? a + per : nb;
+    int64_t sum = 0;
+    bool bad = false;
+    for (int64_t i = a; i < e; i++) {
+        const int64_t v = part[i];
+        bad |= v < 0;
+        sum += v;
+    }
+    const int64_t incl = wave_scan_incl(sum, lane);
+    if (lane == 63) wsum[wv] = incl;
+    const bool any_bad = __any(bad);
+    if (lane == 0) wbad[wv] = any_bad ? 1 : 0;
+    __syncthreads();
+    int64_t run = incl - sum;
+    for (int x = 0; x < wv; x++) run += wsum[x];
+    for (int64_t i = a; i < e; i++) {
+        const int64_t v = part[i];
+        part[i] = run;
+        run += v;
+    }
+    if (t == 0) {
+        int64_t total = 0;
+        bool b = false;
+        for (int x = 0; x < 16; x++) {
+            total += wsum[x];
+            b |= wbad[x] != 0;
+        }
+        part[nb] = b ? -1 : total;
+        gbase[0] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void layout_add_kernel(const int64_t *__restrict__ part, int64_t groups,
+                                                         int64_t *__restrict__ gbase)
+{
+    const int64_t g = (int64_t)blockIdx.x * LAYOUT_GROUPS + threadIdx.x;
+    if (g < groups) gbase[g + 1] += part[blockIdx.x];
 }
 
 /* dst[r][i] = src[r][inv[i]]: one thread per column, rows in turn, so every
@@ -149,6 +341,45 @@ int64_t bl_stripes_layout(const int64_t *h_off, int64_t n, int64_t *h_gbase)
     return total;
 }
 
+size_t bl_stripes_layout_workspace_size(int64_t n)
+{
+    if (n < 0) return 0;
+    const int64_t groups = (n + 63) / 64;
+    const int64_t nb = (groups + LAYOUT_GROUPS - 1) / LAYOUT_GROUPS;
+    return ((size_t)(nb + 1) * sizeof(int64_t) + 255) & ~(size_t)255;
+}
+
+int64_t bl_stripes_layout_device(const int64_t *d_off, int64_t n, int64_t *d_gbase, void *d_work,
+                                 size_t work_bytes, void *stream)
+{
+    const char *who = "bl_stripes_layout_device";
+    if (n < 0) return fail(BL_EINVAL, "%s: negative count", who);
+    if (!d_off || !d_gbase || !d_work) return fail(BL_EINVAL, "%s: null pointer", who);
+    if (work_bytes < bl_stripes_layout_workspace_size(n))
+        return fail(BL_EINVAL, "%s: workspace of %zu bytes, %zu needed", who, work_bytes,
+                    bl_stripes_layout_workspace_size(n));
+    const int64_t groups = (n + 63) / 64;
+    const int64_t nb = (groups + LAYOUT_GROUPS - 1) / LAYOUT_GROUPS;
+    if (nb > 0x7fffffff) return fail(BL_EINVAL, "%s: too many strings", who);
+    hipStream_t s = (hipStream_t)stream;
+    int64_t *part = (int64_t *)d_work;
+    if (nb > 0) {
+        layout_widths_kernel<<<(unsigned)nb, 256, 0, s>>>(d_off, n, groups, d_gbase, part);
+        BL_LAUNCHED("layout_widths_kernel");
+    }
+    layout_parts_kernel<<<1, 1024, 0, s>>>(part, nb, d_gbase);
+    BL_LAUNCHED("layout_parts_kernel");
+    if (nb > 0) {
+        layout_add_kernel<<<(unsigned)nb, LAYOUT_GROUPS, 0, s>>>(part, groups, d_gbase);
+        BL_LAUNCHED("layout_add_kernel");
+    }
+    int64_t total = 0;
+    BL_HIP(hipMemcpyAsync(&total, part + nb, sizeof total, hipMemcpyDeviceToHost, s));
+    BL_HIP(hipStreamSynchronize(s));
+    if (total < 0) return fail(BL_EINVAL, "%s: offsets decrease", who);
+    return total;
+}
+
 int64_t bl_stripes_layout_sorted(const int64_t *h_off, int64_t n, int64_t block, int32_t *h_order,
                                  int64_t *h_gbase)
 {
@@ -198,17 +429,12 @@ static int stripes_build(const char *who, const uint8_t *d_buf, const int64_t *d
     if (n == 0) return BL_OK;
     if (!d_buf || !d_off || !d_gbase || !d_words || !d_len) return fail(BL_EINVAL, "%s: null pointer", who);
     const int64_t groups = (n + 63) / 64;
-    const int64_t blocks = (groups + 3) / 4;
-    if (blocks > 0x7fffffff) return fail(BL_EINVAL, "%s: too many strings", who);
+    if (groups > 0x7fffffff) return fail(BL_EINVAL, "%s: too many strings", who);
     if (((uintptr_t)d_counts & 15) != 0 || ((uintptr_t)d_counts2 & 15) != 0 ||
         ((uintptr_t)d_counts3 & 15) != 0)
         return fail(BL_EINVAL, "%s: profiles not 16-byte aligned", who);
-    /* 32 KB of LDS per block whether or not the 3-gram histogram needs it:
-     * every lane reads its own string with scattered dword loads, and more
-     * resident waves only thrash L1 and L2 (4.2M strings of 200-256 bytes:
-     * 35 ms without the cap, 13.6 ms with it) */
-    const size_t lds = 64 * 256 * sizeof(uint16_t);
-    stripes_build_kernel<<<(unsigned)blocks, 256, lds, (hipStream_t)stream>>>(
+    const size_t lds = stripes_build_lds(d_counts || d_counts2, d_counts3 != nullptr);
+    stripes_build_kernel<<<(unsigned)groups, 64, lds, (hipStream_t)stream>>>(
         d_buf, d_off, n, d_order, d_gbase, d_words, d_len, (Profile *)d_counts, (Profile *)d_counts2,
         (Profile3 *)d_counts3);
     BL_LAUNCHED("stripes_build_kernel");

@@ -35,7 +35,8 @@
     X(bl_event_record) X(bl_stream_wait_event) X(bl_event_sync) X(bl_event_query)             \
     X(bl_event_elapsed_ms) X(bl_graph_begin) X(bl_graph_end) X(bl_graph_launch)               \
     X(bl_graph_destroy) X(bl_pairs_class) X(bl_stripes_layout) X(bl_stripes_build)            \
-    X(bl_stripes_layout_sorted) X(bl_stripes_build_order) X(bl_gather_cols) X(bl_cdist_class) X(bl_cdist_class_stripes) X(bl_long)                  \
+    X(bl_stripes_layout_sorted) X(bl_stripes_build_order) X(bl_stripes_layout_workspace_size)       \
+    X(bl_stripes_layout_device) X(bl_gather_cols) X(bl_cdist_class) X(bl_cdist_class_stripes) X(bl_long) \
     X(bl_cdist_workspace_size) X(bl_cdist) X(bl_cdist_tuned) X(bl_cdist_matches)              \
     X(bl_pairs_workspace_size) X(bl_pairs) X(bl_pairs_tuned) X(bl_cdist_noalloc)
 
@@ -262,6 +263,24 @@ FN(bl_stripes_layout_sorted)
     PARSE("O&LLO&O&", P(off), &n, &block, P(order), P(gbase));
     long long r;
     Py_BEGIN_ALLOW_THREADS r = H.bl_stripes_layout_sorted(off, n, block, order, gbase);
+    Py_END_ALLOW_THREADS return PyLong_FromLongLong(r);
+}
+
+FN(bl_stripes_layout_workspace_size)
+{
+    long long n;
+    PARSE("L", &n);
+    return PyLong_FromSize_t(H.bl_stripes_layout_workspace_size(n));
+}
+
+FN(bl_stripes_layout_device)
+{
+    void *off, *gbase, *work, *s;
+    long long n;
+    unsigned long long work_bytes;
+    PARSE("O&LO&O&KO&", P(off), &n, P(gbase), P(work), &work_bytes, P(s));
+    long long r;
+    Py_BEGIN_ALLOW_THREADS r = H.bl_stripes_layout_device(off, n, gbase, work, (size_t)work_bytes, s);
     Py_END_ALLOW_THREADS return PyLong_FromLongLong(r);
 }
 

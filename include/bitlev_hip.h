@@ -145,9 +145,22 @@ typedef struct {
  * words array needs (h_gbase[ceil(n/64)]), or a negative status. */
 int64_t bl_stripes_layout(const int64_t *h_off, int64_t n, int64_t *h_gbase);
 
+/* Device: bl_stripes_layout for offsets already in HBM: the group bases of
+ * the n strings with device offsets d_off[0..n] into device d_gbase[0..
+ * ceil(n/64)], computed on `stream` with d_work (at least
+ * bl_stripes_layout_workspace_size(n) bytes) as scratch. Synchronises the
+ * stream and returns the number of dwords the words array needs, or a
+ * negative status (decreasing offsets: BL_EINVAL). No counterpart in the
+ * reference, whose kernels read the packed bytes: the host pass over 4.2M
+ * offsets takes about 4 ms, several times the device build it prepares. */
+size_t bl_stripes_layout_workspace_size(int64_t n);
+int64_t bl_stripes_layout_device(const int64_t *d_off, int64_t n, int64_t *d_gbase, void *d_work,
+                                 size_t work_bytes, void *stream);
+
 /* Device: fill words, len and (unless NULL) the class and 2-gram profiles
  * from packed strings (d_buf, d_off) using the group bases d_gbase that
- * bl_stripes_layout computed for the same offsets. */
+ * bl_stripes_layout or bl_stripes_layout_device computed for the same
+ * offsets. */
 int bl_stripes_build(const uint8_t *d_buf, const int64_t *d_off, int64_t n,
                      const int64_t *d_gbase, uint32_t *d_words, int32_t *d_len,
                      uint32_t *d_counts, uint32_t *d_counts2, void *stream);

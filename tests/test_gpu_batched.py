@@ -242,6 +242,109 @@ def test_class_count_bound_in_searches():
         S.desc.counts, S.desc.counts2, S.desc.counts3 = saved, saved2, saved3
 
 
+def _want_profile(s, width):
+    """Saturated class counts of the width-byte windows of s (1: byte class
+    b & 15; 2: 4 c2(x) + c2(y); 3: 16 c2(x) + 4 c2(y) + c2(z), c2(b) = (b >> 1) & 3)."""
+    a = np.frombuffer(s, np.uint8).astype(np.int64)
+    if width == 1:
+        cls, size = a & 15, 16
+    else:
+        c2 = (a >> 1) & 3
+        if len(a) < width:
+            return np.zeros(16 if width == 2 else 64, np.uint8)
+        if width == 2:
+            cls, size = 4 * c2[:-1] + c2[1:], 16
+        else:
+            cls, size = 16 * c2[:-2] + 4 * c2[1:-1] + c2[2:], 64
+    return np.minimum(np.bincount(cls, minlength=size), 255).astype(np.uint8)
+
+
+@pytest.mark.parametrize("order", ["set", "length"])
+def test_stripes_hold_every_string_and_profile(order):
+    """Words, lengths and profiles of a built set against a restatement:
+    groups whose bytes fit the build's LDS stage and groups that do not (a
+    20 kB string), empty strings, a ragged last group, 70,000 bytes of one
+    class (a count past 16 bits), and the length-ordered layout."""
+    rng = np.random.default_rng(11)
+    strs = []
+    for i in range(64 * 37 + 5):
+        r = i % 9
+        if r == 0:
+            strs.append(b"")
+        elif r in (1, 2, 3):
+            strs.append(bytes(rng.choice(np.frombuffer(b"ACGT", np.uint8), int(rng.integers(1, 300)))))
+        else:
+            strs.append(bytes(rng.integers(0, 256, int(rng.integers(1, 260))).astype(np.uint8)))
+    strs[64 * 5 + 3] = bytes(rng.integers(0, 256, 20000).astype(np.uint8))
+    strs[64 * 9 + 63] = b"A" * 70000
+    strs[64 * 9 + 62] = b"AC" * 40000
+    C = hip.Packed(*O.pack(strs))
+    n = C.n
+    if order == "set":
+        S = C.stripes(grams3=True)
+        slot = np.arange(n)
+    else:
+        S = C.stripes(order="length")
+        assert S.d_order is not None
+        slot = hip.DeviceArray((n,), np.int32, S.d_order).get().astype(np.int64)
+        assert sorted(slot.tolist()) == list(range(n))
+    groups = (n + 63) // 64
+    gbase = hip.DeviceArray((groups + 1,), np.int64, S.d_gbase).get()
+    words = hip.DeviceArray((int(gbase[-1]),), np.uint32, S.d_words).get()
+    lens = hip.DeviceArray((n,), np.int32, S.d_len).get()
+    have = {}
+    if S.d_counts is not None:
+        have[1] = hip.DeviceArray((n, 16), np.uint8, S.d_counts).get()
+        have[2] = hip.DeviceArray((n, 16), np.uint8, S.d_counts2).get()
+    if S.d_counts3 is not None:
+        have[3] = hip.DeviceArray((n, 64), np.uint8, S.d_counts3).get()
+    assert (order == "set") == (1 in have and 3 in have)
+    for g in range(groups):
+        width = int(gbase[g + 1] - gbase[g]) // 64
+        members = [strs[slot[c]] for c in range(64 * g, min(64 * g + 64, n))]
+        assert width == max(1, max((len(x) + 3) // 4 for x in members)), g
+        block = words[gbase[g]: gbase[g + 1]].reshape(width, 64)
+        for lane, x in enumerate(members):
+            c = 64 * g + lane
+            assert lens[c] == len(x), c
+            padded = np.zeros(4 * width, np.uint8)
+            padded[: len(x)] = np.frombuffer(x, np.uint8)
+            np.testing.assert_array_equal(block[:, lane], padded.view("<u4"), err_msg=f"slot {c}")
+            for w, prof in have.items():
+                np.testing.assert_array_equal(prof[c], _want_profile(x, w), err_msg=f"slot {c} grams {w}")
+        for lane in range(len(members), 64):
+            assert not block[:, lane].any(), g
+
+
+@pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 16384, 16385, 1_000_003])
+def test_device_layout_equals_the_host_layout(n):
+    """bl_stripes_layout_device against bl_stripes_layout on the same
+    offsets: empty strings, a ragged last group, one block of groups and
+    more (the block-total scan), a heavy tail of long strings."""
+    rng = np.random.default_rng(n)
+    lens = rng.integers(0, 70, n)
+    tail = rng.random(n) < 0.01
+    lens[tail] = rng.integers(1000, 100000, int(tail.sum()))
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    want = np.empty((n + 63) // 64 + 1, np.int64)
+    total = call("bl_stripes_layout", off.ctypes.data, n, want.ctypes.data)
+    d_off = hip.DeviceBuffer.from_host(off)
+    d_gbase = hip.DeviceBuffer(8 * len(want))
+    work = hip.DeviceBuffer(call("bl_stripes_layout_workspace_size", n))
+    got = call("bl_stripes_layout_device", d_off.ptr, n, d_gbase.ptr, work.ptr, work.nbytes, None)
+    assert got == total
+    np.testing.assert_array_equal(hip.DeviceArray((len(want),), np.int64, d_gbase).get(), want)
+    if n >= 2:
+        bad = off.copy()
+        bad[n // 2] = bad[n // 2 + 1] + 1          # one string of negative length
+        hip.DeviceBuffer.upload(d_off, bad)
+        with pytest.raises(ValueError, match="offsets decrease"):
+            call("bl_stripes_layout_device", d_off.ptr, n, d_gbase.ptr, work.ptr, work.nbytes, None)
+    with pytest.raises(ValueError, match="workspace"):
+        call("bl_stripes_layout_device", d_off.ptr, n, d_gbase.ptr, work.ptr, 0, None)
+
+
 def test_filter_mode_with_every_choice_open():
     """The four-choice filter leaves every choice open when all are within k
     of the query (here: copies with at most one substitution, k = 3): the
