@@ -92,6 +92,16 @@ BL_HD int ctz32(u32 x)
 
 BL_HD int clz32(u32 x) { return __builtin_clz(x); }
 
+/* v_alignbit_b32: the low 32 bits of (hi:lo) >> (sh & 31). */
+BL_HD u32 funnel32(u32 hi, u32 lo, u32 sh)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_alignbit(hi, lo, sh);
+#else
+    return (u32)((((u64)hi << 32) | lo) >> (sh & 31));
+#endif
+}
+
 /* bytes [sh, sh+4) of the little-endian 8-byte value hi:lo (v_alignbyte_b32). */
 BL_HD u32 align_bytes(u32 hi, u32 lo, u32 sh)
 {
@@ -368,6 +378,39 @@ BL_HD void word_step(T &Pv, T &Mv, T Eq, u32 hp, u32 hn, T &Ph, T &Mh)
     Mv = Phs & Xv;
 }
 
+/* word_step with the carries taken from the block below as the high dwords
+ * of its Ph / Mh (bit 31: the carry; the lowest block gets 0x80000000 and 0,
+ * row 0's +1), so that every shift is one v_alignbit_b32 per dword: (Ph << 1)
+ * | carry without extracting the carry or shifting a 64-bit value. */
+BL_HD u32 hi32(u32 x) { return x; }
+BL_HD u32 hi32(u64 x) { return (u32)(x >> 32); }
+
+template <typename T>
+BL_HD T shift_in(T x, u32 below)
+{
+    if constexpr (sizeof(T) == 4) {
+        return funnel32(x, below, 31);
+    } else {
+        const u32 lo = (u32)x, hi = (u32)(x >> 32);
+        return ((u64)funnel32(hi, lo, 31) << 32) | funnel32(lo, below, 31);
+    }
+}
+
+template <typename T>
+BL_HD void word_step_f(T &Pv, T &Mv, T Eq, u32 ph_below, u32 mh_below, T &Ph, T &Mh)
+{
+    const T Xv = Eq | Mv;
+    const T E = Eq | (T)(mh_below >> 31);
+    const T S = (E & Pv) + Pv;
+    const T Xh = bop3<tt((TA ^ TB) | TC)>(S, Pv, E);            /* (S ^ Pv) | E   */
+    Ph = bop3<tt(TA | ~(TB | TC))>(Mv, Xh, Pv);                 /* Mv | ~(Xh | Pv) */
+    Mh = Pv & Xh;
+    const T Phs = shift_in<T>(Ph, ph_below);
+    const T Mhs = shift_in<T>(Mh, mh_below);
+    Pv = bop3<tt(TA | ~(TB | TC))>(Mhs, Xv, Phs);               /* Mhs | ~(Xv | Phs) */
+    Mv = Phs & Xv;
+}
+
 /* Rows [0, r) of a block (r clamped to [0, rows of T]). */
 template <typename T>
 BL_HD T rows_below(int r)
@@ -503,14 +546,14 @@ BL_HD int diag_bound(const T (&Pv)[NW], const T (&Mv)[NW], int nw, int jj, int m
 template <typename T, int NW>
 BL_HD void column(T (&Pv)[NW], T (&Mv)[NW], int nw, const T (&E)[NW])
 {
-    u32 hp = 1, hn = 0;                    /* row 0 of a global alignment: +1 per column */
+    u32 ph = 0x80000000u, mh = 0;          /* row 0 of a global alignment: +1 per column */
 #pragma unroll
     for (int w = 0; w < NW; w++) {
         if (NW == 1 || w < nw) {
             T Ph, Mh;
-            word_step<T>(Pv[w], Mv[w], E[w], hp, hn, Ph, Mh);
-            hp = (u32)(Ph >> (Rows<T>::n - 1));
-            hn = (u32)(Mh >> (Rows<T>::n - 1));
+            word_step_f<T>(Pv[w], Mv[w], E[w], ph, mh, Ph, Mh);
+            ph = hi32(Ph);
+            mh = hi32(Mh);
         }
     }
 }
@@ -656,17 +699,17 @@ BL_HD int scan_mixed(int m, const Text &text, int n, int k, Eq64 eq64, Eq32 eq32
     for (int w = 0; w < NW64; w++) { Pv[w] = ~(u64)0; Mv[w] = 0; }
     u32 Pt = ~0u, Mt = 0u;
     auto col = [&](u32 c) {
-        u32 hp = 1, hn = 0;
+        u32 ph = 0x80000000u, mh = 0;
 #pragma unroll
         for (int w = 0; w < NW64; w++) {
             u64 Ph, Mh;
-            word_step<u64>(Pv[w], Mv[w], eq64(c, w), hp, hn, Ph, Mh);
-            hp = (u32)(Ph >> 63);
-            hn = (u32)(Mh >> 63);
+            word_step_f<u64>(Pv[w], Mv[w], eq64(c, w), ph, mh, Ph, Mh);
+            ph = hi32(Ph);
+            mh = hi32(Mh);
         }
         if (TAIL) {
             u32 Ph, Mh;
-            word_step<u32>(Pt, Mt, eq32(c), hp, hn, Ph, Mh);
+            word_step_f<u32>(Pt, Mt, eq32(c), ph, mh, Ph, Mh);
         }
     };
     auto bound = [&](int jj) {
@@ -781,15 +824,6 @@ BL_HD int pair_lane_w32(const u8 *a, int la, const u8 *b, int lb, int trim, int 
  * cell is on the final diagonal, inside the band). Cost per column: one T
  * step plus Eq of WIN / 32 + 1 plane words, whatever m is. */
 
-/* v_alignbit_b32: the low 32 bits of (hi:lo) >> (sh & 31). */
-BL_HD u32 funnel32(u32 hi, u32 lo, u32 sh)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_amdgcn_alignbit(hi, lo, sh);
-#else
-    return (u32)((((u64)hi << 32) | lo) >> (sh & 31));
-#endif
-}
 
 /* The largest threshold a window of WIN rows at diagonal offset d0 decides for
  * a pair with m - n = dl (-1: none). */
