@@ -316,6 +316,44 @@ def test_stripes_hold_every_string_and_profile(order):
             assert not block[:, lane].any(), g
 
 
+@pytest.mark.parametrize("shift", [1, 3, 5, 8, 13, 15])
+def test_stripes_build_from_an_unaligned_buffer(shift):
+    """The packed bytes may start anywhere (a caller's sub-buffer): the build
+    stages each group from the aligned 16-byte block holding its first byte,
+    so the words and profiles must not depend on the buffer's alignment."""
+    rng = np.random.default_rng(shift)
+    strs = [bytes(rng.integers(0, 256, int(rng.integers(0, 120))).astype(np.uint8))
+            for _ in range(64 * 5 + 17)]
+    buf, off = O.pack(strs)
+    n = len(strs)
+    d_buf = hip.DeviceBuffer(len(buf) + 32)
+    d_buf.upload(np.frombuffer(bytes(buf), np.uint8), None, shift)
+    d_off = hip.DeviceBuffer.from_host(np.asarray(off, np.int64))
+    groups = (n + 63) // 64
+    gbase = np.empty(groups + 1, np.int64)
+    total = call("bl_stripes_layout", np.asarray(off, np.int64).ctypes.data, n, gbase.ctypes.data)
+    d_gbase = hip.DeviceBuffer.from_host(gbase)
+    d_words, d_len = hip.DeviceBuffer(4 * total), hip.DeviceBuffer(4 * n)
+    d_c1, d_c2, d_c3 = hip.DeviceBuffer(16 * n), hip.DeviceBuffer(16 * n), hip.DeviceBuffer(64 * n)
+    call("bl_stripes_build_order", d_buf.ptr + shift, d_off.ptr, n, None, d_gbase.ptr, d_words.ptr,
+         d_len.ptr, d_c1.ptr, d_c2.ptr, d_c3.ptr, None)
+    call("bl_stream_sync", None)
+    words = hip.DeviceArray((total,), np.uint32, d_words).get()
+    prof = {1: hip.DeviceArray((n, 16), np.uint8, d_c1).get(),
+            2: hip.DeviceArray((n, 16), np.uint8, d_c2).get(),
+            3: hip.DeviceArray((n, 64), np.uint8, d_c3).get()}
+    assert (hip.DeviceArray((n,), np.int32, d_len).get() == np.diff(off)).all()
+    for c, x in enumerate(strs):
+        g, lane = divmod(c, 64)
+        width = int(gbase[g + 1] - gbase[g]) // 64
+        padded = np.zeros(4 * width, np.uint8)
+        padded[: len(x)] = np.frombuffer(x, np.uint8)
+        np.testing.assert_array_equal(words[gbase[g] + lane: gbase[g + 1]: 64], padded.view("<u4"),
+                                      err_msg=f"string {c}")
+        for w, p in prof.items():
+            np.testing.assert_array_equal(p[c], _want_profile(x, w), err_msg=f"string {c} grams {w}")
+
+
 @pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 16384, 16385, 1_000_003])
 def test_device_layout_equals_the_host_layout(n):
     """bl_stripes_layout_device against bl_stripes_layout on the same
