@@ -162,9 +162,9 @@ def _refine(res, full_fn, stream, refine, log, tag, axes):
     for axis, values in axes.items():
         if axis == "band" and not best.get("band"):
             continue
-        if axis in ("static", "gram") and best.get("mode") != "count":
+        if axis in ("static", "gram", "list") and best.get("mode") != "count":
             continue
-        if axis not in best and axis not in ("band", "static", "gram"):
+        if axis not in best and axis not in ("band", "static", "gram", "list"):
             continue
         for v in values:
             q = dict(best, **{axis: v})
@@ -221,7 +221,7 @@ def tune_cdist(Q: hip.Packed, C: hip.Packed, out: hip.DeviceArray, k: int = -1, 
                            stream, 1, log, "cdist",
                            {"waves_per_cu": [8, 16, 32], "tile": [1024, 2048, 4096, 8192, 32768],
                             "qgroup": [16, 32, 64], "static": [False, True],
-                            "gram": [False, True]})
+                            "gram": [False, True], "list": [False, True]})
     else:
         ms, best = min(res, key=lambda r: r[0])
     _store(sig, {"params": best, "ms": ms, "sample_queries": nq})

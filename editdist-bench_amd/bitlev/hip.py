@@ -349,7 +349,8 @@ def _cdist_params(params):
                      "queue": 6}.get(params.get("mode"), 0)
     p.reserved[2] = int(params.get("qgroup", 0))
     p.reserved[1] = int(params.get("grab", 0))
-    p.reserved[3] = (0 if params.get("gram", True) else 1) | (0 if params.get("static", True) else 2)
+    p.reserved[3] = ((0 if params.get("gram", True) else 1) | (0 if params.get("static", True) else 2)
+                     | (0 if params.get("list", True) else 4))
     return p
 
 

@@ -2127,7 +2127,10 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
                     need_ws);
     const size_t ws_bytes = d_work ? work_bytes : bl_cdist_workspace_size(nq, max_qlen);
     const size_t lstart = cdist_list_start(nq, max_qlen);
-    const int64_t ecap = ws_bytes >= lstart + 256 + 8 ? (int64_t)((ws_bytes - lstart - 256) / 8) : 0;
+    /* the escalation list: what the workspace holds past its core, unless
+     * switched off (bit 2 of reserved[3]) */
+    const bool use_list = !(prm && (prm->reserved[3] & 4));
+    const int64_t ecap = use_list && ws_bytes >= lstart + 256 + 8 ? (int64_t)((ws_bytes - lstart - 256) / 8) : 0;
     hipStream_t s = (hipStream_t)stream;
     const int wpc = prm && prm->waves_per_cu > 0 ? prm->waves_per_cu : 16;
     /* tiles start on stripe groups (multiples of 64 choices; of 256 for the

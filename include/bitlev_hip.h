@@ -256,7 +256,11 @@ typedef struct {
                               tested 32 x 32 pairs at a time with an int8 MFMA over
                               the class counts, a squared-L2 bound); bit 1 makes
                               every tile come from the work queue (by default each
-                              wave's first tile is assigned statically) */
+                              wave's first tile is assigned statically); bit 2
+                              leaves the escalation list unused, so the count
+                              kernel scans its near matches itself (for short
+                              queries that costs less than the list kernel's
+                              launch: 1,024 x 1e6 at k = 2, 0.316 -> 0.311 ms) */
 } bl_cdist_params;
 
 typedef struct {

@@ -87,7 +87,8 @@ def test_every_cdist_launch_shape_gives_the_same_results(gen, k):
     want = O.cross(pq, pc, k=k, threads=8)
     space = {"mode": ["strided", "queue", "swar", "gap", "count", "exact"], "waves_per_cu": [4, 32],
              "tile": [64, 1000, 8192]}
-    for p in autotune._grid(space):
+    shapes = list(autotune._grid(space)) + [{"mode": "count", "tile": 1000, "list": False}]
+    for p in shapes:
         out = hip.DeviceArray.empty((Q.n, C.n), np.uint8)
         hip.cdist_batched_into(Q, C, out, k, None, None, p)
         np.testing.assert_array_equal(out.get(), O.saturate(want, np.uint8), err_msg=str(p))
@@ -769,8 +770,9 @@ def test_count_mode_escalation_list_sizes(gen, k):
     entries = min(Q.n * 2048, 1 << 20)
     core = full - 256 - 8 * entries
     assert core > 0
-    p = hip._cdist_params({"mode": "count", "tile": 1024, "qgroup": 16})
-    for size in (core, core + 256 + 8 * 10, core + 256 + 8 * 100, full):
+    for size, use_list in ((core, True), (core + 256 + 8 * 10, True), (core + 256 + 8 * 100, True),
+                           (full, True), (full, False)):
+        p = hip._cdist_params({"mode": "count", "tile": 1024, "qgroup": 16, "list": use_list})
         ws = hip.DeviceBuffer(size)
         out = hip.DeviceArray.empty((Q.n, C.n), np.uint8)
         call("bl_cdist_tuned", Q.d_buf.ptr, Q.d_off.ptr, Q.n, st.ref, mq, out.ptr, hip.DTYPE_CODE[out.dtype], k,
