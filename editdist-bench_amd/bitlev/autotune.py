@@ -96,24 +96,31 @@ def _grid(space):
         yield dict(zip(keys, vals))
 
 
-def _time(fn, stream, reps=3):
-    """Median ms of fn() on stream, measured with HIP events."""
+def _time(fn, stream, reps=None, budget_ms=10.0):
+    """Median ms of fn() on stream, measured with HIP events: reps timed
+    runs after one untimed one; by default as many as fit about budget_ms
+    (3 to 15), so that shapes of sub-millisecond calls, which differ by a few
+    percent, are ranked on more than three samples."""
     h0, h1 = ctypes.c_void_p(), ctypes.c_void_p()
     call("bl_event_create", h0)
     call("bl_event_create", h1)
     e0, e1 = h0.value, h1.value
+
+    def once():
+        call("bl_event_record", e0, stream)
+        fn()
+        call("bl_event_record", e1, stream)
+        call("bl_event_sync", e1)
+        v = ctypes.c_float()
+        call("bl_event_elapsed_ms", e0, e1, v)
+        return v.value
+
     try:
         fn()
         call("bl_stream_sync", stream)
-        ts = []
-        for _ in range(reps):
-            call("bl_event_record", e0, stream)
-            fn()
-            call("bl_event_record", e1, stream)
-            call("bl_event_sync", e1)
-            v = ctypes.c_float()
-            call("bl_event_elapsed_ms", e0, e1, v)
-            ts.append(v.value)
+        ts = [once()]
+        n = reps if reps else int(min(15, max(3, budget_ms / max(ts[0], 1e-3))))
+        ts += [once() for _ in range(n - 1)]
         return float(np.median(ts))
     finally:
         call("bl_event_destroy", e0)
