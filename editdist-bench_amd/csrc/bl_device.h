@@ -929,7 +929,7 @@ BL_HD int band_scan(const u8 *p, int m, const Text &text, int n, int d0, int kq,
     /* one column (byte c), the window staying put */
     auto step = [&](u32 c) {
         T Ph, Mh;
-        word_step<T>(Pv, Mv, eqwin(c), 1u, 0u, Ph, Mh);
+        word_step_f<T>(Pv, Mv, eqwin(c), 0x80000000u, 0u, Ph, Mh);
         V += 1;
     };
     /* one column, then the window one row down. The new deltas shifted down a
