@@ -76,16 +76,28 @@ __global__ __launch_bounds__(64) void stripes_build_kernel(const u8 *__restrict_
     const int64_t c = c0 + lane;
     const int rows = counts3 ? 96 : (counts || counts2 ? 32 : 0);
     for (int x = 0; x < rows; x++) h[x][lane] = 0;
+    int ln = 0;
+    int64_t o = 0;
+    if (c < n) {
+        const int64_t src = order ? (int64_t)order[c] : c;
+        o = off[src];
+        ln = (int)(off[src + 1] - o);
+        len[c] = ln;
+    }
+    const u8 *s = buf + o;
     /* stage the group's bytes when they lie together: the aligned 16-byte
      * blocks from the one holding byte off[c0] up to the one holding byte
      * off[c1] - 1 (an aligned block never crosses a page, and each holds a
-     * byte of the buffer) */
+     * byte of the buffer); only when every lane's string lies in that span
+     * (offsets that decrease, a caller error, take the direct reads) */
     bool staged = false;
     const u8 *pa = buf;
     if (!order) {
-        const u8 *p0 = buf + off[c0], *p1 = buf + off[c1];
+        const int64_t s0 = off[c0], s1 = off[c1];
+        const u8 *p0 = buf + s0, *p1 = buf + s1;
         pa = (const u8 *)((uintptr_t)p0 & ~(uintptr_t)15);
-        staged = p1 > p0 && p1 - pa <= STAGE;
+        const bool inside = c >= n || (ln >= 0 && o >= s0 && o + ln <= s1);
+        staged = p1 > p0 && p1 - pa <= STAGE && __all(inside ? 1 : 0);
         if (staged) {
             const int64_t span = p1 - pa;
 #pragma unroll 16
@@ -96,15 +108,6 @@ __global__ __launch_bounds__(64) void stripes_build_kernel(const u8 *__restrict_
     __syncthreads();
     const int64_t gb = gbase[g];
     const int64_t width = (gbase[g + 1] - gb) >> 6;
-    int ln = 0;
-    const u8 *s = buf;
-    if (c < n) {
-        const int64_t src = order ? (int64_t)order[c] : c;
-        const int64_t o = off[src];
-        ln = (int)(off[src + 1] - o);
-        s = buf + o;
-        len[c] = ln;
-    }
     const u8 *ls = stage + (s - pa);          /* (staged) the string in LDS */
     u32 b0 = 0, b1 = 0;                       /* the two bytes before the current one */
     /* four dwords a step: their LDS reads issue together, one wait a step */
