@@ -51,6 +51,9 @@ PAIRS_DIRECT_SPACE = {"mode": ["direct"], "waves_per_cu": [8, 16, 32], "block": 
                       "trim": [True, False]}
 # band tiles from this many 32-row units up (tried on the best bucketed shapes)
 PAIRS_BAND_UNITS = [2, 3, 4, 5]
+# lanes per pair: pairs above this many 32-row units get a whole wave each
+# (the anti-diagonal wavefront) instead of one lane (8: only above 256 bytes)
+PAIRS_LONG_UNITS = [4, 6]
 
 _lock = threading.Lock()
 
@@ -171,7 +174,9 @@ def _refine(res, full_fn, stream, refine, log, tag, axes):
             continue
         if axis in ("static", "gram", "list") and best.get("mode") != "count":
             continue
-        if axis not in best and axis not in ("band", "static", "gram", "list"):
+        if axis in ("long_units", "unit64") and (best.get("mode") is not None or "waves_per_cu" not in best):
+            continue                          # bucketed pairs shapes only
+        if axis not in best and axis not in ("band", "static", "gram", "list", "long_units", "unit64"):
             continue
         for v in values:
             q = dict(best, **{axis: v})
@@ -265,12 +270,19 @@ def tune_pairs(A: hip.Packed, B: hip.Packed, out: hip.DeviceArray, k: int = -1, 
         for _, p in bucketed[:2]:
             for b in PAIRS_BAND_UNITS:
                 run(dict(p, band=b))
+        # lanes per pair (a whole wave for pairs above this many units) and
+        # words per lane (64-row words only) on the best bucketed shape
+        best_b = min((r for r in res if r[1].get("mode") != "direct"), key=lambda r: r[0])[1]
+        for lu in PAIRS_LONG_UNITS:
+            run(dict(best_b, long_units=lu))
+        run(dict(best_b, unit64=True))
     if n < A.n and refine > 0:
         ms, best = _refine(res, lambda p: hip.pairs_batched_into(A, B, out, k, trim, stream, ws, p),
                            stream, 2, log, "pairs",
                            {"band": PAIRS_BAND_UNITS, "waves_per_cu": [8, 16, 32],
                             "chunk": [16384, 65536, 262144], "bin_width": [8, 16, 32],
-                            "block": [64, 256]})
+                            "block": [64, 256], "long_units": PAIRS_LONG_UNITS + [8],
+                            "unit64": [False, True]})
     else:
         ms, best = min(res, key=lambda r: r[0])
     _store(sig, {"params": best, "ms": ms, "sample_pairs": n})

@@ -364,7 +364,8 @@ def _pairs_params(params):
     p.reserved[0] = int(params.get("chunk", 0))
     p.reserved[1] = 1 if params.get("mode") == "direct" else 0
     p.reserved[2] = int(params.get("grab", 0))
-    p.reserved[3] = int(params.get("band", 0))
+    p.reserved[3] = (int(params.get("band", 0)) | (int(params.get("long_units", 0)) << 4)
+                     | ((1 if params.get("unit64", False) else 0) << 8))
     return p
 
 

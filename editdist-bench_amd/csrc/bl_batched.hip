@@ -1686,8 +1686,8 @@ __device__ __forceinline__ int pair_k(int k_all, const int32_t *kvec, int64_t i)
 
 __global__ __launch_bounds__(1024) void pairs_classify_kernel(
     const int64_t *__restrict__ oa, const int64_t *__restrict__ ob, int64_t n, void *out, int dtype,
-    int k_all, const int32_t *__restrict__ kvec, int bin_width, int band, int64_t chunk,
-    unsigned short *key, unsigned long long *hist)
+    int k_all, const int32_t *__restrict__ kvec, int bin_width, int band, int long_units, int unit64,
+    int64_t chunk, unsigned short *key, unsigned long long *hist)
 {
     __shared__ unsigned int lh[PKEYS];
     for (int x = threadIdx.x; x < PKEYS; x += blockDim.x) lh[x] = 0;
@@ -1696,7 +1696,7 @@ __global__ __launch_bounds__(1024) void pairs_classify_kernel(
     if (i < n) {
         int result = 0;
         const int kk = pair_key((int)(oa[i + 1] - oa[i]), (int)(ob[i + 1] - ob[i]),
-                                pair_k(k_all, kvec, i), bin_width, band, result);
+                                pair_k(k_all, kvec, i), bin_width, band, long_units, unit64 != 0, result);
         if (kk == 0) store_out(out, i, result, dtype);
         key[i] = (unsigned short)kk;
         if (kk) atomicAdd(&lh[kk], 1u);
@@ -2428,7 +2428,13 @@ int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t
     const int wg = prm && (prm->block == 64 || prm->block == 128 || prm->block == 256) ? prm->block : 256;
     const bool direct = prm && prm->reserved[1] == 1;
     const int grab = prm && prm->reserved[2] > 0 ? prm->reserved[2] : 1;
-    const int band = prm && prm->reserved[3] > 0 ? prm->reserved[3] : 0;
+    /* reserved[3]: bits 0-3 band tiles from this many units, bits 4-7 whole
+     * waves for pairs of more units (0: 8), bit 8 64-row words */
+    const int r3 = prm && prm->reserved[3] > 0 ? prm->reserved[3] : 0;
+    const int band = r3 & 15;
+    const int lu = (r3 >> 4) & 15;
+    const int long_units = lu >= 1 && lu <= 8 ? lu : 8;
+    const int unit64 = (r3 >> 8) & 1;
 
     char *w = (char *)d_work;
     if (!w) BL_HIP(hipMallocAsync((void **)&w, L.total, s));
@@ -2447,7 +2453,14 @@ int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t
 
     int rc = BL_OK;
     int64_t blocks = (int64_t)device_cus() * wpc * 64 / wg;
-    const int64_t maxb = (n + 63) / 64 * 64 / wg + 1;
+    /* no more waves than tiles: 64 pairs a tile, but a whole wave for each
+     * long pair when there can be any (so a handful of long pairs do not
+     * queue on the few waves 64 pairs a tile would give them) */
+    int64_t maxb = (n + 63) / 64 * 64 / wg + 1;
+    if (!direct && (max_len > 256 || long_units < 8)) {
+        const int64_t mb = (n + wg / 64 - 1) / (wg / 64) + 1;
+        if (mb > maxb) maxb = mb;
+    }
     if (blocks > maxb) blocks = maxb;
     if (scratch && blocks * (wg / 64) > 2048) blocks = 2048 / (wg / 64);
     if (direct) {
@@ -2469,7 +2482,8 @@ int bl_pairs_tuned(const uint8_t *d_buf_a, const int64_t *d_off_a, const uint8_t
         const unsigned cblocks = (unsigned)((n + 1023) / 1024);  /* 1024 pairs per block */
         if (rc == BL_OK) {
             pairs_classify_kernel<<<cblocks, 1024, 0, s>>>(d_off_a, d_off_b, n, d_out, dtype, k, d_kvec,
-                                                          bw, band, L.chunk, key, hist);
+                                                          bw, band, long_units, unit64, L.chunk, key,
+                                                          hist);
             pairs_chunk_scan_kernel<<<(unsigned)L.nchunks, 256, 0, s>>>(hist, cursor, chunk_cnt,
                                                                         tile_cnt);
             prefix_kernel<<<1, 1024, 0, s>>>(chunk_cnt, chunk_base, tile_cnt, tile_base, L.nchunks);

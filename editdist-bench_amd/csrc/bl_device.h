@@ -1484,11 +1484,17 @@ BL_HD int long_prepare(const u8 *&p, int &m, const u8 *&t, int &n, int is_pair, 
  * the same amount, so the pattern words of the untrimmed shorter string bound
  * the trimmed ones and the length gap is unchanged):
  *   1 + (w - 1) * LB_BINS + min(max(la, lb) / bin_width, LB_BINS - 1)
- *       for a shorter string of w <= 8 units of 32 bytes (whole-pattern tiles);
+ *       for a shorter string of w <= long_units units of 32 bytes
+ *       (whole-pattern tiles, one lane per pair); with unit64 w counts
+ *       64-byte words twice (w even: the lanes hold u64 blocks only, in fewer
+ *       and fuller buckets);
  *   PKEY_BAND0 + that length bin * GAP_BINS + |la - lb| / 4
  *       instead when band > 0, w >= band and the gap is at most 63 (band tiles:
  *       one window offset serves a tile well when its gaps are close);
- *   PKEY_LONG for a shorter string above 256 bytes;
+ *   PKEY_LONG for a shorter string of more units (above 256 bytes by
+ *       default; long_units < 8 sends shorter ones there too): one whole wave
+ *       per pair on the anti-diagonal wavefront, which for a handful of long
+ *       pairs finishes sooner than lanes of their own;
  *   0 for a pair the gap or an empty string decides.
  * Larger keys mean more work; the pipeline schedules large keys first. */
 constexpr int LB_BINS = 64;
@@ -1499,13 +1505,15 @@ constexpr int PKEYS = PKEY_LONG + 1;
 
 /* Key of a pair of lengths la, lb; for key 0, *result holds the answer
  * (the gap rule, csrc/myers.c:137, and the empty rule, csrc/myers.c:90-99). */
-BL_HD int pair_key(int la, int lb, int k, int bin_width, int band, int &result)
+BL_HD int pair_key(int la, int lb, int k, int bin_width, int band, int long_units, bool unit64,
+                  int &result)
 {
     const int mn = la < lb ? la : lb, mx = la < lb ? lb : la;
     if (k >= 0 && mx - mn > k) { result = k + 1; return 0; }
     if (mn == 0) { result = mx; return 0; }
-    const int w = (mn + 31) >> 5;                 /* 32-row units of the pattern */
-    if (w > 8) return PKEY_LONG;
+    if (mn > 256) return PKEY_LONG;
+    const int w = unit64 ? 2 * ((mn + 63) >> 6) : (mn + 31) >> 5;   /* 32-row units */
+    if (w > long_units) return PKEY_LONG;
     int bin = mx / bin_width;
     bin = bin < LB_BINS - 1 ? bin : LB_BINS - 1;
     if (band > 0 && w >= band && mx - mn <= 63) return PKEY_BAND0 + bin * GAP_BINS + (mx - mn) / 4;

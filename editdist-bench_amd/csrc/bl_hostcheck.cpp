@@ -139,7 +139,7 @@ static int blh_pairs_batched_core(const u8 *ba, const int64_t *oa, const u8 *bb,
     for (int64_t i = 0; i < n; i++) {
         int result = 0;
         const int kk = pair_key((int)(oa[i + 1] - oa[i]), (int)(ob[i + 1] - ob[i]), k, bin_width,
-                                band, result);
+                                band, 8, false, result);
         if (kk < 0 || kk >= PKEYS) return -10;
         if (kk == 0) out[i] = result;
         key[(size_t)i] = kk;

@@ -270,11 +270,17 @@ typedef struct {
     int reserved[4];       /* [0]: pairs per bucketing chunk (default 65536);
                               [1]: 1 = direct mode (one launch, no bucketing);
                               [2]: tiles per work-queue ticket (default 1);
-                              [3]: tiles of at least this many 32-row units (1..8)
-                              scan a diagonal band first (Ukkonen: an alignment of
-                              cost <= k stays within k + 1 diagonals), pairs the
-                              band cannot decide then the whole pattern (default
-                              0: no band tiles) */
+                              [3]: bits 0-3: tiles of at least this many 32-row
+                              units (1..8) scan a diagonal band first (Ukkonen: an
+                              alignment of cost <= k stays within k + 1 diagonals),
+                              pairs the band cannot decide then the whole pattern
+                              (default 0: no band tiles); bits 4-7 (lanes per
+                              pair): pairs whose shorter string needs more than
+                              this many 32-row units (1..8; default 0 = 8, i.e.
+                              above 256 bytes) run one whole wave per pair on the
+                              anti-diagonal wavefront instead of a lane each;
+                              bit 8 (words per lane): lanes hold their pattern in
+                              64-row words only, no 32-row tail unit */
 } bl_pairs_params;
 
 /* Bytes of device workspace bl_cdist needs for nq queries of at most

@@ -40,7 +40,10 @@ def test_every_pairs_launch_shape_gives_the_same_results():
     space = {"waves_per_cu": [4, 32], "block": [64, 256], "bin_width": [1, 64],
              "chunk": [256, 1 << 20], "band": [0, 1]}
     direct = {"mode": ["direct"], "waves_per_cu": [1, 32], "block": [64, 256]}
-    for p in list(autotune._grid(space)) + list(autotune._grid(direct)):
+    # lanes per pair (a whole wave above this many units) and words per lane
+    lanes = {"waves_per_cu": [32], "chunk": [256, 1 << 20], "band": [0, 3],
+             "long_units": [1, 4, 7], "unit64": [False, True]}
+    for p in list(autotune._grid(space)) + list(autotune._grid(direct)) + list(autotune._grid(lanes)):
         for k in (-1, 5):
             out = hip.DeviceArray.empty((A.n,), np.int32)
             hip.pairs_batched_into(A, B, out, k, True, None, None, p)
@@ -64,7 +67,8 @@ def test_band_tiles_on_uneven_pairs(band):
         cut = int(rng.integers(0, 30))
         Ah.append(bytes(a))
         Bh.append(bytes(b[cut:]) + bytes(rng.integers(0, 256, int(rng.integers(0, 30))).astype(np.uint8)))
-    p = {"waves_per_cu": 16, "block": 256, "bin_width": 8, "chunk": 16384, "band": band}
+    p = {"waves_per_cu": 16, "block": 256, "bin_width": 8, "chunk": 16384, "band": band,
+         "unit64": band == 3, "long_units": 6 if band == 5 else 0}
     for pa, pb in ((A, B), (O.pack(Ah), O.pack(Bh))):
         X, Y = hip.Packed(*pa), hip.Packed(*pb)
         exact = O.pairs(pa, pb, threads=8)
