@@ -904,6 +904,8 @@ __global__ __launch_bounds__(256) void cdist_batched_kernel(
  * slot c holds choice order[c]: the lanes of a wave scan strings of about the
  * same length and finish together, and each result goes to the caller's
  * column order[c]. */
+constexpr int PEQ_STRIDE = 257;              /* u64 entries per word of an LDS Peq table */
+
 template <int NWMAX, typename Sink>
 __global__ __launch_bounds__(256) void cdist_exact_kernel(
     const int64_t *__restrict__ oq, int64_t nq, const u64 *__restrict__ gpeq,
@@ -911,7 +913,12 @@ __global__ __launch_bounds__(256) void cdist_exact_kernel(
     const int32_t *__restrict__ clen, const int32_t *__restrict__ order, int64_t nc, Sink sink,
     int k_all, const int32_t *__restrict__ kvec, int64_t tile)
 {
-    __shared__ u64 T[NWMAX * 256];
+    /* word-major, word w of byte c at T[w * PEQ_STRIDE + c]: one u64 read per
+     * word, lanes with bytes less than 32 apart on distinct banks; the stride
+     * of 257 keeps the compiler from pairing two words of a byte in one
+     * ds_read2 (their entries then sat on the same banks: 0.65 bank-conflict
+     * cycles per LDS cycle on the serving-shaped search) */
+    __shared__ u64 T[NWMAX * PEQ_STRIDE];
     /* workgroups go round-robin to the XCDs (b % 8), so with the query index
      * fastest query q would run on XCD q % 8 for every block and an XCD's
      * share would follow its queries' lengths; rotating by the block spreads
@@ -933,14 +940,14 @@ __global__ __launch_bounds__(256) void cdist_exact_kernel(
         return;
     }
     const u64 *qp = gpeq + (size_t)q * 256 * NWMAX;
-    for (int i = threadIdx.x; i < 256 * NWMAX; i += 256) T[(i % NWMAX) * 256 + i / NWMAX] = qp[i];
+    for (int i = threadIdx.x; i < 256 * NWMAX; i += 256) T[(i % NWMAX) * PEQ_STRIDE + i / NWMAX] = qp[i];
     __syncthreads();
     const int lane = lane_id();
     const int64_t first = c0 + 64 * (threadIdx.x >> 6);
     auto run = [&](auto uc) {
         constexpr int U = decltype(uc)::value;
-        auto eq64 = [&](u32 ch, int w) { return T[w * 256 + ch]; };
-        auto eq32 = [&](u32 ch, int u) { return ((const u32 *)T)[2 * ((u >> 1) * 256 + ch) + (u & 1)]; };
+        auto eq64 = [&](u32 ch, int w) { return T[w * PEQ_STRIDE + ch]; };
+        auto eq32 = [&](u32 ch, int u) { return ((const u32 *)T)[2 * ((u >> 1) * PEQ_STRIDE + ch) + (u & 1)]; };
         for (int64_t s = first; s < c1; s += 256) {
             const int64_t c = s + lane;
             if (c < c1) {

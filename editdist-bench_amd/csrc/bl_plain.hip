@@ -67,29 +67,44 @@ struct StripeChoices {
     BL_HDM StripeText text(int64_t c) const { return StripeText{words + gbase[c >> 6] + (c & 63)}; }
 };
 
+/* The query's Peq table in LDS, word-major: word w of byte c at
+ * t[w * STRIDE + c], so that one u64 read per word puts lanes whose bytes
+ * are less than 32 apart on distinct banks (byte-major rows of NW words gave
+ * bytes 64 / NW apart the same banks: 0.43 bank-conflict cycles per LDS
+ * cycle for two-word queries), and the stride of 257 keeps two words of a
+ * byte out of one ds_read2. Read through peq_lane's row-major indexing. */
+template <int NW>
+struct WordMajorPeq {
+    static constexpr int STRIDE = 257;
+    const u64 *t;
+    BL_HDM u64 operator[](int i) const { return t[(i % NW) * STRIDE + i / NW]; }
+};
+
 template <int NW, typename Choices>
 __global__ __launch_bounds__(THREADS) void cdist_class_kernel(
     const u8 *__restrict__ bq, const int64_t *__restrict__ oq, const int32_t *__restrict__ qidx,
     const Choices ch, int64_t nc, void *out, int dtype, int k)
 {
-    __shared__ u64 peq[256 * NW];
+    constexpr int STRIDE = WordMajorPeq<NW>::STRIDE;
+    __shared__ u64 peq[STRIDE * NW];
     __shared__ u32 lo32[256];               /* rows 0..31: the 32-row stage's table */
     const int64_t q = qidx[blockIdx.x];
     const int64_t q0 = oq[q];
     const int m = (int)(oq[q + 1] - q0);
     const bool fits = m <= 64 * NW;         /* a longer query is a caller error */
-    for (int x = threadIdx.x; x < 256 * NW; x += THREADS) peq[x] = 0;
+    for (int x = threadIdx.x; x < STRIDE * NW; x += THREADS) peq[x] = 0;
     __syncthreads();
     for (int i = threadIdx.x; fits && i < m; i += THREADS)
-        atomicOr((unsigned long long *)&peq[(int)bq[q0 + i] * NW + (i >> 6)],
+        atomicOr((unsigned long long *)&peq[(i >> 6) * STRIDE + (int)bq[q0 + i]],
                  (unsigned long long)1 << (i & 63));
     __syncthreads();
-    lo32[threadIdx.x] = (u32)peq[threadIdx.x * NW];
+    lo32[threadIdx.x] = (u32)peq[threadIdx.x];
     __syncthreads();
+    const WordMajorPeq<NW> tab{peq};
     const int64_t stride = (int64_t)gridDim.y * THREADS;
     for (int64_t c = (int64_t)blockIdx.y * THREADS + threadIdx.x; c < nc; c += stride) {
         const int n = ch.len(c);
-        const int d = !fits ? -1 : m == 0 ? empty_pattern(n, k) : peq_lane<NW, 0>(lo32, peq, m, ch.text(c), n, k);
+        const int d = !fits ? -1 : m == 0 ? empty_pattern(n, k) : peq_lane<NW, 0>(lo32, tab, m, ch.text(c), n, k);
         store_out(out, q * nc + c, d, dtype);
     }
 }
