@@ -26,7 +26,8 @@ run
     the scanning kernels) and the measured HBM traffic from the committed
     counter profiles;
   * times a cold call (the choice set's stripes and profiles built inside the
-    timed region) and the match-list output;
+    timed region), the match-list output and the dense result's copy-out to
+    pinned host memory (GB/s over PCIe, and search plus copy as one rate);
   * adds compact records of the other benchmark workloads (65,536 ACGT pairs
     of 32 bytes; 16.8M pairs of 1-256 bytes; 256 x 4.19M DNA search at
     max_dist=8) and of two exact searches (the headline shape without
@@ -713,6 +714,48 @@ def matches_leg(W_, ev, args):
             "output_bytes": int(n * 16), "equal_to_dense": ok}
 
 
+def copy_out_leg(W_, ev):
+    """The dense result copied to pinned host memory after the search, as a
+    caller that wants it on the host (return_device=False) waits for it:
+    device-to-host GB/s over the PCIe link, and search plus copy-out as one
+    rate. Median of 3 runs of each, HIP events on the workload's stream."""
+    from bitlev import hip
+    from bitlev._capi import call
+    from bitlev.ingest import _Pinned
+    sp = W_.stream.ptr
+    nbytes = int(np.prod(W_.out.shape)) * W_.out.dtype.itemsize
+    pin = _Pinned(nbytes)
+    ws = hip.Workspace()
+    step = W_.step("batched", W_.params, ws)
+    try:
+        def copy():
+            call("bl_memcpy_dtoh", pin.ptr, W_.out.ptr, nbytes, sp)
+        copy()
+        W_.stream.sync()
+        copies, both = [], []
+        for _ in range(3):
+            a, b, c = ev.new(), ev.new(), ev.new()
+            ev.record(a)
+            copy()
+            ev.record(b)
+            W_.stream.sync()
+            copies.append(ev.ms(a, b))
+            ev.record(a)
+            step()
+            copy()
+            ev.record(c)
+            W_.stream.sync()
+            both.append(ev.ms(a, c))
+            ev.destroy(a, b, c)
+        cm, bm = float(np.median(copies)), float(np.median(both))
+        return {"bytes": nbytes, "copy_ms": round(cm, 3), "gb_s": round(nbytes / (cm / 1e3) / 1e9, 1),
+                "search_and_copy_ms": round(bm, 3),
+                "value": round(W_.pairs_total / (bm / 1e3) / 1e6, 1), "unit": "M pairs/s",
+                "note": "into pinned host memory (hipHostMalloc); PCIe 5.0 x16 carries 64 GB/s a direction"}
+    finally:
+        pin.free()
+
+
 def ingest_leg(W_, n):
     """End-to-end pairwise from Python lists of bytes (the first n pairs of the
     workload): the staged pipeline (chunks packed on a thread pool into pinned
@@ -783,6 +826,8 @@ def bench_workload(name, args, seed, ev, stream, full):
             rec["cold_call"] = cold_call(W_, ev)
         if W_.k >= 0 and "batched" in measured:
             rec["matches"] = matches_leg(W_, ev, args)
+        if "batched" in measured:
+            rec["copy_out"] = copy_out_leg(W_, ev)
     if full and W_.kind == "pairs" and args.ingest_pairs > 0 and "batched" in measured:
         rec["ingest"] = ingest_leg(W_, args.ingest_pairs)
     W_.free()
