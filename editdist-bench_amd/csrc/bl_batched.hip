@@ -252,8 +252,18 @@ __global__ __launch_bounds__(256) void peq_build_kernel(const u8 *__restrict__ b
                                                         const int64_t *__restrict__ oq, int nwmax,
                                                         u64 *__restrict__ peq, Profile *__restrict__ qcnt,
                                                         Profile *__restrict__ qcnt2,
-                                                        Profile3 *__restrict__ qcnt3)
+                                                        Profile3 *__restrict__ qcnt3,
+                                                        unsigned long long *clr, int nclr,
+                                                        unsigned long long *clr1,
+                                                        unsigned long long *clr2)
 {
+    /* block 0 also zeroes the search's counters (the work queues' nclr and
+     * up to two single ones), saving their own launches */
+    if (blockIdx.x == 0) {
+        for (int i = threadIdx.x; i < nclr; i += 256) clr[i] = 0;
+        if (threadIdx.x == 0 && clr1) *clr1 = 0;
+        if (threadIdx.x == 0 && clr2) *clr2 = 0;
+    }
     __shared__ u8 qs[512];
     __shared__ u32 cls[16];
     __shared__ u32 cls2[16];
@@ -2186,11 +2196,12 @@ static int cdist_launch(const char *who, const uint8_t *d_buf_q, const int64_t *
     /* the 3-gram profiles after the 2-gram ones (64 bytes a query) */
     const Profile3 *ccnt3 = ccnt2 ? (const Profile3 *)choices->counts3 : nullptr;
     Profile3 *qcnt3 = ccnt3 ? (Profile3 *)((Profile *)(ws + cdist_peq_end(nq, max_qlen)) + 2 * nq) : nullptr;
-    int rc = launch_clear(ctr, QUEUE_BYTES / 8, s);
-    if (rc == BL_OK && extra_clear) rc = launch_clear(extra_clear, 1, s);
-    if (rc == BL_OK && mode == 4 && elist_count) rc = launch_clear(elist_count, 1, s);
-    if (rc == BL_OK) {
-        peq_build_kernel<<<(unsigned)nq, 256, 0, s>>>(d_buf_q, d_off_q, nwmax, gpeq, qcnt, qcnt2, qcnt3);
+    int rc = BL_OK;
+    {
+        /* the Peq tables and profiles; block 0 zeroes the counters */
+        peq_build_kernel<<<(unsigned)nq, 256, 0, s>>>(d_buf_q, d_off_q, nwmax, gpeq, qcnt, qcnt2, qcnt3,
+                                                     ctr, (int)(QUEUE_BYTES / 8), extra_clear,
+                                                     mode == 4 ? elist_count : nullptr);
         auto capped = [&](auto kernel, size_t lds_bytes) {
             const int64_t cap = resident_blocks(kernel, 256, lds_bytes);
             return (unsigned)(blocks < cap ? blocks : cap);
