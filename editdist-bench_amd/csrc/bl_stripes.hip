@@ -211,17 +211,25 @@ __global__ __launch_bounds__(256) void layout_widths_kernel(const int64_t *__res
     const int64_t g0 = (int64_t)blockIdx.x * LAYOUT_GROUPS + 64 * wv;
     int64_t widest = 0;                 /* lane i: group g0 + i */
     bool bad = false;
-    for (int i = 0; i < 64 && g0 + i < groups; i++) {
-        const int64_t c = 64 * (g0 + i) + lane;
-        const int64_t l = c < n ? off[c + 1] - off[c] : 0;
-        bad |= l < 0;
-        int64_t mx = l;
+    /* eight groups a step: their sixteen offset loads issue together */
+    for (int i0 = 0; i0 < 64 && g0 + i0 < groups; i0 += 8) {
+        int64_t l[8];
 #pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            const int64_t o = __shfl_xor(mx, d, 64);
-            mx = o > mx ? o : mx;
+        for (int u = 0; u < 8; u++) {
+            const int64_t c = 64 * (g0 + i0 + u) + lane;
+            l[u] = g0 + i0 + u < groups && c < n ? off[c + 1] - off[c] : 0;
         }
-        if (lane == i) widest = mx;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            bad |= l[u] < 0;
+            int64_t mx = l[u];
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) {
+                const int64_t o = __shfl_xor(mx, d, 64);
+                mx = o > mx ? o : mx;
+            }
+            if (lane == i0 + u) widest = mx;
+        }
     }
     const int64_t g = g0 + lane;
     const int64_t w = g < groups ? 64 * (widest > 0 ? (widest + 3) / 4 : 1) : 0;
