@@ -40,20 +40,27 @@ namespace {
  * 1 KB of consecutive bytes); every lane then builds its stripe words from
  * LDS. Otherwise each lane reads its own string's dwords from global memory.
  * Either way the profiles are counted from the bytes the words are built
- * from, with LDS adds (no return value, so nothing waits on them) into
- * per-lane u32 histograms, counter x of lane l at h[x][l] (bank l whatever
- * x): byte classes, 2-gram classes and, with counts3, 3-gram classes.
- * (Reading every string three times with scattered loads -- the words, then
- * byte by byte for each profile -- took 13.9 ms for 4.2M strings of 200-256
- * bytes.) */
+ * from, 32 bytes a step. The byte-class and 2-gram counts are bit-sliced in
+ * registers: an 8 x 8 bit transpose gives the planes of bits 0-3 of the 32
+ * bytes, and each of the 16 classes adds the popcount of the AND of its four
+ * planes (or their complements) under the mask of valid positions; a 2-gram
+ * class takes bits 1-2 of the byte and of the one before it (the planes
+ * shifted by one position, the previous step's last byte shifted in). With
+ * counts3 all three profiles go through per-lane u32 histograms in LDS
+ * instead, counter x of lane l at h[x][l] (bank l whatever x), by adds that
+ * return nothing. (The LDS histograms for the first two cost their LDS in
+ * occupancy: 1.44 ms for 4.2M strings of 200-256 bytes against 0.56 ms
+ * without profiles; reading every string three times with scattered loads
+ * took 13.9 ms.) */
 typedef u32 v4u __attribute__((ext_vector_type(4)));
 constexpr int STAGE = 16384;                 /* bytes of a group staged in LDS */
 constexpr int H1 = 0, H2 = 16, H3 = 32;      /* histogram rows: classes, 2-grams, 3-grams */
 
-static size_t stripes_build_lds(bool counts, bool counts3)
+constexpr int STAGE_SPARE = 48;              /* read past the span, masked off */
+
+static size_t stripes_build_lds(bool counts3)
 {
-    const int rows = counts3 ? 96 : (counts ? 32 : 0);
-    return (size_t)STAGE + 32 + (size_t)rows * 64 * sizeof(u32);
+    return (size_t)STAGE + STAGE_SPARE + (counts3 ? (size_t)96 * 64 * sizeof(u32) : 0);
 }
 
 __global__ __launch_bounds__(64) void stripes_build_kernel(const u8 *__restrict__ buf,
@@ -68,13 +75,14 @@ __global__ __launch_bounds__(64) void stripes_build_kernel(const u8 *__restrict_
                                                            Profile3 *__restrict__ counts3)
 {
     extern __shared__ __align__(16) u32 lds[];
-    u8 *stage = (u8 *)lds;                                      /* STAGE + 32 bytes */
-    u32(*h)[64] = (u32(*)[64])(stage + STAGE + 32);              /* 32 or 96 rows of 64 lanes */
+    u8 *stage = (u8 *)lds;                                      /* STAGE + STAGE_SPARE */
+    u32(*h)[64] = (u32(*)[64])(stage + STAGE + STAGE_SPARE);     /* with counts3: 96 rows */
     const int64_t g = blockIdx.x;
     const int lane = (int)threadIdx.x;
     const int64_t c0 = g * 64, c1 = c0 + 64 < n ? c0 + 64 : n;
     const int64_t c = c0 + lane;
-    const int rows = counts3 ? 96 : (counts || counts2 ? 32 : 0);
+    const int rows = counts3 ? 96 : 0;
+    const bool bits = !counts3 && (counts || counts2);
     for (int x = 0; x < rows; x++) h[x][lane] = 0;
     int ln = 0;
     int64_t o = 0;
@@ -110,72 +118,108 @@ __global__ __launch_bounds__(64) void stripes_build_kernel(const u8 *__restrict_
     const int64_t width = (gbase[g + 1] - gb) >> 6;
     const u8 *ls = stage + (s - pa);          /* (staged) the string in LDS */
     u32 b0 = 0, b1 = 0;                       /* the two bytes before the current one */
-    /* four dwords a step: their LDS reads issue together, one wait a step */
-    for (int64_t j0 = 0; j0 < width; j0 += 4) {
-        u32 w[4] = {0, 0, 0, 0};
+    u32 n1[16], n2[16];                       /* bit-sliced class and 2-gram counts */
+#pragma unroll
+    for (int x = 0; x < 16; x++) n1[x] = n2[x] = 0;
+    u32 prev1 = 0, prev2 = 0;                 /* bit planes 1 and 2 of the previous step */
+    /* eight dwords a step: their LDS reads issue together, one wait a step */
+    for (int64_t j0 = 0; j0 < width; j0 += 8) {
+        u32 w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         if (4 * j0 < ln) {
             if (staged) {
-                /* five aligned LDS dwords from the one holding byte 4 j0 (the
-                 * stage starts 16-byte aligned; the reads end at most 19
-                 * bytes past the staged span, inside the stage's 32 spare
+                /* nine aligned LDS dwords from the one holding byte 4 j0 (the
+                 * stage starts 16-byte aligned; the reads end at most 35
+                 * bytes past the staged span, inside the stage's spare
                  * bytes, and what passes the string is masked off) */
                 const u8 *p = ls + 4 * j0;
                 const u32 sh = (u32)((uintptr_t)p & 3);
                 const u32 *al = (const u32 *)(p - sh);
-                u32 d[5];
+                u32 d[9];
 #pragma unroll
-                for (int k = 0; k < 5; k++) d[k] = al[k];
+                for (int k = 0; k < 9; k++) d[k] = al[k];
 #pragma unroll
-                for (int k = 0; k < 4; k++) w[k] = align_bytes(d[k + 1], d[k], sh);
+                for (int k = 0; k < 8; k++) w[k] = align_bytes(d[k + 1], d[k], sh);
             } else {
 #pragma unroll
-                for (int k = 0; k < 4; k++) w[k] = j0 + k < width ? stripe_word(s, ln, j0 + k) : 0u;
+                for (int k = 0; k < 8; k++) w[k] = j0 + k < width ? stripe_word(s, ln, j0 + k) : 0u;
             }
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
+            for (int k = 0; k < 8; k++) {
                 const int64_t r = ln - 4 * (j0 + k);
                 w[k] = r <= 0 ? 0u : (r < 4 ? w[k] & ((1u << (8 * r)) - 1u) : w[k]);
             }
         }
 #pragma unroll
-        for (int k = 0; k < 4; k++)
+        for (int k = 0; k < 8; k++)
             if (j0 + k < width) words[gb + 64 * (j0 + k) + lane] = w[k];
+        if (bits && 4 * j0 < ln) {
+            /* planes of bits 0-3 of the 32 bytes: bit r of P[b] = bit b of
+             * byte r (transpose8: byte b of y holds bit b of each byte of x) */
+            u32 P[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const u64 y = transpose8(((u64)w[2 * q + 1] << 32) | w[2 * q]);
+#pragma unroll
+                for (int b = 0; b < 4; b++) P[b] |= ((u32)(y >> (8 * b)) & 0xFFu) << (8 * q);
+            }
+            const int64_t rem = ln - 4 * j0;
+            const u32 V = rem >= 32 ? ~0u : (1u << rem) - 1u;      /* bytes of the string */
+            const u32 V2 = j0 == 0 ? V & ~1u : V;                  /* with a byte before */
+            /* class x = b & 15: bits 0-1 of x from A, bits 2-3 from B */
+            const u32 A[4] = {V & ~P[0] & ~P[1], V & P[0] & ~P[1], V & ~P[0] & P[1], V & P[0] & P[1]};
+            const u32 B[4] = {~P[2] & ~P[3], P[2] & ~P[3], ~P[2] & P[3], P[2] & P[3]};
+#pragma unroll
+            for (int x = 0; x < 16; x++) n1[x] += (u32)popc(A[x & 3] & B[x >> 2]);
+            /* 2-gram class 4 c2(previous) + c2(this), c2 = bits 1-2 */
+            const u32 S1 = funnel32(P[1], prev1, 31), S2 = funnel32(P[2], prev2, 31);
+            const u32 C[4] = {V2 & ~P[1] & ~P[2], V2 & P[1] & ~P[2], V2 & ~P[1] & P[2], V2 & P[1] & P[2]};
+            const u32 D[4] = {~S1 & ~S2, S1 & ~S2, ~S1 & S2, S1 & S2};
+#pragma unroll
+            for (int x = 0; x < 16; x++) n2[x] += (u32)popc(C[x & 3] & D[x >> 2]);
+            prev1 = P[1];
+            prev2 = P[2];
+        }
         if (rows && 4 * j0 < ln) {
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
+            for (int k = 0; k < 8; k++) {
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
                     const int64_t at = 4 * (j0 + k) + i;
-                    const u32 b2 = (w[k] >> (8 * i)) & 0xFFu;
+                    const u32 bb = (w[k] >> (8 * i)) & 0xFFu;
                     if (at < ln) {
-                        atomicAdd(&h[H1 + (b2 & 15u)][lane], 1u);
-                        if (at >= 1) atomicAdd(&h[H2 + 4u * class2(b1) + class2(b2)][lane], 1u);
-                        if (counts3 && at >= 2) atomicAdd(&h[H3 + class3(b0, b1, b2)][lane], 1u);
+                        atomicAdd(&h[H1 + (bb & 15u)][lane], 1u);
+                        if (at >= 1) atomicAdd(&h[H2 + 4u * class2(b1) + class2(bb)][lane], 1u);
+                        atomicAdd(&h[H3 + class3(b0, b1, bb)][lane], at >= 2 ? 1u : 0u);
                     }
                     b0 = b1;
-                    b1 = b2;
+                    b1 = bb;
                 }
             }
         }
     }
-    if (c < n && rows) {
-        auto pack16 = [&](int row0) {
+    if (c < n && (bits || rows)) {
+        auto pack16 = [&](auto get) {
             u32 v[4];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 u32 x = 0;
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    const u32 cnt = h[row0 + 4 * q + i][lane];
+                    const u32 cnt = get(4 * q + i);
                     x |= (cnt > 255u ? 255u : cnt) << (8 * i);
                 }
                 v[q] = x;
             }
             return Profile{v[0], v[1], v[2], v[3]};
         };
-        if (counts) counts[c] = pack16(H1);
-        if (counts2) counts2[c] = pack16(H2);
-        if (counts3) counts3[c] = pack_profile3([&](int x) { return h[H3 + x][lane]; });
+        if (bits) {
+            if (counts) counts[c] = pack16([&](int x) { return n1[x]; });
+            if (counts2) counts2[c] = pack16([&](int x) { return n2[x]; });
+        } else {
+            if (counts) counts[c] = pack16([&](int x) { return h[H1 + x][lane]; });
+            if (counts2) counts2[c] = pack16([&](int x) { return h[H2 + x][lane]; });
+            counts3[c] = pack_profile3([&](int x) { return h[H3 + x][lane]; });
+        }
     }
 }
 
@@ -444,7 +488,7 @@ static int stripes_build(const char *who, const uint8_t *d_buf, const int64_t *d
     if (((uintptr_t)d_counts & 15) != 0 || ((uintptr_t)d_counts2 & 15) != 0 ||
         ((uintptr_t)d_counts3 & 15) != 0)
         return fail(BL_EINVAL, "%s: profiles not 16-byte aligned", who);
-    const size_t lds = stripes_build_lds(d_counts || d_counts2, d_counts3 != nullptr);
+    const size_t lds = stripes_build_lds(d_counts3 != nullptr);
     stripes_build_kernel<<<(unsigned)groups, 64, lds, (hipStream_t)stream>>>(
         d_buf, d_off, n, d_order, d_gbase, d_words, d_len, (Profile *)d_counts, (Profile *)d_counts2,
         (Profile3 *)d_counts3);
