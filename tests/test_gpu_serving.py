@@ -16,10 +16,15 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def test_each_result_is_the_same_alone_and_in_the_mix():
+@pytest.mark.parametrize("sort_sets", [True, False])
+def test_each_result_is_the_same_alone_and_in_the_mix(sort_sets):
+    """Both set layouts: stored in length order with the result columns
+    gathered back, and kept in the caller's order (exact scans over stripes
+    sorted within blocks)."""
     rng = np.random.default_rng(11)
     sets = {f"s{i}": W.generate(5, 20000, seed=100 + i) for i in range(2)}
-    srv = serving.Server(slots=4, budget_bytes=2 << 30)
+    srv = serving.Server(slots=4, budget_bytes=2 << 30,
+                         executor=serving._GpuExecutor(4, sort_sets=sort_sets))
     for name, packed in sets.items():
         srv.add_set(name, packed=packed)
     specs = []
